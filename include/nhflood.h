@@ -1,0 +1,189 @@
+/*
+ * nhflood.h - C ABI of the compact local-inertial shallow-water step.
+ *
+ * One library, libnhflood.so, implements this ABI with hand-written HIP kernels for gfx950.
+ * Every array in nhf_mesh is a DEVICE pointer owned by the caller (the Python host allocates
+ * them as torch tensors); the library never frees them and, apart from the nhf_clock_probe test
+ * hook, allocates no device memory of its own: its scratch lives in the caller's workspace
+ * (nhf_workspace_bytes). The CPU oracle under oracle/ (test infrastructure) exports the same
+ * entry points over host arrays so one host driver can run either.
+ *
+ * The reference has no FFI: its device step is a set of Taichi kernels closed over one Python
+ * function (src/nh_flood_2d/core/solver_compact.py:179-1008). Each entry point below replaces
+ * one piece of that Python/Taichi contract; the citation on each says which.
+ *
+ * Conventions kept from the reference:
+ *   - index 0 of every cell and side array is the exterior sentinel (README.md:407);
+ *   - a side's lower/higher cells are bottom/top for horizontal sides (orient 1) and left/right
+ *     for vertical sides (orient 2) (preprocess/domain.py:456-460);
+ *   - a cell's side list holds its left, right, bottom, top faces in that order
+ *     (preprocess/domain.py:83-174);
+ *   - the clock counts 2^-40 s ticks (core/clock.py:1-30), here as one uint64_t;
+ *   - the per-step trace row has 9 uint32 columns (core/clock.py:35-39): dt bits, end tick hi,
+ *     end tick lo, status, CFL candidate bits, event-short flag, limited cells, limited faces,
+ *     minimum donor scale bits.
+ * All functions return 0 on success, a negative value on a HIP or argument error
+ * (nhf_last_error() then describes it).
+ */
+#ifndef NHFLOOD_H
+#define NHFLOOD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NHF_ABI_VERSION 1
+#define NHF_TRACE_COLS 9
+#define NHF_MAX_BATCH 64
+
+/* Clock status codes (core/clock.py:42-46). */
+#define NHF_DONE 0
+#define NHF_COMPLETED 1
+#define NHF_BAD_CFL 2
+#define NHF_BAD_DT 3
+#define NHF_OVERSHOOT 4
+
+/* side_meta bits */
+#define NHF_SIDE_ORIENT_MASK 3u /* 1 = horizontal (bottom/top), 2 = vertical (left/right) */
+#define NHF_SIDE_EXTERIOR 4u    /* sbf == 1: exactly one neighbour is the sentinel */
+
+/* Physics constants, all already rounded to F32 the way the reference's Taichi kernels see
+ * them (solver_compact.py:208 g = 9.81; input/domain.py:34-36 afa, sita, min_h). */
+typedef struct nhf_params {
+    float g;          /* 9.81f */
+    float afa;        /* CFL number */
+    float theta;      /* sita */
+    float theta_half; /* F32 of the F64 value (1 - sita) / 2 (solver_compact.py:461) */
+    float min_h;      /* wet/dry threshold */
+    float min_dt;     /* F32 of 1e-4, the smallest CFL candidate (clock.py:49-57) */
+    int32_t batch_capacity; /* steps per batch the trace can hold, 1..NHF_MAX_BATCH */
+    int32_t reserved;
+} nhf_params;
+
+/* Device arrays of one mesh in the library's layout. Sizes: e_num cells (with sentinel),
+ * s_num sides (with sentinel), r_len side-list entries, b_num boundaries, n_pixels rain pixels,
+ * n_grad gradient cells and n_hang hanging edges (both 0 on aligned meshes). */
+typedef struct nhf_mesh {
+    int32_t e_num, s_num, r_len, b_num;
+    int32_t n_pixels, n_grad, n_hang, flags; /* flags: bit0 pipe_q present */
+
+    /* per-cell state (read-write) */
+    float *depth;      /* water depth h (m) */
+    float *bed;        /* bed elevation z (m) */
+    float *rain_time;  /* Horton cumulative rain time (s) */
+    float *capacity;   /* Horton infiltration capacity (m/s) */
+    float *max_depth;  /* extrema since reset */
+    float *max_speed;
+    uint64_t *peak_tick; /* first tick of max_depth */
+    /* per-cell static */
+    const float *area, *width, *height;
+    const float *f0, *fc, *hk;       /* Horton f0, fc (m/s), k (1/h) */
+    const int32_t *rain_index;        /* cell -> rain pixel */
+    const int32_t *boundary_id;       /* -1 or boundary index */
+    const float *bweight;             /* discharge share of the boundary */
+    const int32_t *csr_start;         /* first entry of the cell's l, r, b, t side list */
+    const uint32_t *csr_count;        /* side counts: l | r << 8 | b << 16 | t << 24 */
+    const float *pipe_q;              /* optional pipe exchange (m^3/s), NULL when absent */
+    const float *open_face;           /* [e_num*4] exterior open fraction, NULL when sita == 1 */
+    const float *x, *y;               /* cell centres, NULL on aligned meshes */
+    const int32_t *isl;               /* [r_len] side ids */
+    /* per-side */
+    float *q;                         /* unit discharge (m^2/s), state */
+    const int32_t *side_cells;        /* [s_num*2] lower, higher */
+    const uint8_t *side_meta;         /* orient | exterior flag */
+    const float *length, *sdc, *manning;
+    const uint32_t *side_key;         /* reference side id, smallest wins CFL ties */
+    /* hanging-edge pressure reconstruction (core/pressure.py:19-45), NULL on aligned meshes */
+    const int32_t *edge_index;        /* [s_num] -1 or hanging edge */
+    const float *hang_offset;         /* [n_hang*2] tangential offsets of lower, higher centre */
+    const int32_t *hang_slot;         /* [n_hang*2] gradient slot of lower, higher or -1 */
+    const int32_t *grad_cell;         /* [n_grad] */
+    const int32_t *grad_axes;         /* [n_grad] bit0 x-gradient needed, bit1 y-gradient */
+    /* per-boundary */
+    const int32_t *kind;              /* [b_num] 0 level, 1 discharge */
+} nhf_mesh;
+
+typedef struct nhf_ctx nhf_ctx;
+
+/* Version of this ABI; the loader refuses a library that reports another. */
+int32_t nhf_abi_version(void);
+
+/* Thread-local text of the last error. */
+const char *nhf_last_error(void);
+
+/* Bytes of scratch the library needs for this mesh (control block, budget accumulators, CFL
+ * partials, face averages, gradients, donor scales, unscaled discharge). */
+size_t nhf_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params);
+
+/* Bind a mesh, parameters, a caller-owned device workspace and a HIP stream (hipStream_t, may be
+ * NULL for the null stream). Replaces the field allocation and kernel definitions of _evolve
+ * (solver_compact.py:204-687). */
+int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params,
+                   void *workspace, size_t workspace_bytes, void *stream);
+int32_t nhf_destroy(nhf_ctx *ctx);
+
+/* Set the device clock (StepClock.__init__ now words, clock.py:58-69). Also zeroes the budget
+ * accumulators, donor scales and the CFL limit capture. */
+int32_t nhf_reset(nhf_ctx *ctx, uint64_t now_ticks);
+
+/* Upload the forcing of a batch (upload_forcing, solver_compact.py:769-790): coefficients
+ * [b_num*2] (value at batch start, slope per second) and, when rain_rates != NULL, the rain rate
+ * of every pixel (m/s). Host pointers. */
+int32_t nhf_set_forcing(nhf_ctx *ctx, const float *coefficients, const float *rain_rates,
+                        int32_t raining);
+
+/* Start a batch (StepClock.begin, clock.py:75-92): target and origin ticks, trace cleared. */
+int32_t nhf_begin(nhf_ctx *ctx, uint64_t target_ticks, uint64_t origin_ticks);
+
+/* Run `steps` slots of [prepare_step, (update_pressure), tick, compute_cfl, finish_step]
+ * (the graph of solver_compact.py:894-909, run by graph.run at :941) and copy the
+ * steps x 9 trace to `trace` (host). Slots after the target, or after a failure status, do no
+ * physics (clock.py:94-167). */
+int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace);
+
+/* Standalone compute_cfl on the current state (solver_compact.py:617-641, called once at
+ * :889 before the first batch). */
+int32_t nhf_compute_cfl(nhf_ctx *ctx);
+
+/* Interval water-budget totals (DeviceBudget.flush, device_budget.py:68-84) as double-double
+ * pairs: out[2*t] + out[2*t+1] for each of the 4 + 6*b_num terms (solver_compact.py:393-396,
+ * 533, 578-590); the accumulators are cleared. Returns 1 (not an error code) if a term is
+ * non-finite, after clearing nothing. */
+int32_t nhf_flush_budget(nhf_ctx *ctx, double *out);
+
+/* CFL-limit capture of prepare_step (solver_compact.py:374-382): values = candidate, lower depth,
+ * higher depth, face depth, face velocity; ids = side, lower cell, higher cell. `reset` puts the
+ * captured candidate back to +inf (solver_compact.py:854). */
+int32_t nhf_limit_capture(nhf_ctx *ctx, float *values, int32_t *ids, int32_t reset);
+
+/* Last CFL candidate and its side (ndt_t, cfl_arg_t; used by supported_dt, :644-655). */
+int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side);
+
+/* Wait for all work of the context's stream. */
+int32_t nhf_sync(nhf_ctx *ctx);
+
+/* Launch mode: 1 replays each batch as one captured hipGraph, 0 launches kernels one by one. */
+int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable);
+
+/* Per-kernel timing with HIP events on the context's stream (eager mode only). Kernel ids:
+ * 0 prepare, 1 gradients, 2 face averages, 3 flux, 4 donor, 5 depth, 6 cfl. `ms` and `count`
+ * receive totals since the last call with reset != 0. */
+int32_t nhf_set_timing(nhf_ctx *ctx, int32_t enable);
+int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);
+
+/* Test hook with no reference entry point: evaluate StepClock.choose (clock.py:94-167) for n
+ * independent cases on the device. Inputs and outputs are host arrays. out_words[i*4 + 0..3]
+ * = status, dt bits, CFL bits, event-short flag; out_ticks[i*2 + 0..1] = step end tick and the
+ * tick distance of the F32 dt; out_times[i*2 + 0..1] = local, end_local. */
+int32_t nhf_clock_probe(const uint64_t *now, const uint64_t *target, const uint64_t *origin,
+                        const float *candidate, float min_dt, int32_t n, uint32_t *out_words,
+                        uint64_t *out_ticks, float *out_times);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NHFLOOD_H */

@@ -1,0 +1,104 @@
+"""ctypes mirror of include/nhflood.h and the loader of the HIP library."""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+ABI_VERSION = 1
+TRACE_COLS = 9
+MAX_BATCH = 64
+DONE, COMPLETED, BAD_CFL, BAD_DT, OVERSHOOT = range(5)
+SIDE_EXTERIOR = 4
+
+PACKAGE_DIR = Path(__file__).resolve().parent
+HIP_LIBRARY = PACKAGE_DIR / 'libnhflood.so'
+
+
+class Params(ctypes.Structure):
+    _fields_ = [('g', c_float), ('afa', c_float), ('theta', c_float), ('theta_half', c_float),
+                ('min_h', c_float), ('min_dt', c_float), ('batch_capacity', c_int32),
+                ('reserved', c_int32)]
+
+
+# (field name, kind) in header order; kind 'i' = int32 scalar, else an array pointer
+MESH_FIELDS = [
+    ('e_num', 'i'), ('s_num', 'i'), ('r_len', 'i'), ('b_num', 'i'),
+    ('n_pixels', 'i'), ('n_grad', 'i'), ('n_hang', 'i'), ('flags', 'i'),
+    ('depth', 'p'), ('bed', 'p'), ('rain_time', 'p'), ('capacity', 'p'),
+    ('max_depth', 'p'), ('max_speed', 'p'), ('peak_tick', 'p'),
+    ('area', 'p'), ('width', 'p'), ('height', 'p'),
+    ('f0', 'p'), ('fc', 'p'), ('hk', 'p'),
+    ('rain_index', 'p'), ('boundary_id', 'p'), ('bweight', 'p'),
+    ('csr_start', 'p'), ('csr_count', 'p'), ('pipe_q', 'p'), ('open_face', 'p'),
+    ('x', 'p'), ('y', 'p'), ('isl', 'p'),
+    ('q', 'p'), ('side_cells', 'p'), ('side_meta', 'p'),
+    ('length', 'p'), ('sdc', 'p'), ('manning', 'p'), ('side_key', 'p'),
+    ('edge_index', 'p'), ('hang_offset', 'p'), ('hang_slot', 'p'),
+    ('grad_cell', 'p'), ('grad_axes', 'p'),
+    ('kind', 'p'),
+]
+
+
+class MeshStruct(ctypes.Structure):
+    _fields_ = [(name, c_int32 if kind == 'i' else c_void_p) for name, kind in MESH_FIELDS]
+
+
+def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
+    """Declare every entry point of include/nhflood.h on a loaded library."""
+    vp = c_void_p
+    sig = {
+        'nhf_abi_version': (c_int32, []),
+        'nhf_last_error': (ctypes.c_char_p, []),
+        'nhf_workspace_bytes': (c_size_t, [POINTER(MeshStruct), POINTER(Params)]),
+        'nhf_create': (c_int32, [POINTER(vp), POINTER(MeshStruct), POINTER(Params), vp, c_size_t, vp]),
+        'nhf_destroy': (c_int32, [vp]),
+        'nhf_reset': (c_int32, [vp, c_uint64]),
+        'nhf_set_forcing': (c_int32, [vp, vp, vp, c_int32]),
+        'nhf_begin': (c_int32, [vp, c_uint64, c_uint64]),
+        'nhf_run_batch': (c_int32, [vp, c_int32, vp]),
+        'nhf_compute_cfl': (c_int32, [vp]),
+        'nhf_flush_budget': (c_int32, [vp, vp]),
+        'nhf_limit_capture': (c_int32, [vp, vp, vp, c_int32]),
+        'nhf_cfl_state': (c_int32, [vp, vp, vp]),
+        'nhf_sync': (c_int32, [vp]),
+        'nhf_set_graph': (c_int32, [vp, c_int32]),
+        'nhf_set_timing': (c_int32, [vp, c_int32]),
+        'nhf_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
+        'nhf_clock_probe': (c_int32, [vp, vp, vp, vp, c_float, c_int32, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.nhf_abi_version() != ABI_VERSION:
+        raise RuntimeError(f'{lib._name}: ABI version {lib.nhf_abi_version()} != {ABI_VERSION}')
+    return lib
+
+
+EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_create',
+            'nhf_destroy', 'nhf_reset', 'nhf_set_forcing', 'nhf_begin', 'nhf_run_batch',
+            'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
+            'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe']
+
+_hip_lib = None
+
+
+def load_hip_library() -> ctypes.CDLL:
+    """Load libnhflood.so (the gfx950 kernels). There is no fallback: a missing build raises."""
+    global _hip_lib
+    if _hip_lib is None:
+        if not HIP_LIBRARY.exists():
+            raise RuntimeError(f'{HIP_LIBRARY} is not built; run __graft_entry__.build() first')
+        _hip_lib = bind(ctypes.CDLL(str(HIP_LIBRARY)))
+    return _hip_lib
+
+
+def check(lib, rc: int, what: str) -> None:
+    if rc < 0:
+        raise RuntimeError(f'{what} failed ({rc}): {lib.nhf_last_error().decode(errors="replace")}')
+
+
+__all__ = ['Params', 'MeshStruct', 'MESH_FIELDS', 'bind', 'load_hip_library', 'check',
+           'TRACE_COLS', 'MAX_BATCH', 'DONE', 'COMPLETED', 'BAD_CFL', 'BAD_DT', 'OVERSHOOT',
+           'c_double', 'c_int64', 'c_uint32']

@@ -1,0 +1,1176 @@
+// libnhflood.so: the compact local-inertial step as hand-written HIP kernels for gfx950.
+//
+// One time step is five launches on the context's stream (plus two optional ones):
+//   k_gradients   hanging-edge tangential water-level gradients   (core/pressure.py:108-116)
+//   k_face_avg    theta < 1 face averages of the previous q         (solver_compact.py:430-436)
+//   k_flux        face flux: theta-weighted discharge, semi-implicit Manning friction, min_h
+//                 gating, exterior zeroing, hanging pressure correction (solver_compact.py:439-486)
+//   k_donor       positivity limiter: outgoing vs available volume   (solver_compact.py:491-514)
+//   k_depth       shared-subedge scaling fused with sources, Horton capacity, face totals,
+//                 continuity, infiltration removal, positivity floor, level cells, extrema and
+//                 the per-block FP64 water budget                      (solver_compact.py:516-613)
+//   k_cfl         CFL bound per interior side, wave-shuffle + LDS min/argmin, then the last
+//                 workgroup reduces the partials, commits the clock and prepares the next slot
+//                 (solver_compact.py:617-641, 364-391; core/clock.py:94-181)
+// The clock, dt and every reduction stay on the device; the host reads one trace per batch.
+//
+// Layout: structure of arrays, cells and sides renumbered along a Morton curve by the host so a
+// wavefront's neighbour gathers stay within a few cache lines. Cell 0 / side 0 are sentinels.
+// The path is HBM-bandwidth bound (about 0.2 flop/byte); no MFMA.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "nhflood.h"
+
+#define NHF_BLOCK 256
+#define NHF_WAVES (NHF_BLOCK / 64)
+#define TICKS_PER_SECOND_F 1099511627776.0f  // 2^40 exactly representable in F32
+#define SNAP_TICKS 1099512ull                 // ceil(1e-6 s * 2^40)
+
+static thread_local char g_error[512];
+
+static int32_t set_error(const char *what, hipError_t err) {
+    snprintf(g_error, sizeof g_error, "%s: %s", what, hipGetErrorString(err));
+    return -1;
+}
+
+static int32_t set_error_text(const char *what) {
+    snprintf(g_error, sizeof g_error, "%s", what);
+    return -2;
+}
+
+#define HIP_TRY(expr)                                      \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return set_error(#expr, e_); \
+    } while (0)
+
+// --------------------------------------------------------------------------- device state
+
+struct Ctl {
+    uint64_t now, next, origin, step_end;
+    float dt, local, end_local, ndt;
+    int32_t active, cfl_arg, raining, slot;
+    uint32_t ticket, limited_cells, limited_faces, min_scale_bits;
+    float limit_value[5];
+    int32_t limit_ids[3];
+    uint32_t trace[NHF_MAX_BATCH * NHF_TRACE_COLS];
+};
+
+struct Dev {
+    nhf_mesh m;
+    nhf_params p;
+    Ctl *ctl;
+    float *coefficients, *values, *level_values, *rain_rates;
+    float *donor, *qflux, *face_qn, *grad;
+    uint64_t *cfl_key;
+    int32_t *cfl_idx;
+    double2 *acc;  // per depth-block, per term: (sum, compensation)
+    int32_t terms, cfl_blocks, depth_blocks, theta_one;
+};
+
+// Comparisons written as selects so +0/-0 and NaN behave the same in every pass.
+__device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
+
+__device__ __forceinline__ int face_n(uint32_t counts, int d) { return (int)((counts >> (8 * d)) & 255u); }
+
+// --------------------------------------------------------------------------- x^y and e^x
+// Evaluated in FP64 and rounded once to F32, so the result is the same on any IEEE machine:
+// log2 by range reduction to m in [sqrt(1/2), sqrt(2)) and the odd series of
+// ln m = 2 atanh((m-1)/(m+1)); 2^t by t = n + f, |f| <= 1/2, and the Taylor series of
+// e^(f ln 2) (J.-M. Muller, Elementary Functions, ch. 5). Only IEEE add, multiply, divide and
+// explicit fma are used.
+__device__ __forceinline__ double det_log2(double x) {
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double p = 1.0 / 25.0;
+    p = fma(p, s2, 1.0 / 23.0);
+    p = fma(p, s2, 1.0 / 21.0);
+    p = fma(p, s2, 1.0 / 19.0);
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    p = fma(p, s2, 1.0);
+    double ln_m = 2.0 * s * p;
+    return fma(ln_m, 1.44269504088896340736, (double)e);
+}
+
+__device__ __forceinline__ double det_exp2(double t) {
+    if (t < -1100.0) return 0.0;
+    if (t > 1100.0) return INFINITY;
+    double n = rint(t);
+    double u = (t - n) * 0.69314718055994530942;
+    double p = 1.0 / 87178291200.0;
+    p = fma(p, u, 1.0 / 6227020800.0);
+    p = fma(p, u, 1.0 / 479001600.0);
+    p = fma(p, u, 1.0 / 39916800.0);
+    p = fma(p, u, 1.0 / 3628800.0);
+    p = fma(p, u, 1.0 / 362880.0);
+    p = fma(p, u, 1.0 / 40320.0);
+    p = fma(p, u, 1.0 / 5040.0);
+    p = fma(p, u, 1.0 / 720.0);
+    p = fma(p, u, 1.0 / 120.0);
+    p = fma(p, u, 1.0 / 24.0);
+    p = fma(p, u, 1.0 / 6.0);
+    p = fma(p, u, 0.5);
+    p = fma(p, u, 1.0);
+    p = fma(p, u, 1.0);
+    return ldexp(p, (int)n);
+}
+
+__device__ __forceinline__ float det_powf(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+    if (x == 1.0f) return 1.0f;
+    return (float)det_exp2((double)y * det_log2((double)x));
+}
+
+__device__ __forceinline__ float det_expf(float x) { return (float)det_exp2((double)x * 1.44269504088896340736); }
+
+// --------------------------------------------------------------------------- block reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// (key, index) minimum: the key orders by CFL bits, then by the reference side id.
+__device__ __forceinline__ void wave_min_key(uint64_t &key, int32_t &idx) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t k2 = __shfl_xor((unsigned long long)key, o, 64);
+        int32_t i2 = __shfl_xor(idx, o, 64);
+        if (k2 < key) {
+            key = k2;
+            idx = i2;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- clock
+
+__device__ __forceinline__ float ticks_local(uint64_t ticks) {
+    uint32_t hi = (uint32_t)(ticks >> 32), lo = (uint32_t)ticks;
+    return (float)hi * (1.0f / 256.0f) + (float)lo * (1.0f / TICKS_PER_SECOND_F);
+}
+
+// Exact F32 seconds -> 2^-40 s ticks, half-even below one tick; 0 when not representable.
+__device__ __forceinline__ uint64_t f32_ticks(float v) {
+    uint32_t bits = __float_as_uint(v);
+    int e = (int)((bits >> 23) & 255u) - 127;
+    if (!(v > 0.0f) || e < -40 || e > 0) return 0;
+    uint64_t mant = (uint64_t)((bits & 0x7fffffu) | 0x800000u);
+    int shift = e + 17;
+    if (shift >= 0) return mant << shift;
+    int right = -shift;
+    uint64_t t = mant >> right;
+    uint64_t dropped = mant & ((1ull << right) - 1ull);
+    uint64_t half = 1ull << (right - 1);
+    if (dropped > half || (dropped == half && (t & 1ull))) t += 1;
+    return t;
+}
+
+// StepClock.choose (core/clock.py:94-167) with u64 tick arithmetic. Writes the trace row's
+// status / candidate / event-short columns and the step's dt, local times and end tick.
+__device__ void clock_choose(Ctl *c, float candidate, float min_dt, int slot) {
+    uint32_t *row = &c->trace[slot * NHF_TRACE_COLS];
+    c->active = 0;
+    c->dt = 0.0f;
+    c->local = 0.0f;
+    c->end_local = 0.0f;
+    uint64_t now = c->now, target = c->next;
+    if (!(now < target)) return;
+    if (!(candidate >= min_dt)) {
+        row[3] = NHF_BAD_CFL;
+        return;
+    }
+    uint64_t rem = target - now;
+    float remaining = ticks_local(rem);
+    float capped = fmn(candidate, 1.0f);
+    float chosen = fmn(capped, remaining);
+    row[4] = __float_as_uint(candidate);
+    row[5] = chosen < capped ? 1u : 0u;
+    uint64_t dt_ticks = f32_ticks(chosen);
+    if (dt_ticks == 0) {
+        row[3] = NHF_BAD_DT;
+        return;
+    }
+    bool exceeds = dt_ticks > rem;
+    uint64_t diff = exceeds ? dt_ticks - rem : rem - dt_ticks;
+    uint64_t end;
+    if (diff < SNAP_TICKS) {
+        end = target;
+    } else if (exceeds) {
+        row[3] = NHF_OVERSHOOT;
+        return;
+    } else {
+        end = now + dt_ticks;
+    }
+    c->local = ticks_local(now - c->origin);
+    c->end_local = ticks_local(end - c->origin);
+    c->dt = chosen;
+    c->step_end = end;
+    c->active = 1;
+}
+
+// prepare_step (solver_compact.py:364-382), run by one thread.
+__device__ void prepare_slot(const Dev &d, int slot) {
+    Ctl *c = d.ctl;
+    clock_choose(c, c->ndt, d.p.min_dt, slot);
+    if (!c->active) return;
+    c->limited_cells = 0;
+    c->limited_faces = 0;
+    c->min_scale_bits = __float_as_uint(1.0f);
+    const float dt = c->dt, local = c->local, end_local = c->end_local;
+    for (int bi = 0; bi < d.m.b_num; ++bi) {
+        float c0 = d.coefficients[2 * bi], c1 = d.coefficients[2 * bi + 1];
+        d.values[bi] = c0 + c1 * (local + dt / 2.0f);
+        d.level_values[bi] = c0 + c1 * end_local;
+    }
+    if (c->ndt < c->limit_value[0]) {
+        int si = c->cfl_arg;
+        int el = d.m.side_cells[2 * si], eh = d.m.side_cells[2 * si + 1];
+        float zl = d.m.bed[el], zh = d.m.bed[eh], hl = d.m.depth[el], hh = d.m.depth[eh];
+        float head = fmx(zl + hl, zh + hh);
+        float h = fmx(head - fmx(fmx(zl, zh), -999.0f), 0.01f);
+        c->limit_value[0] = c->ndt;
+        c->limit_value[1] = hl;
+        c->limit_value[2] = hh;
+        c->limit_value[3] = h;
+        c->limit_value[4] = d.m.q[si] / fmx(h, 1e-30f);
+        c->limit_ids[0] = si;
+        c->limit_ids[1] = el;
+        c->limit_ids[2] = eh;
+    }
+}
+
+// finish_step + StepClock.commit (solver_compact.py:384-391, core/clock.py:173-181).
+__device__ void commit_slot(Ctl *c, int slot) {
+    if (!c->active) return;
+    uint32_t *row = &c->trace[slot * NHF_TRACE_COLS];
+    row[6] = c->limited_cells;
+    row[7] = c->limited_faces;
+    row[8] = c->min_scale_bits;
+    c->now = c->step_end;
+    row[0] = __float_as_uint(c->dt);
+    row[1] = (uint32_t)(c->step_end >> 32);
+    row[2] = (uint32_t)c->step_end;
+    row[3] = NHF_COMPLETED;
+}
+
+// --------------------------------------------------------------------------- small kernels
+
+__global__ void k_reset(Dev d, uint64_t now) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int stride = gridDim.x * blockDim.x;
+    if (tid == 0) {
+        Ctl *c = d.ctl;
+        c->now = c->next = c->origin = c->step_end = now;
+        c->dt = c->local = c->end_local = 0.0f;
+        c->active = 0;
+        c->slot = 0;
+        c->ticket = 0;
+        c->limited_cells = c->limited_faces = 0;
+        c->min_scale_bits = __float_as_uint(1.0f);
+        c->limit_value[0] = INFINITY;
+    }
+    for (int i = tid; i < d.depth_blocks * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i < d.m.e_num; i += stride) d.donor[i] = 0.0f;
+    for (int i = tid; i < d.m.s_num; i += stride) d.qflux[i] = 0.0f;
+    if (d.face_qn)
+        for (int i = tid; i < d.m.e_num * 4; i += stride) d.face_qn[i] = 0.0f;
+}
+
+__global__ void k_begin(Dev d, uint64_t target, uint64_t origin) {
+    Ctl *c = d.ctl;
+    for (int i = threadIdx.x; i < NHF_MAX_BATCH * NHF_TRACE_COLS; i += blockDim.x) c->trace[i] = 0u;
+    if (threadIdx.x == 0) {
+        c->next = target;
+        c->origin = origin;
+        c->slot = 0;
+    }
+}
+
+__global__ void k_set_raining(Dev d, int raining) {
+    if (threadIdx.x == 0) d.ctl->raining = raining;
+}
+
+__global__ void k_prepare(Dev d, int slot) {
+    if (threadIdx.x == 0) prepare_slot(d, slot);
+}
+
+__global__ void k_clear_acc(Dev d) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int i = tid; i < d.depth_blocks * d.terms; i += gridDim.x * blockDim.x)
+        d.acc[i] = make_double2(0.0, 0.0);
+}
+
+__global__ void k_reset_limit(Dev d) {
+    if (threadIdx.x == 0) d.ctl->limit_value[0] = INFINITY;
+}
+
+__global__ void k_clock_probe(const uint64_t *now, const uint64_t *target, const uint64_t *origin,
+                              const float *candidate, float min_dt, int n, uint32_t *words,
+                              uint64_t *ticks, float *times, Ctl *scratch) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ctl *c = &scratch[i];
+    for (int k = 0; k < NHF_TRACE_COLS; ++k) c->trace[k] = 0u;
+    c->now = now[i];
+    c->next = target[i];
+    c->origin = origin[i];
+    c->step_end = 0;
+    clock_choose(c, candidate[i], min_dt, 0);
+    words[4 * i + 0] = c->active ? NHF_COMPLETED : c->trace[3];
+    words[4 * i + 1] = __float_as_uint(c->dt);
+    words[4 * i + 2] = c->trace[4];
+    words[4 * i + 3] = c->trace[5];
+    ticks[2 * i + 0] = c->active ? c->step_end : 0ull;
+    ticks[2 * i + 1] = c->active ? f32_ticks(c->dt) : 0ull;
+    times[2 * i + 0] = c->local;
+    times[2 * i + 1] = c->end_local;
+}
+
+// --------------------------------------------------------------------------- hanging gradients
+
+// Length-weighted mean water-level difference and centre distance over one face's wet
+// neighbours (PressureReconstruction.face_difference, core/pressure.py:66-90).
+__device__ void face_difference(const Dev &d, int cell, int dir, int axis, float &value,
+                                float &distance, int &usable) {
+    const nhf_mesh &m = d.m;
+    float v = 0.0f, dist = 0.0f, total = 0.0f;
+    int ok = 1;
+    float eta = m.bed[cell] + m.depth[cell];
+    uint32_t cnt = m.csr_count[cell];
+    int start = m.csr_start[cell];
+    for (int k = 0; k < dir; ++k) start += face_n(cnt, k);
+    int n = face_n(cnt, dir);
+    for (int j = 0; j < n; ++j) {
+        int side = m.isl[start + j];
+        int other = m.side_cells[2 * side] + m.side_cells[2 * side + 1] - cell;
+        if (other <= 0 || m.depth[other] <= d.p.min_h) {
+            ok = 0;
+        } else {
+            float w = m.length[side];
+            v += w * ((m.bed[other] + m.depth[other]) - eta);
+            float delta = axis == 1 ? m.y[other] - m.y[cell] : m.x[other] - m.x[cell];
+            dist += w * delta;
+            total += w;
+        }
+    }
+    if (total > 0.0f) {
+        v /= total;
+        dist /= total;
+    } else {
+        ok = 0;
+    }
+    value = v;
+    distance = dist;
+    usable = ok;
+}
+
+__global__ __launch_bounds__(NHF_BLOCK) void k_gradients(Dev d) {
+    if (!d.ctl->active) return;
+    int slot = blockIdx.x * NHF_BLOCK + threadIdx.x;
+    if (slot >= d.m.n_grad) return;
+    int cell = d.m.grad_cell[slot];
+    int axes = d.m.grad_axes[slot];
+    float h = d.m.depth[cell];
+    for (int axis = 0; axis < 2; ++axis) {
+        if (!(axes & (1 << axis))) continue;
+        float value = 0.0f;
+        if (h > d.p.min_h) {
+            float dm, xm, dp, xp;
+            int okm, okp;
+            face_difference(d, cell, axis == 1 ? 2 : 0, axis, dm, xm, okm);
+            face_difference(d, cell, axis == 1 ? 3 : 1, axis, dp, xp, okp);
+            if (okm && okp)
+                value = (dp - dm) / (xp - xm);
+            else if (okm)
+                value = dm / xm;
+            else if (okp)
+                value = dp / xp;
+        }
+        d.grad[2 * slot + axis] = value;
+    }
+}
+
+// --------------------------------------------------------------------------- theta < 1
+
+__global__ __launch_bounds__(NHF_BLOCK) void k_face_avg(Dev d) {
+    if (!d.ctl->active) return;
+    int cell = blockIdx.x * NHF_BLOCK + threadIdx.x;
+    if (cell >= d.m.e_num) return;
+    const nhf_mesh &m = d.m;
+    uint32_t cnt = m.csr_count[cell];
+    int start = m.csr_start[cell];
+    for (int dir = 0; dir < 4; ++dir) {
+        int n = face_n(cnt, dir);
+        float result = 0.0f;
+        if (n == 1) {
+            result = m.q[m.isl[start]];
+        } else if (n > 1) {
+            float total = 0.0f, width = 0.0f;
+            for (int j = 0; j < n; ++j) {
+                int side = m.isl[start + j];
+                total += m.q[side] * m.length[side];
+                width += m.length[side];
+            }
+            result = total / width;
+        }
+        d.face_qn[4 * cell + dir] = result;
+        start += n;
+    }
+}
+
+// Linear continuation of missing theta neighbours at prescribed faces
+// (boundary_neighbor_sum, solver_compact.py:135-149).
+__device__ __forceinline__ float neighbour_sum(float q, float left, float right, float fl, float fr) {
+    float result = left + right;
+    if (fl > 0.0f || fr > 0.0f) {
+        if (fl == 1.0f || fr == 1.0f) {
+            result = 2.0f * q;
+        } else {
+            float product = fl * fr;
+            result = ((1.0f - fr) * left + (1.0f - fl) * right + 2.0f * q * (fl + fr - 2.0f * product)) /
+                     (1.0f - product);
+        }
+    }
+    return result;
+}
+
+// --------------------------------------------------------------------------- (a) face flux
+
+__global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
+    if (!d.ctl->active) return;
+    int si = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
+    if (si >= d.m.s_num) return;
+    const nhf_mesh &m = d.m;
+    const float dt = d.ctl->dt;
+    const float g = d.p.g;
+    int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
+    uint32_t meta = m.side_meta[si];
+    float bf = (meta & NHF_SIDE_EXTERIOR) ? 0.0f : 1.0f;
+    float zl = m.bed[lh.x], zh = m.bed[lh.y];
+    float hl = zl + m.depth[lh.x], hr = zh + m.depth[lh.y];
+    float q = m.q[si];
+    float wh = fmx(hl, hr) - fmx(fmx(zl, zh), -999.0f);
+    float difference = hr - hl;
+    if (m.edge_index) {
+        int idx = m.edge_index[si];
+        if (idx >= 0 && m.depth[lh.x] > d.p.min_h && m.depth[lh.y] > d.p.min_h) {
+            int axis = (meta & NHF_SIDE_ORIENT_MASK) == 2 ? 1 : 0;
+            float o0 = m.hang_offset[2 * idx], o1 = m.hang_offset[2 * idx + 1];
+            if (o0 != 0.0f) difference -= d.grad[2 * m.hang_slot[2 * idx] + axis] * o0;
+            if (o1 != 0.0f) difference += d.grad[2 * m.hang_slot[2 * idx + 1] + axis] * o1;
+        }
+    }
+    float dq = ((-g * difference) / m.sdc[si]) * fmx(wh, 0.0f) * dt;
+    float n = m.manning[si];
+    // q == 0 gives f == 1 exactly for any finite positive power, so the power is skipped there.
+    float f = 1.0f;
+    if (q != 0.0f) f = 1.0f + g * dt * (n * n) * fabsf(q / det_powf(fmx(wh, 0.00001f), 7.0f / 3.0f));
+    float neighbours = 0.0f;
+    if (!d.theta_one) {
+        int vertical = (meta & NHF_SIDE_ORIENT_MASK) == 2;
+        int fa = vertical ? 0 : 2, fb = vertical ? 1 : 3;
+        neighbours = neighbour_sum(q, d.face_qn[4 * lh.x + fa], d.face_qn[4 * lh.y + fb],
+                                   m.open_face[4 * lh.x + fa], m.open_face[4 * lh.y + fb]);
+    }
+    float nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / f;
+    nq = wh < d.p.min_h ? 0.0f : nq;
+    nq *= bf;
+    d.qflux[si] = nq;
+}
+
+// --------------------------------------------------------------------------- donor scale
+
+__global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
+    Ctl *c = d.ctl;
+    if (!c->active) return;
+    __shared__ uint32_t s_count[NHF_WAVES], s_min[NHF_WAVES];
+    const nhf_mesh &m = d.m;
+    const float dt = c->dt;
+    int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
+    uint32_t limited = 0, scale_bits = __float_as_uint(1.0f);
+    if (cell < m.e_num) {
+        int bi = m.boundary_id[cell];
+        float discharge = 0.0f;
+        bool level = false;
+        if (bi >= 0) {
+            int kind = m.kind[bi];
+            if (kind == 1) discharge = d.values[bi] * m.bweight[cell];
+            level = kind == 0;
+        }
+        float rainq = c->raining ? d.rain_rates[m.rain_index[cell]] : 0.0f;
+        float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+        float area = m.area[cell];
+        float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
+        float available = m.depth[cell] * area + positive + rainq * dt * area;
+        float outgoing = (fmx(-pq, 0.0f) + fmx(-discharge, 0.0f)) * dt;
+        uint32_t cnt = m.csr_count[cell];
+        int k = m.csr_start[cell];
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                float sq = d.qflux[side] * m.length[side];
+                if (dir == 0 || dir == 2) sq = -sq;
+                outgoing += fmx(sq, 0.0f) * dt;
+            }
+        }
+        float ds = 1.0f;
+        if (outgoing > available && !level) {
+            ds = available / outgoing;
+            limited = 1;
+            scale_bits = __float_as_uint(ds);
+        }
+        d.donor[cell] = ds;
+    }
+    uint32_t wc = wave_sum_u(limited), wm = wave_min_u(scale_bits);
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_count[wave] = wc;
+        s_min[wave] = wm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tc = 0, tm = __float_as_uint(1.0f);
+        for (int w = 0; w < NHF_WAVES; ++w) {
+            tc += s_count[w];
+            tm = s_min[w] < tm ? s_min[w] : tm;
+        }
+        if (tc) {
+            atomicAdd(&c->limited_cells, tc);
+            atomicMin(&c->min_scale_bits, tm);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- (b) depth update
+
+// Neumaier step on a (sum, compensation) pair.
+__device__ __forceinline__ void acc_add(double2 &a, double v) {
+    double s = a.x + v;
+    if (fabs(a.x) >= fabs(v))
+        a.y += (a.x - s) + v;
+    else
+        a.y += (v - s) + a.x;
+    a.x = s;
+}
+
+__global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
+    Ctl *c = d.ctl;
+    if (!c->active) return;
+    __shared__ double s_red[NHF_WAVES];
+    __shared__ uint32_t s_faces[NHF_WAVES];
+    const nhf_mesh &m = d.m;
+    const float dt = c->dt;
+    const int b_num = m.b_num;
+    int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
+    bool live = cell < m.e_num;
+    // per-cell budget contributions (solver_compact.py:533, 578-590)
+    float t_infil = 0.0f, t_floor = 0.0f, t_level_floor = 0.0f, t_pipe = 0.0f, t_disc = 0.0f;
+    float t_lv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int bi = -1, kind = -1;
+    uint32_t faces_limited = 0;
+    if (live) {
+        bi = m.boundary_id[cell];
+        if (bi >= 0) kind = m.kind[bi];
+        float ds = d.donor[cell];
+        float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+        float pipe = fmx(pq, 0.0f) + fmn(pq, 0.0f) * ds;
+        float discharge = 0.0f;
+        if (kind == 1) {
+            float requested = d.values[bi] * m.bweight[cell];
+            discharge = fmx(requested, 0.0f) + fmn(requested, 0.0f) * ds;
+            t_disc = discharge * dt;
+        }
+        float ssq = pipe + discharge;
+        float rainq = c->raining ? d.rain_rates[m.rain_index[cell]] : 0.0f;
+        float cap;
+        if (rainq > 0.0f) {
+            float rt = m.rain_time[cell] + dt;
+            m.rain_time[cell] = rt;
+            float f0 = m.f0[cell], fc = m.fc[cell];
+            cap = fc + (f0 - fc) * det_expf(-m.hk[cell] * rt / 3600.0f);
+            m.capacity[cell] = cap;
+        } else {
+            cap = m.capacity[cell];
+        }
+        // face totals of the donor-scaled discharge; the owner writes the scaled side value
+        float fsum[4];
+        uint32_t cnt = m.csr_count[cell];
+        int k = m.csr_start[cell];
+#pragma unroll
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            float acc = 0.0f;
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                float qf = d.qflux[side];
+                int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[side];
+                int donor_cell = qf >= 0.0f ? lh.x : lh.y;
+                float scale = d.donor[donor_cell];
+                float qs = qf * scale;
+                acc += qs * m.length[side];
+                int owner = lh.x != 0 ? lh.x : lh.y;
+                if (owner == cell) {
+                    m.q[side] = qs;
+                    if (qf != 0.0f && scale < 1.0f) faces_limited += 1;
+                }
+            }
+            fsum[dir] = acc;
+        }
+        float ql = fsum[0], qr = fsum[1], qb = fsum[2], qt = fsum[3];
+        float tq = ql - qr + qb - qt + ssq;
+        float area = m.area[cell];
+        float rain_depth = rainq * dt;
+        float increment = tq * dt / area + rain_depth;
+        float previous = m.depth[cell];
+        float raw = previous + increment;
+        float removed = fmn(cap * dt, fmx(raw, 0.0f));
+        raw -= removed;
+        float floor_v = fmx(-raw, 0.0f);
+        t_infil = removed * area;
+        t_pipe = pipe * dt;
+        float h_new;
+        if (kind == 0) {
+            t_level_floor = floor_v * area;
+            float target = fmx(d.level_values[bi] - m.bed[cell], 0.0f);
+            t_lv[0] = (target - previous) * area;
+            t_lv[1] = -(ql - qr + qb - qt) * dt;
+            t_lv[2] = -pipe * dt;
+            t_lv[3] = -rain_depth * area;
+            t_lv[4] = removed * area;
+            h_new = target;
+        } else {
+            t_floor = floor_v * area;
+            h_new = fmx(raw, 0.0f);
+        }
+        m.depth[cell] = h_new;
+        float hv = fmx(h_new, 0.01f);
+        float u = 0.0f, v = 0.0f;
+        if (h_new >= d.p.min_h) {
+            u = (ql + qr) / m.height[cell] / hv / 2.0f;
+            v = (qb + qt) / m.width[cell] / hv / 2.0f;
+        }
+        float speed = sqrtf(u * u + v * v);
+        float ms = m.max_speed[cell];
+        m.max_speed[cell] = fmx(ms, speed);
+        if (h_new > m.max_depth[cell]) {
+            m.max_depth[cell] = h_new;
+            m.peak_tick[cell] = c->step_end;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // limited-face count
+    uint32_t wf = wave_sum_u(faces_limited);
+    if (lane == 0) s_faces[wave] = wf;
+    // FP64 budget: one block reduction per term that is nonzero somewhere in the block
+    double2 *acc = d.acc + (size_t)blockIdx.x * d.terms;
+    for (int t = 0; t < d.terms; ++t) {
+        float v = 0.0f;
+        if (t == 0)
+            v = t_infil;
+        else if (t == 1)
+            v = t_floor;
+        else if (t == 2)
+            v = t_level_floor;
+        else if (t == 3)
+            v = t_pipe;
+        else if (t < 4 + 5 * b_num) {
+            int tb = (t - 4) / 5, comp = (t - 4) % 5;
+            if (kind == 0 && bi == tb) v = t_lv[comp];
+        } else if (kind == 1 && bi == t - 4 - 5 * b_num) {
+            v = t_disc;
+        }
+        if (!__syncthreads_or(v != 0.0f)) continue;
+        double w = wave_sum((double)v);
+        if (lane == 0) s_red[wave] = w;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double total = 0.0;
+            for (int i = 0; i < NHF_WAVES; ++i) total += s_red[i];
+            double2 a = acc[t];
+            acc_add(a, total);
+            acc[t] = a;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tf = 0;
+        for (int i = 0; i < NHF_WAVES; ++i) tf += s_faces[i];
+        if (tf) atomicAdd(&c->limited_faces, tf);
+    }
+}
+
+// --------------------------------------------------------------------------- (c) CFL dt
+
+// mode 0: standalone compute_cfl (state only). mode 1: inside a batch; the last workgroup also
+// commits `slot` and prepares `slot + 1` when it is below `steps`.
+__global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int slot, int steps) {
+    Ctl *c = d.ctl;
+    if (mode == 1 && !c->active) return;
+    __shared__ uint64_t s_key[NHF_WAVES];
+    __shared__ int32_t s_idx[NHF_WAVES];
+    __shared__ int s_last;
+    const nhf_mesh &m = d.m;
+    int si = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
+    uint64_t key = ~0ull;
+    int32_t idx = 0;
+    if (si < m.s_num) {
+        int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
+        if (lh.x != 0 && lh.y != 0) {
+            float zl = m.bed[lh.x], zh = m.bed[lh.y];
+            float head = fmx(zl + m.depth[lh.x], zh + m.depth[lh.y]);
+            float h = fmx(head - fmx(fmx(zl, zh), -999.0f), 0.01f);
+            float cfl = d.p.afa * m.length[si] / (sqrtf(d.p.g * h) + fabsf(m.q[si]) / h);
+            key = ((uint64_t)__float_as_uint(cfl) << 32) | m.side_key[si];
+            idx = si;
+        }
+    }
+    wave_min_key(key, idx);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_key[wave] = key;
+        s_idx[wave] = idx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NHF_WAVES; ++w)
+            if (s_key[w] < key) {
+                key = s_key[w];
+                idx = s_idx[w];
+            }
+        d.cfl_key[blockIdx.x] = key;
+        d.cfl_idx[blockIdx.x] = idx;
+        __threadfence();
+        uint32_t ticket = atomicAdd(&c->ticket, 1u);
+        s_last = ticket == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    // last workgroup: reduce the per-block partials
+    __threadfence();
+    key = ~0ull;
+    idx = 0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += NHF_BLOCK) {
+        uint64_t k2 = __atomic_load_n(&d.cfl_key[b], __ATOMIC_RELAXED);
+        int32_t i2 = __atomic_load_n(&d.cfl_idx[b], __ATOMIC_RELAXED);
+        if (k2 < key) {
+            key = k2;
+            idx = i2;
+        }
+    }
+    wave_min_key(key, idx);
+    __syncthreads();
+    if (lane == 0) {
+        s_key[wave] = key;
+        s_idx[wave] = idx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NHF_WAVES; ++w)
+            if (s_key[w] < key) {
+                key = s_key[w];
+                idx = s_idx[w];
+            }
+        c->ticket = 0;
+        if (key == ~0ull) {
+            c->ndt = INFINITY;
+            c->cfl_arg = 0;
+        } else {
+            c->ndt = __uint_as_float((uint32_t)(key >> 32));
+            c->cfl_arg = idx;
+        }
+        if (mode == 1) {
+            commit_slot(c, slot);
+            if (slot + 1 < steps) prepare_slot(d, slot + 1);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- host side
+
+enum { K_PREPARE = 0, K_GRAD, K_FACE, K_FLUX, K_DONOR, K_DEPTH, K_CFL, K_COUNT };
+
+struct nhf_ctx {
+    Dev d;
+    hipStream_t stream;
+    int graph_enabled, timing;
+    hipGraphExec_t exec[NHF_MAX_BATCH + 1];
+    std::vector<hipEvent_t> events;
+    std::vector<int> event_kind;
+    double kernel_ms[K_COUNT];
+    int64_t kernel_count[K_COUNT];
+    uint32_t *host_trace;
+    double2 *host_acc;
+    Ctl *probe_scratch;
+};
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Carve {
+    size_t ctl, coeff, values, level, rain, donor, qflux, face_qn, grad, key, idx, acc, total;
+};
+
+static int blocks_for(int n) { return (n + NHF_BLOCK - 1) / NHF_BLOCK; }
+
+static Carve carve(const nhf_mesh *m, const nhf_params *p) {
+    Carve c;
+    size_t off = 0;
+    int b = m->b_num > 0 ? m->b_num : 1;
+    int terms = 4 + 6 * m->b_num;
+    int cfl_blocks = blocks_for(m->s_num - 1 > 0 ? m->s_num - 1 : 1);
+    int depth_blocks = blocks_for(m->e_num - 1 > 0 ? m->e_num - 1 : 1);
+    bool theta_one = p->theta == 1.0f;
+    c.ctl = off; off = align_up(off + sizeof(Ctl));
+    c.coeff = off; off = align_up(off + sizeof(float) * 2 * b);
+    c.values = off; off = align_up(off + sizeof(float) * b);
+    c.level = off; off = align_up(off + sizeof(float) * b);
+    c.rain = off; off = align_up(off + sizeof(float) * (m->n_pixels > 0 ? m->n_pixels : 1));
+    c.donor = off; off = align_up(off + sizeof(float) * m->e_num);
+    c.qflux = off; off = align_up(off + sizeof(float) * m->s_num);
+    c.face_qn = off; off = align_up(off + (theta_one ? 0 : sizeof(float) * 4 * (size_t)m->e_num));
+    c.grad = off; off = align_up(off + sizeof(float) * 2 * (m->n_grad > 0 ? m->n_grad : 1));
+    c.key = off; off = align_up(off + sizeof(uint64_t) * cfl_blocks);
+    c.idx = off; off = align_up(off + sizeof(int32_t) * cfl_blocks);
+    c.acc = off; off = align_up(off + sizeof(double2) * (size_t)depth_blocks * terms);
+    c.total = off;
+    return c;
+}
+
+extern "C" {
+
+int32_t nhf_abi_version(void) { return NHF_ABI_VERSION; }
+
+const char *nhf_last_error(void) { return g_error; }
+
+size_t nhf_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params) {
+    if (!mesh || !params) return 0;
+    return carve(mesh, params).total;
+}
+
+int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, void *workspace,
+                   size_t workspace_bytes, void *stream) {
+    if (!out || !mesh || !params || !workspace) return set_error_text("nhf_create: null argument");
+    if (params->batch_capacity < 1 || params->batch_capacity > NHF_MAX_BATCH)
+        return set_error_text("nhf_create: batch_capacity out of range");
+    if (mesh->e_num < 2 || mesh->s_num < 1) return set_error_text("nhf_create: empty mesh");
+    Carve cv = carve(mesh, params);
+    if (workspace_bytes < cv.total) return set_error_text("nhf_create: workspace too small");
+    if ((params->theta != 1.0f) && !mesh->open_face)
+        return set_error_text("nhf_create: open_face is required when sita != 1");
+    if (mesh->n_hang > 0 && (!mesh->edge_index || !mesh->x || !mesh->y || !mesh->grad_cell))
+        return set_error_text("nhf_create: hanging reconstruction arrays missing");
+    nhf_ctx *ctx = new nhf_ctx();
+    char *ws = (char *)workspace;
+    Dev &d = ctx->d;
+    d.m = *mesh;
+    if (mesh->n_hang == 0) d.m.edge_index = nullptr;
+    d.p = *params;
+    d.ctl = (Ctl *)(ws + cv.ctl);
+    d.coefficients = (float *)(ws + cv.coeff);
+    d.values = (float *)(ws + cv.values);
+    d.level_values = (float *)(ws + cv.level);
+    d.rain_rates = (float *)(ws + cv.rain);
+    d.donor = (float *)(ws + cv.donor);
+    d.qflux = (float *)(ws + cv.qflux);
+    d.face_qn = params->theta == 1.0f ? nullptr : (float *)(ws + cv.face_qn);
+    d.grad = (float *)(ws + cv.grad);
+    d.cfl_key = (uint64_t *)(ws + cv.key);
+    d.cfl_idx = (int32_t *)(ws + cv.idx);
+    d.acc = (double2 *)(ws + cv.acc);
+    d.terms = 4 + 6 * mesh->b_num;
+    d.cfl_blocks = blocks_for(mesh->s_num - 1 > 0 ? mesh->s_num - 1 : 1);
+    d.depth_blocks = blocks_for(mesh->e_num - 1);
+    d.theta_one = params->theta == 1.0f;
+    ctx->stream = (hipStream_t)stream;
+    ctx->graph_enabled = 0;
+    ctx->timing = 0;
+    for (int i = 0; i <= NHF_MAX_BATCH; ++i) ctx->exec[i] = nullptr;
+    hipError_t err = hipHostMalloc((void **)&ctx->host_trace, sizeof(uint32_t) * NHF_MAX_BATCH * NHF_TRACE_COLS,
+                                   hipHostMallocDefault);
+    if (err != hipSuccess) {
+        delete ctx;
+        return set_error("hipHostMalloc trace", err);
+    }
+    ctx->host_acc = nullptr;
+    ctx->probe_scratch = nullptr;
+    memset(ctx->kernel_ms, 0, sizeof ctx->kernel_ms);
+    memset(ctx->kernel_count, 0, sizeof ctx->kernel_count);
+    *out = ctx;
+    return 0;
+}
+
+int32_t nhf_destroy(nhf_ctx *ctx) {
+    if (!ctx) return 0;
+    hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i <= NHF_MAX_BATCH; ++i)
+        if (ctx->exec[i]) hipGraphExecDestroy(ctx->exec[i]);
+    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+    if (ctx->host_trace) hipHostFree(ctx->host_trace);
+    if (ctx->host_acc) hipHostFree(ctx->host_acc);
+    delete ctx;
+    return 0;
+}
+
+int32_t nhf_reset(nhf_ctx *ctx, uint64_t now_ticks) {
+    hipLaunchKernelGGL(k_reset, dim3(256), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, now_ticks);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhf_set_forcing(nhf_ctx *ctx, const float *coefficients, const float *rain_rates, int32_t raining) {
+    const Dev &d = ctx->d;
+    if (d.m.b_num > 0 && coefficients)
+        HIP_TRY(hipMemcpyAsync(d.coefficients, coefficients, sizeof(float) * 2 * d.m.b_num,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (rain_rates && d.m.n_pixels > 0)
+        HIP_TRY(hipMemcpyAsync(d.rain_rates, rain_rates, sizeof(float) * d.m.n_pixels,
+                               hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_set_raining, dim3(1), dim3(64), 0, ctx->stream, ctx->d, (int)raining);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhf_begin(nhf_ctx *ctx, uint64_t target_ticks, uint64_t origin_ticks) {
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, target_ticks, origin_ticks);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static hipEvent_t timing_event(nhf_ctx *ctx, size_t i) {
+    while (ctx->events.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->events.push_back(e);
+    }
+    return ctx->events[i];
+}
+
+// Launch the kernels of `steps` slots. With timing on, each launch is bracketed by events.
+static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
+    const Dev &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    size_t ev = 0;
+    ctx->event_kind.clear();
+#define LAUNCH(kind, ...)                                                   \
+    do {                                                                    \
+        if (timed) {                                                        \
+            hipEvent_t e0 = timing_event(ctx, ev++);                        \
+            if (!e0) return set_error_text("hipEventCreate failed");        \
+            HIP_TRY(hipEventRecord(e0, s));                                 \
+        }                                                                   \
+        hipLaunchKernelGGL(__VA_ARGS__);                                    \
+        if (timed) {                                                        \
+            hipEvent_t e1 = timing_event(ctx, ev++);                        \
+            if (!e1) return set_error_text("hipEventCreate failed");        \
+            HIP_TRY(hipEventRecord(e1, s));                                 \
+            ctx->event_kind.push_back(kind);                                \
+        }                                                                   \
+    } while (0)
+    const int cells = d.m.e_num - 1, sides = d.m.s_num - 1;
+    LAUNCH(K_PREPARE, k_prepare, dim3(1), dim3(64), 0, s, d, 0);
+    for (int slot = 0; slot < steps; ++slot) {
+        if (d.m.n_grad > 0) LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
+        if (!d.theta_one) LAUNCH(K_FACE, k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
+        if (sides > 0) LAUNCH(K_FLUX, k_flux, dim3(blocks_for(sides)), dim3(NHF_BLOCK), 0, s, d);
+        LAUNCH(K_DONOR, k_donor, dim3(blocks_for(cells)), dim3(NHF_BLOCK), 0, s, d);
+        LAUNCH(K_DEPTH, k_depth, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
+        LAUNCH(K_CFL, k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, slot, steps);
+    }
+#undef LAUNCH
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
+    if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhf_run_batch: steps out of range");
+    hipStream_t s = ctx->stream;
+    if (ctx->graph_enabled && !ctx->timing && s != nullptr) {
+        if (!ctx->exec[steps]) {
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int32_t rc = launch_batch(ctx, steps, false);
+            hipError_t end = hipStreamEndCapture(s, &graph);
+            if (rc) return rc;
+            if (end != hipSuccess) return set_error("hipStreamEndCapture", end);
+            hipError_t inst = hipGraphInstantiate(&ctx->exec[steps], graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (inst != hipSuccess) return set_error("hipGraphInstantiate", inst);
+        }
+        HIP_TRY(hipGraphLaunch(ctx->exec[steps], s));
+    } else {
+        int32_t rc = launch_batch(ctx, steps, ctx->timing != 0);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->host_trace, ctx->d.ctl->trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (ctx->timing) {
+        for (size_t i = 0; i < ctx->event_kind.size(); ++i) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->events[2 * i], ctx->events[2 * i + 1]));
+            ctx->kernel_ms[ctx->event_kind[i]] += ms;
+            ctx->kernel_count[ctx->event_kind[i]] += 1;
+        }
+    }
+    if (trace) memcpy(trace, ctx->host_trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps);
+    return 0;
+}
+
+int32_t nhf_compute_cfl(nhf_ctx *ctx) {
+    const Dev &d = ctx->d;
+    hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0, 0);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhf_flush_budget(nhf_ctx *ctx, double *out) {
+    const Dev &d = ctx->d;
+    size_t n = (size_t)d.depth_blocks * d.terms;
+    if (!ctx->host_acc) HIP_TRY(hipHostMalloc((void **)&ctx->host_acc, sizeof(double2) * n, hipHostMallocDefault));
+    HIP_TRY(hipMemcpyAsync(ctx->host_acc, d.acc, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < n; ++i)
+        if (!isfinite(ctx->host_acc[i].x) || !isfinite(ctx->host_acc[i].y)) return 1;
+    for (int t = 0; t < d.terms; ++t) {
+        // two-sum accumulation of every block's (sum, compensation) pair
+        double hi = 0.0, lo = 0.0;
+        for (int b = 0; b < d.depth_blocks; ++b) {
+            const double2 a = ctx->host_acc[(size_t)b * d.terms + t];
+            const double parts[2] = {a.x, a.y};
+            for (double x : parts) {
+                double s = hi + x;
+                double bp = s - hi;
+                lo += (hi - (s - bp)) + (x - bp);
+                hi = s;
+            }
+        }
+        out[2 * t] = hi;
+        out[2 * t + 1] = lo;
+    }
+    hipLaunchKernelGGL(k_clear_acc, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhf_limit_capture(nhf_ctx *ctx, float *values, int32_t *ids, int32_t reset) {
+    Ctl *ctl = ctx->d.ctl;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (values) HIP_TRY(hipMemcpy(values, ctl->limit_value, sizeof(float) * 5, hipMemcpyDeviceToHost));
+    if (ids) HIP_TRY(hipMemcpy(ids, ctl->limit_ids, sizeof(int32_t) * 3, hipMemcpyDeviceToHost));
+    if (reset) {
+        hipLaunchKernelGGL(k_reset_limit, dim3(1), dim3(64), 0, ctx->stream, ctx->d);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side) {
+    Ctl *ctl = ctx->d.ctl;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ndt) HIP_TRY(hipMemcpy(ndt, &ctl->ndt, sizeof(float), hipMemcpyDeviceToHost));
+    if (side) HIP_TRY(hipMemcpy(side, &ctl->cfl_arg, sizeof(int32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int32_t nhf_sync(nhf_ctx *ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable) {
+    ctx->graph_enabled = enable != 0;
+    return 0;
+}
+
+int32_t nhf_set_timing(nhf_ctx *ctx, int32_t enable) {
+    ctx->timing = enable != 0;
+    return 0;
+}
+
+int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset) {
+    for (int i = 0; i < n && i < K_COUNT; ++i) {
+        ms[i] = ctx->kernel_ms[i];
+        count[i] = ctx->kernel_count[i];
+    }
+    if (reset) {
+        memset(ctx->kernel_ms, 0, sizeof ctx->kernel_ms);
+        memset(ctx->kernel_count, 0, sizeof ctx->kernel_count);
+    }
+    return 0;
+}
+
+int32_t nhf_clock_probe(const uint64_t *now, const uint64_t *target, const uint64_t *origin, const float *candidate,
+                        float min_dt, int32_t n, uint32_t *out_words, uint64_t *out_ticks, float *out_times) {
+    if (n <= 0) return 0;
+    char *buf = nullptr;
+    size_t in_bytes = n * (3 * sizeof(uint64_t) + sizeof(float));
+    size_t out_bytes = n * (4 * sizeof(uint32_t) + 2 * sizeof(uint64_t) + 2 * sizeof(float));
+    size_t scratch = n * sizeof(Ctl);
+    HIP_TRY(hipMalloc((void **)&buf, in_bytes + out_bytes + scratch));
+    uint64_t *d_now = (uint64_t *)buf, *d_target = d_now + n, *d_origin = d_target + n;
+    float *d_cand = (float *)(d_origin + n);
+    uint32_t *d_words = (uint32_t *)(d_cand + n);
+    uint64_t *d_ticks = (uint64_t *)(d_words + 4 * n);
+    float *d_times = (float *)(d_ticks + 2 * n);
+    Ctl *d_scratch = (Ctl *)(buf + in_bytes + out_bytes);
+    int32_t rc = 0;
+    hipError_t e = hipSuccess;
+    if ((e = hipMemcpy(d_now, now, n * sizeof(uint64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(d_target, target, n * sizeof(uint64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(d_origin, origin, n * sizeof(uint64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(d_cand, candidate, n * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = set_error("nhf_clock_probe upload", e);
+    } else {
+        hipLaunchKernelGGL(k_clock_probe, dim3(blocks_for(n)), dim3(NHF_BLOCK), 0, 0, d_now, d_target, d_origin,
+                           d_cand, min_dt, n, d_words, d_ticks, d_times, d_scratch);
+        if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
+            (e = hipMemcpy(out_words, d_words, 4 * n * sizeof(uint32_t), hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(out_ticks, d_ticks, 2 * n * sizeof(uint64_t), hipMemcpyDeviceToHost)) != hipSuccess ||
+            (e = hipMemcpy(out_times, d_times, 2 * n * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = set_error("nhf_clock_probe", e);
+    }
+    hipFree(buf);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+"""One mesh bound to a library implementing include/nhflood.h.
+
+:class:`Engine` owns the device arrays of one scenario (torch tensors on the GPU), passes their
+addresses to ``nhf_create`` and exposes the batch protocol of the reference's host loop
+(``solver_compact.py:930-970``): set forcing, begin a batch, run it, read the trace. All state
+exchanged with the caller is in the reference's numbering; the Morton permutation stays inside.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .layout import Layout
+
+STATE_CELLS = ('depth', 'bed', 'rain_time', 'capacity', 'max_depth', 'max_speed', 'peak_tick')
+STATE_SIDES = ('q',)
+
+
+class TorchDevice:
+    """Device memory and stream for the HIP library (torch is plumbing only)."""
+
+    def __init__(self, device: str = 'cuda:0'):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError('no GPU visible: the HIP step needs a ROCm device')
+        self.torch = torch
+        self.device = torch.device(device)
+        self.stream = torch.cuda.Stream(device=self.device)
+        self.lib = abi.load_hip_library()
+        self._dtypes = {np.dtype(np.float32): torch.float32, np.dtype(np.int32): torch.int32,
+                        np.dtype(np.uint32): torch.int32, np.dtype(np.uint8): torch.uint8,
+                        np.dtype(np.uint64): torch.int64, np.dtype(np.int64): torch.int64,
+                        np.dtype(np.float64): torch.float64}
+
+    def array(self, values: np.ndarray):
+        values = np.ascontiguousarray(values)
+        host = values.view(np.int32) if values.dtype == np.uint32 else (
+            values.view(np.int64) if values.dtype == np.uint64 else values)
+        with self.torch.cuda.stream(self.stream):
+            t = self.torch.from_numpy(host.copy()).to(self.device, non_blocking=False)
+        return t
+
+    def empty(self, nbytes: int):
+        with self.torch.cuda.stream(self.stream):
+            return self.torch.zeros(max(nbytes, 1), dtype=self.torch.uint8, device=self.device)
+
+    def read(self, buf, dtype) -> np.ndarray:
+        self.stream.synchronize()
+        out = buf.cpu().numpy()
+        return out.view(dtype)
+
+    def write(self, buf, values: np.ndarray) -> None:
+        values = np.ascontiguousarray(values)
+        host = values.view(np.int32) if values.dtype == np.uint32 else (
+            values.view(np.int64) if values.dtype == np.uint64 else values)
+        with self.torch.cuda.stream(self.stream):
+            buf.copy_(self.torch.from_numpy(host.copy()).to(self.device))
+        self.stream.synchronize()
+
+    @staticmethod
+    def ptr(buf) -> int:
+        return buf.data_ptr()
+
+    def stream_handle(self) -> int:
+        return self.stream.cuda_stream
+
+    def synchronize(self) -> None:
+        self.stream.synchronize()
+
+
+class Engine:
+    """A mesh, its state and the step library (``nhf_*`` entry points)."""
+
+    def __init__(self, layout: Layout, params: abi.Params, state: dict, device=None,
+                 pipe_q: np.ndarray | None = None):
+        self.layout = layout
+        self.device = device if device is not None else TorchDevice()
+        self.lib = self.device.lib
+        self.params = params
+        self.buffers = {}
+        dev = self.device
+        arr = layout.arrays
+        e_num, s_num = layout.e_num, layout.s_num
+        # state (device order)
+        for name in STATE_CELLS:
+            self.buffers[name] = dev.array(layout.cells_to_device(state[name]))
+        self.buffers['q'] = dev.array(layout.sides_to_device(state['q']))
+        for name, values in arr.items():
+            if not name.startswith('_'):
+                self.buffers[name] = dev.array(values)
+        if pipe_q is not None:
+            self.buffers['pipe_q'] = dev.array(layout.cells_to_device(pipe_q.astype(np.float32)))
+        m = abi.MeshStruct()
+        m.e_num, m.s_num = e_num, s_num
+        m.r_len = int(arr['_r_len'][0])
+        m.b_num = layout.b_num
+        m.n_pixels = layout.n_pixels
+        m.n_grad, m.n_hang = layout.n_grad, layout.n_hang
+        m.flags = 1 if pipe_q is not None else 0
+        for name, kind in abi.MESH_FIELDS:
+            if kind == 'p':
+                buf = self.buffers.get(name)
+                setattr(m, name, dev.ptr(buf) if buf is not None else None)
+        self.mesh_struct = m
+        nbytes = self.lib.nhf_workspace_bytes(ctypes.byref(m), ctypes.byref(params))
+        self.workspace = dev.empty(nbytes) if nbytes else None
+        handle = ctypes.c_void_p()
+        rc = self.lib.nhf_create(ctypes.byref(handle), ctypes.byref(m), ctypes.byref(params),
+                                 dev.ptr(self.workspace) if self.workspace is not None else None,
+                                 nbytes, dev.stream_handle())
+        abi.check(self.lib, rc, 'nhf_create')
+        self.ctx = handle
+        self.terms = 4 + 6 * layout.b_num
+        self._trace = np.zeros((abi.MAX_BATCH, abi.TRACE_COLS), dtype=np.uint32)
+
+    def close(self):
+        if getattr(self, 'ctx', None):
+            self.lib.nhf_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----------------------------------------------------------------- state in original order
+    def read_state(self, name: str) -> np.ndarray:
+        dtype = np.uint64 if name == 'peak_tick' else np.float32
+        values = self.device.read(self.buffers[name], dtype)
+        if name in STATE_SIDES:
+            return self.layout.sides_from_device(values)
+        return self.layout.cells_from_device(values)
+
+    def write_state(self, name: str, values: np.ndarray) -> None:
+        dtype = np.uint64 if name == 'peak_tick' else np.float32
+        values = np.asarray(values, dtype=dtype)
+        if name in STATE_SIDES:
+            values = self.layout.sides_to_device(values)
+        else:
+            values = self.layout.cells_to_device(values)
+        self.device.write(self.buffers[name], values)
+
+    # ----------------------------------------------------------------- batch protocol
+    def reset(self, now_ticks: int) -> None:
+        abi.check(self.lib, self.lib.nhf_reset(self.ctx, int(now_ticks)), 'nhf_reset')
+
+    def set_forcing(self, coefficients: np.ndarray, rain_rates: np.ndarray | None, raining: bool):
+        coeff = np.ascontiguousarray(coefficients, dtype=np.float32)
+        rates = None if rain_rates is None else np.ascontiguousarray(rain_rates, dtype=np.float32)
+        rc = self.lib.nhf_set_forcing(self.ctx, coeff.ctypes.data if coeff.size else None,
+                                      rates.ctypes.data if rates is not None else None, int(raining))
+        abi.check(self.lib, rc, 'nhf_set_forcing')
+
+    def begin(self, target_ticks: int, origin_ticks: int) -> None:
+        abi.check(self.lib, self.lib.nhf_begin(self.ctx, int(target_ticks), int(origin_ticks)), 'nhf_begin')
+
+    def run_batch(self, steps: int) -> np.ndarray:
+        rc = self.lib.nhf_run_batch(self.ctx, int(steps), self._trace.ctypes.data)
+        abi.check(self.lib, rc, 'nhf_run_batch')
+        return self._trace[:steps].copy()
+
+    def compute_cfl(self) -> None:
+        abi.check(self.lib, self.lib.nhf_compute_cfl(self.ctx), 'nhf_compute_cfl')
+
+    def cfl_state(self):
+        ndt = ctypes.c_float()
+        side = ctypes.c_int32()
+        abi.check(self.lib, self.lib.nhf_cfl_state(self.ctx, ctypes.byref(ndt), ctypes.byref(side)),
+                  'nhf_cfl_state')
+        return float(ndt.value), int(self.layout.side_perm[side.value])
+
+    def flush_budget(self) -> np.ndarray:
+        """Interval budget totals as [terms, 2] double-double pairs."""
+        out = np.zeros(2 * self.terms, dtype=np.float64)
+        rc = self.lib.nhf_flush_budget(self.ctx, out.ctypes.data)
+        if rc == 1:
+            raise FloatingPointError('Nonfinite F32 device budget diagnostic')
+        abi.check(self.lib, rc, 'nhf_flush_budget')
+        return out.reshape(self.terms, 2)
+
+    def limit_capture(self, reset: bool = False):
+        values = np.zeros(5, dtype=np.float32)
+        ids = np.zeros(3, dtype=np.int32)
+        rc = self.lib.nhf_limit_capture(self.ctx, values.ctypes.data, ids.ctypes.data, int(reset))
+        abi.check(self.lib, rc, 'nhf_limit_capture')
+        return values, np.array([self.layout.side_perm[ids[0]], self.layout.cell_perm[ids[1]],
+                                 self.layout.cell_perm[ids[2]]], dtype=np.int64)
+
+    def set_graph(self, enable: bool) -> None:
+        self.lib.nhf_set_graph(self.ctx, int(enable))
+
+    def set_timing(self, enable: bool) -> None:
+        self.lib.nhf_set_timing(self.ctx, int(enable))
+
+    def kernel_times(self, reset: bool = True):
+        ms = np.zeros(7, dtype=np.float64)
+        count = np.zeros(7, dtype=np.int64)
+        self.lib.nhf_kernel_times(self.ctx, ms.ctypes.data, count.ctypes.data, 7, int(reset))
+        return ms, count
+
+    def synchronize(self) -> None:
+        abi.check(self.lib, self.lib.nhf_sync(self.ctx), 'nhf_sync')
+
+
+def make_params(afa: float, sita: float, min_h: float, batch: int = 32) -> abi.Params:
+    """Constants as F32, the way the reference's kernels see its Python floats."""
+    return abi.Params(g=np.float32(9.81), afa=np.float32(afa), theta=np.float32(sita),
+                      theta_half=np.float32((1.0 - sita) / 2.0), min_h=np.float32(min_h),
+                      min_dt=np.float32(1e-4), batch_capacity=batch, reserved=0)

@@ -1,0 +1,186 @@
+"""Device layout: Morton-ordered structure-of-arrays for the HIP kernels.
+
+The host keeps every array in the reference's (original) numbering; a :class:`Layout` holds the
+permutation to the device order and the packed arrays the kernels read:
+
+* cells sorted by the Morton (Z-order) code of their lower-left corner on the finest grid, so the
+  cells of one quadtree block are contiguous and a wavefront's neighbours are a few cache lines
+  away across all refinement levels;
+* sides sorted by their owner cell in that order (the lower cell, or the interior cell of an
+  exterior side), so the face pass reads cell data almost sequentially and the depth pass writes
+  each side from one cell;
+* a compact side list per cell: ``csr_start`` plus four 8-bit face counts packed in one word
+  (8 B of header per cell instead of the reference's 32 B of pointers and counts);
+* ``side_cells`` as interleaved (lower, higher) pairs and a 1-byte orient/exterior word.
+
+Within each face the reference's side order is kept, so every per-cell sum adds the same values
+in the same order as ``solver_compact.py`` and results do not depend on the renumbering.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .hydraulics import hanging_geometry, map_land_parameters, open_face_fractions, side_centre_distance
+from .mesh import Mesh
+
+
+def morton_codes(ix: np.ndarray, iy: np.ndarray) -> np.ndarray:
+    """Interleave the bits of two non-negative integer arrays (x in even bits)."""
+    def spread(v):
+        v = v.astype(np.uint64) & np.uint64(0xFFFFFFFF)
+        v = (v | (v << np.uint64(16))) & np.uint64(0x0000FFFF0000FFFF)
+        v = (v | (v << np.uint64(8))) & np.uint64(0x00FF00FF00FF00FF)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x0F0F0F0F0F0F0F0F)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x3333333333333333)
+        v = (v | (v << np.uint64(1))) & np.uint64(0x5555555555555555)
+        return v
+    return spread(ix) | (spread(iy) << np.uint64(1))
+
+
+def cell_order(mesh: Mesh, mode: str = 'morton') -> np.ndarray:
+    """Permutation new -> original cell id (index 0 stays the sentinel)."""
+    n = mesh.e_num
+    if mode == 'identity' or n <= 2:
+        return np.arange(n, dtype=np.int64)
+    bounds, widths, _, _ = mesh.geometry()
+    fine = widths[1:].min()
+    x0 = bounds[1:, 0] - bounds[1:, 0].min()
+    y0 = bounds[1:, 1] - bounds[1:, 1].min()
+    ix = np.floor(x0 / fine + 0.5).astype(np.int64)
+    iy = np.floor(y0 / fine + 0.5).astype(np.int64)
+    code = morton_codes(ix, iy)
+    order = np.argsort(code, kind='stable') + 1
+    return np.concatenate([[0], order]).astype(np.int64)
+
+
+@dataclass
+class Forcing:
+    """Per-cell forcing geometry in original numbering (loaded or synthetic)."""
+    rain_pixel: np.ndarray               # [e_num] cell -> pixel
+    n_pixels: int
+    boundary_id: np.ndarray              # [e_num] -1 or boundary index
+    bweight: np.ndarray                  # [e_num] F32 discharge share
+    kinds: np.ndarray                    # [b] 0 level, 1 discharge
+    selected_sides: np.ndarray           # [s_num] bool: side belongs to a prescribed boundary
+
+
+@dataclass
+class Layout:
+    e_num: int
+    s_num: int
+    cell_perm: np.ndarray      # new -> original
+    cell_inv: np.ndarray       # original -> new
+    side_perm: np.ndarray
+    side_inv: np.ndarray
+    arrays: dict = field(default_factory=dict)   # static device arrays (numpy, device order)
+    n_pixels: int = 1
+    b_num: int = 0
+    n_grad: int = 0
+    n_hang: int = 0
+    theta: float = 1.0
+
+    # --- permutations between the reference numbering and the device order
+    def cells_to_device(self, a: np.ndarray) -> np.ndarray:
+        return np.ascontiguousarray(a[self.cell_perm])
+
+    def cells_from_device(self, a: np.ndarray) -> np.ndarray:
+        return np.ascontiguousarray(a[self.cell_inv])
+
+    def sides_to_device(self, a: np.ndarray) -> np.ndarray:
+        return np.ascontiguousarray(a[self.side_perm])
+
+    def sides_from_device(self, a: np.ndarray) -> np.ndarray:
+        return np.ascontiguousarray(a[self.side_inv])
+
+
+def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: float,
+                 order: str = 'morton') -> Layout:
+    """Derive every static kernel array from the mesh, land table and forcing geometry."""
+    e_num, s_num = mesh.e_num, mesh.s_num
+    topo = mesh.topo
+    _, widths, heights, area = mesh.geometry()
+    side_n, f0, fc, hk = map_land_parameters(land_parameters, mesh.land, topo)
+    sdc = side_centre_distance(mesh.x, mesh.y, topo)
+
+    cperm = cell_order(mesh, order)
+    cinv = np.empty_like(cperm)
+    cinv[cperm] = np.arange(e_num)
+    # sides ordered by owner cell in the new order, then by original id
+    owner = np.where(topo[:, 1] != 0, topo[:, 1], topo[:, 2])
+    owner_new = cinv[owner]
+    owner_new[0] = -1
+    sperm = np.lexsort((np.arange(s_num), owner_new)).astype(np.int64)
+    if order == 'identity':
+        sperm = np.arange(s_num, dtype=np.int64)
+    sinv = np.empty_like(sperm)
+    sinv[sperm] = np.arange(s_num)
+
+    # compact side lists in the new numbering, keeping each face's side order
+    counts = mesh.face_count[cperm]
+    totals = counts.sum(axis=1).astype(np.int64)
+    start = np.concatenate([[0], np.cumsum(totals)[:-1]]).astype(np.int64)
+    src_start = mesh.face_ptr[cperm, 0].astype(np.int64)
+    r_len = int(totals.sum())
+    gather = np.repeat(src_start - start, totals) + np.arange(r_len)
+    isl = sinv[mesh.isl[gather]].astype(np.int32) if r_len else np.zeros(1, np.int32)
+    if np.any(counts > 255):
+        raise ValueError('a cell face has more than 255 sides')
+    packed = (counts[:, 0].astype(np.uint32) | (counts[:, 1].astype(np.uint32) << 8) |
+              (counts[:, 2].astype(np.uint32) << 16) | (counts[:, 3].astype(np.uint32) << 24))
+
+    side_cells = np.empty((s_num, 2), dtype=np.int32)
+    side_cells[:, 0] = cinv[topo[sperm, 1]]
+    side_cells[:, 1] = cinv[topo[sperm, 2]]
+    meta = topo[sperm, 0].astype(np.uint8) | (mesh.sbf[sperm] * 4).astype(np.uint8)
+
+    arrays = dict(
+        area=area.astype(np.float32)[cperm], width=widths.astype(np.float32)[cperm],
+        height=heights.astype(np.float32)[cperm],
+        f0=f0[cperm], fc=fc[cperm], hk=hk[cperm],
+        rain_index=forcing.rain_pixel.astype(np.int32)[cperm],
+        boundary_id=forcing.boundary_id.astype(np.int32)[cperm],
+        bweight=forcing.bweight.astype(np.float32)[cperm],
+        csr_start=start.astype(np.int32), csr_count=packed.astype(np.uint32), isl=isl,
+        side_cells=side_cells.ravel(), side_meta=meta,
+        length=mesh.length[sperm].astype(np.float32), sdc=sdc[sperm],
+        manning=side_n[sperm], side_key=sperm.astype(np.uint32),
+        kind=(forcing.kinds.astype(np.int32) if len(forcing.kinds) else np.zeros(1, np.int32)),
+    )
+    layout = Layout(e_num, s_num, cperm, cinv, sperm, sinv, arrays, forcing.n_pixels,
+                    len(forcing.kinds), 0, 0, float(theta))
+    if theta != 1.:
+        fr = open_face_fractions(forcing.selected_sides, mesh.length, topo, mesh.face_ptr,
+                                 mesh.face_count, mesh.isl)
+        arrays['open_face'] = np.ascontiguousarray(fr[cperm]).ravel()
+    geometry = hanging_geometry(mesh.x, mesh.y, mesh.sx, mesh.sy, topo)
+    if geometry is not None:
+        arrays['x'] = mesh.x[cperm].astype(np.float32)
+        arrays['y'] = mesh.y[cperm].astype(np.float32)
+        arrays['edge_index'] = geometry.edge_index[sperm].astype(np.int32)
+        arrays['hang_offset'] = geometry.offsets.astype(np.float32).ravel()
+        arrays['hang_slot'] = geometry.slots.astype(np.int32).ravel()
+        arrays['grad_cell'] = cinv[geometry.cells].astype(np.int32)
+        arrays['grad_axes'] = (geometry.axes[:, 0] | (geometry.axes[:, 1] << 1)).astype(np.int32)
+        layout.n_grad = len(geometry.cells)
+        layout.n_hang = len(geometry.offsets)
+    layout.arrays['_r_len'] = np.array([r_len])
+    return layout
+
+
+def initial_state(mesh: Mesh, layout: Layout) -> dict:
+    """Cold-start state in original numbering (``initialize``, solver_compact.py:398-410)."""
+    e_num, s_num = mesh.e_num, mesh.s_num
+    f0 = map_land_parameters_cached(layout)
+    return dict(depth=np.zeros(e_num, np.float32), bed=mesh.z.astype(np.float32).copy(),
+                rain_time=np.zeros(e_num, np.float32), capacity=f0,
+                max_depth=np.zeros(e_num, np.float32), max_speed=np.zeros(e_num, np.float32),
+                peak_tick=np.zeros(e_num, np.uint64), q=np.zeros(s_num, np.float32))
+
+
+def map_land_parameters_cached(layout: Layout) -> np.ndarray:
+    f0 = layout.cells_from_device(layout.arrays['f0'])
+    f0 = f0.copy()
+    f0[0] = 0.
+    return f0

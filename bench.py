@@ -1,0 +1,242 @@
+"""Benchmark: cell-updates/s of the HIP local-inertial step on one MI355X.
+
+Workload: the single large multi-resolution mesh (about 1.58 M cells over 2/4/8 m levels around a
+seeded channel network, about 3.17 M sides, 2:1 balanced, diamond-square DEM amplitude 40 m and
+roughness 0.6, four land classes with Horton infiltration, triangular hyetograph peaking at
+80 mm/h, tide boundary 1.0 m sin(2 pi t / 44640 s)). The state is wet (channels and low ground
+filled to 2 m, a 1 cm film elsewhere) and is developed for `--warmup` steps before `--steps` steps
+are timed. A step is one pass of the device time step over the whole mesh, clock and CFL
+included; batches of 32 steps run as one captured hipGraph with one trace readback each, exactly
+the host protocol of the solver. Mesh, state and forcing are resident in HBM before timing.
+
+Prints one JSON line (rank 0, N = 1 only):
+  value        whole-job cell-updates/s = cells x steps / wall seconds of the timed region
+  roofline     the dominant kernel's algorithmic bytes / its mean duration (HIP events)
+  cpu_baseline the CPU oracle (C, OpenMP) on the host cores, on a bounded sample of steps
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = 'cell-updates/s + %HBM roofline, batched/served vs plain; p50/p99 solve latency'
+HBM_PEAK_GBS = 8000.0
+KERNELS = ['prepare', 'gradients', 'face_avg', 'flux', 'donor', 'depth', 'cfl', 'finish']
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def build_workload(seed: int):
+    from nh_flood_2d_amd import synth
+    sc = synth.multires_channel(seed)
+    forcing = synth.scenario_forcing(sc)
+    depth = np.zeros(sc.mesh.e_num, dtype=np.float32)
+    depth[1:] = np.maximum(np.float32(2.0) - sc.mesh.z[1:], np.float32(0.01))
+    return sc, forcing, depth
+
+
+def kernel_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
+    """Algorithmic (compulsory) bytes per launch of each kernel: every input it needs read once,
+    every output written once (conditional extrema writes excluded). Nc cells, Ns sides, R
+    side-list entries, E hanging edges, G gradient cells."""
+    nc, ns, r = e_num - 1, s_num - 1, r_len
+    e_hang, g = layout.n_hang, layout.n_grad
+    pixel = 4 if layout.n_pixels > 1 else 0
+    return {
+        # per side: side_cells 8, meta 1, q 4, sdc 4, manning 4, qflux out 4 (+ edge_index 4);
+        # per cell: bed + depth 8; per hanging edge: offsets + slots 16; per gradient cell 8
+        'flux': ns * (25 + (4 if e_hang else 0)) + 8 * nc + 16 * e_hang + 8 * g,
+        # per cell: meta 4, area 4, depth 4, csr start + counts 8, donor out 4 (+ rain pixel);
+        # per side-list entry: side id 4; per side: qflux 4, length 4
+        'donor': (24 + pixel) * nc + 4 * r + 8 * ns,
+        # per cell: meta 4, donor 4, rain time r+w 8, capacity r+w 8, area 4, size 4, depth r+w 8,
+        # max speed 4, max depth 4, csr 8 (+ rain pixel); per side-list entry: side id 4,
+        # neighbour id 4; per side: qflux 4, length 4, q out 4
+        'depth': (56 + pixel) * nc + 8 * r + 12 * ns,
+        # per side: side_cells 8, length 4, q 4, reference id 4; per cell: bed + depth 8
+        'cfl': 20 * ns + 8 * nc,
+        # per gradient cell: its face neighbours' bed, depth, centre (about 5 x 12 B)
+        'gradients': g * 60,
+        'face_avg': 0, 'prepare': 0, 'finish': 0,
+    }
+
+
+def cpu_baseline(sc, forcing, depth, seconds_budget: float = 15.0):
+    """The CPU oracle (C restatement, OpenMP) timed on a bounded sample of the same workload."""
+    threads = int(os.environ.get('OMP_NUM_THREADS', '16'))
+    os.environ['OMP_NUM_THREADS'] = str(threads)
+    import oracle
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state
+    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, forcing), sc.sita)
+    state = initial_state(sc.mesh, layout)
+    state['depth'] = depth.copy()
+    eng = Engine(layout, make_params(sc.afa, sc.sita, sc.min_h, 32), state, device=oracle.HostDevice('f32'))
+    _set_forcing(eng, forcing, 3600.)
+    eng.reset(seconds_to_ticks(3600.))
+    eng.compute_cfl()
+    eng.begin(seconds_to_ticks(3600. + 1e5), seconds_to_ticks(3600.))
+    eng.run_batch(1)  # first touch
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds_budget:
+        rows = eng.run_batch(4)
+        steps += int(np.count_nonzero(rows[:, 3] == 1))
+    wall = time.perf_counter() - t0
+    cells = sc.mesh.e_num - 1
+    return dict(value=cells * steps / wall, unit='cell-updates/s', cores=threads, kind='port',
+                sample=f'{steps} steps of the same {cells}-cell mesh and state, '
+                       f'{wall:.1f} s wall, C oracle with {threads} OpenMP threads')
+
+
+def _set_forcing(eng, forcing, t: float):
+    c = np.zeros((max(1, len(forcing.boundaries)), 2), dtype=np.float32)
+    for bi, b in enumerate(forcing.boundaries):
+        times = b.series.times
+        i = min(int(np.searchsorted(times, t, side='right')) - 1, len(times) - 2)
+        c[bi] = (b.series.at(t), (b.series.values[i + 1] - b.series.values[i]) / (times[i + 1] - times[i]))
+    rain = forcing.rain
+    eng.set_forcing(c.ravel() if forcing.boundaries else np.zeros(0, np.float32),
+                    rain.rates(rain.index_at(t)).astype(np.float32), True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=640)
+    ap.add_argument('--warmup', type=int, default=128)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--kernel-steps', type=int, default=64, help='steps of the per-kernel timing pass')
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='also copy a 256 MiB buffer 3 times (reference dispatch for HBM counters)')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+              '(independent scenarios would shard one per GPU; see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, TorchDevice, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state
+
+    t0 = time.time()
+    sc, forcing, depth = build_workload(args.seed)
+    mesh = sc.mesh
+    layout = build_layout(mesh, sc.land_parameters, cell_forcing_arrays(mesh, forcing), sc.sita)
+    r_len = int(layout.arrays['_r_len'][0])
+    log(f'workload: {mesh.e_num - 1} cells, {mesh.s_num - 1} sides, R={r_len}, '
+        f'hanging edges {layout.n_hang}, gradient cells {layout.n_grad} ({time.time() - t0:.1f} s)')
+    state = initial_state(mesh, layout)
+    state['depth'] = depth
+    device = TorchDevice()
+    eng = Engine(layout, make_params(sc.afa, sc.sita, sc.min_h, 32), state, device=device)
+    start = seconds_to_ticks(3600.)
+    _set_forcing(eng, forcing, 3600.)
+    eng.reset(start)
+    eng.compute_cfl()
+    eng.set_graph(True)
+    horizon = seconds_to_ticks(3600. + 1e5)
+    now = start
+
+    def run(steps):
+        nonlocal now
+        done = 0
+        while done < steps:
+            n = min(32, steps - done)
+            eng.begin(horizon, now)
+            rows = eng.run_batch(n)
+            ok = rows[:, 3] == 1
+            if not ok.all():
+                raise RuntimeError(f'batch stopped early: status {rows[~ok][0, 3]}')
+            last = rows[-1]
+            now = (int(last[1]) << 32) | int(last[2])
+            done += n
+        return done
+
+    run(args.warmup)
+    eng.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(device.stream):
+        ev0.record()
+    w0 = time.perf_counter()
+    steps = run(args.steps)
+    with torch.cuda.stream(device.stream):
+        ev1.record()
+    eng.synchronize()
+    wall = time.perf_counter() - w0
+    event_ms = ev0.elapsed_time(ev1)
+    cells = mesh.e_num - 1
+    value = cells * steps / wall
+    sim_seconds = (now - start) / 2 ** 40
+    log(f'timed: {steps} steps in {wall:.4f} s wall ({event_ms:.2f} ms by events), '
+        f'{value / 1e9:.3f} G cell-updates/s, simulated {sim_seconds:.2f} s')
+
+    # per-kernel timing pass (eager launches bracketed by HIP events on the same stream)
+    eng.set_graph(False)
+    eng.set_timing(True)
+    eng.kernel_times(reset=True)
+    run(args.kernel_steps)
+    ms, count = eng.kernel_times(reset=True)
+    eng.set_timing(False)
+    nbytes = kernel_bytes(layout, mesh.e_num, mesh.s_num, r_len)
+    kernels = {}
+    for i, name in enumerate(KERNELS):
+        if count[i]:
+            avg_ms = ms[i] / count[i]
+            kernels[name] = dict(avg_us=avg_ms * 1e3, launches=int(count[i]),
+                                 bytes=int(nbytes[name]),
+                                 gbs=nbytes[name] / (avg_ms * 1e-3) / 1e9 if nbytes[name] else None)
+    dominant = max((k for k in kernels if nbytes[k]), key=lambda k: kernels[k]['avg_us'])
+    dk = kernels[dominant]
+    step_bytes = sum(nbytes[k] for k in ('flux', 'donor', 'depth', 'cfl', 'gradients'))
+    log('kernels: ' + ', '.join(f"{k} {v['avg_us']:.1f} us" + (f" {v['gbs']:.0f} GB/s" if v['gbs'] else '')
+                                for k, v in kernels.items()))
+    result = {
+        'metric': METRIC, 'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': steps,
+        'warmup': args.warmup, 'ms_per_step': event_ms / steps, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': f'multi-resolution 2/4/8 m mesh, {cells} cells, {mesh.s_num - 1} sides, '
+                               'seeded channel network + diamond-square DEM, 4 Horton land classes, '
+                               'tide boundary, wet start, plain single-scenario path',
+                   'cells': cells, 'sides': mesh.s_num - 1, 'side_list': r_len,
+                   'hanging_edges': layout.n_hang, 'batch_steps': 32, 'launch': 'hipGraph per 32-step batch'},
+        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': dk['gbs'], 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': dk['gbs'] / HBM_PEAK_GBS, 'traffic': None,
+                     'algorithmic_bytes': dk['bytes'], 'avg_us': dk['avg_us']},
+        'kernels': kernels,
+        'step': {'algorithmic_bytes': step_bytes, 'achieved_gbs': step_bytes / (event_ms / steps * 1e-3) / 1e9},
+    }
+    if args.calibrate_copy:
+        with torch.cuda.stream(device.stream):
+            src = torch.ones(64 << 20, dtype=torch.float32, device=device.device)
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+        device.stream.synchronize()
+        result['copy_calibration_bytes'] = 2 * src.numel() * 4
+    if not args.no_cpu:
+        del eng
+        torch.cuda.synchronize()
+        result['cpu_baseline'] = cpu_baseline(sc, forcing, depth, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -79,10 +79,11 @@ typedef struct nhf_mesh {
     float *max_speed;
     uint64_t *peak_tick; /* first tick of max_depth */
     /* per-cell static */
-    const float *area, *width, *height;
-    const float *f0, *fc, *hk;       /* Horton f0, fc (m/s), k (1/h) */
-    const int32_t *rain_index;        /* cell -> rain pixel */
-    const int32_t *boundary_id;       /* -1 or boundary index */
+    const float *area;                /* F32 of the F64 width x height (MeshGeometry.area) */
+    const float *size;                /* edge length; cells are square (force.py:52-53) */
+    const uint32_t *cell_meta;        /* land class | (boundary index + 1) << 8, 0 = none */
+    const float *class_horton;        /* [256*3] Horton f0, fc (m/s), k (1/h) per land class */
+    const int32_t *rain_index;        /* cell -> rain pixel, read only when n_pixels > 1 */
     const float *bweight;             /* discharge share of the boundary */
     const int32_t *csr_start;         /* first entry of the cell's l, r, b, t side list */
     const uint32_t *csr_count;        /* side counts: l | r << 8 | b << 16 | t << 24 */
@@ -90,6 +91,7 @@ typedef struct nhf_mesh {
     const float *open_face;           /* [e_num*4] exterior open fraction, NULL when sita == 1 */
     const float *x, *y;               /* cell centres, NULL on aligned meshes */
     const int32_t *isl;               /* [r_len] side ids */
+    const int32_t *isl_cell;          /* [r_len] the cell across each listed side, 0 = exterior */
     /* per-side */
     float *q;                         /* unit discharge (m^2/s), state */
     const int32_t *side_cells;        /* [s_num*2] lower, higher */
@@ -169,7 +171,7 @@ int32_t nhf_sync(nhf_ctx *ctx);
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable);
 
 /* Per-kernel timing with HIP events on the context's stream (eager mode only). Kernel ids:
- * 0 prepare, 1 gradients, 2 face averages, 3 flux, 4 donor, 5 depth, 6 cfl. `ms` and `count`
+ * 0 prepare, 1 gradients, 2 face averages, 3 flux, 4 donor, 5 depth, 6 cfl, 7 finish. `ms` and `count`
  * receive totals since the last call with reset != 0. */
 int32_t nhf_set_timing(nhf_ctx *ctx, int32_t enable);
 int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);

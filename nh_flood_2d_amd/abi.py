@@ -27,11 +27,10 @@ MESH_FIELDS = [
     ('n_pixels', 'i'), ('n_grad', 'i'), ('n_hang', 'i'), ('flags', 'i'),
     ('depth', 'p'), ('bed', 'p'), ('rain_time', 'p'), ('capacity', 'p'),
     ('max_depth', 'p'), ('max_speed', 'p'), ('peak_tick', 'p'),
-    ('area', 'p'), ('width', 'p'), ('height', 'p'),
-    ('f0', 'p'), ('fc', 'p'), ('hk', 'p'),
-    ('rain_index', 'p'), ('boundary_id', 'p'), ('bweight', 'p'),
+    ('area', 'p'), ('size', 'p'), ('cell_meta', 'p'), ('class_horton', 'p'),
+    ('rain_index', 'p'), ('bweight', 'p'),
     ('csr_start', 'p'), ('csr_count', 'p'), ('pipe_q', 'p'), ('open_face', 'p'),
-    ('x', 'p'), ('y', 'p'), ('isl', 'p'),
+    ('x', 'p'), ('y', 'p'), ('isl', 'p'), ('isl_cell', 'p'),
     ('q', 'p'), ('side_cells', 'p'), ('side_meta', 'p'),
     ('length', 'p'), ('sdc', 'p'), ('manning', 'p'), ('side_key', 'p'),
     ('edge_index', 'p'), ('hang_offset', 'p'), ('hang_slot', 'p'),

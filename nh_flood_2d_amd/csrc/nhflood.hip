@@ -58,7 +58,7 @@ struct Ctl {
     uint64_t now, next, origin, step_end;
     float dt, local, end_local, ndt;
     int32_t active, cfl_arg, raining, slot;
-    uint32_t ticket, limited_cells, limited_faces, min_scale_bits;
+    uint32_t limited_cells, limited_faces, min_scale_bits, pad0;
     float limit_value[5];
     int32_t limit_ids[3];
     uint32_t trace[NHF_MAX_BATCH * NHF_TRACE_COLS];
@@ -81,6 +81,17 @@ __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
 
 __device__ __forceinline__ int face_n(uint32_t counts, int d) { return (int)((counts >> (8 * d)) & 255u); }
+
+// Workgroups are dealt to the 8 XCDs round-robin. For a grid that is a multiple of 8, this maps
+// the hardware order so XCD x runs the contiguous x-th eighth of the logical blocks: Morton-near
+// cells then share one XCD's L2 instead of being spread over all eight.
+__device__ __forceinline__ int xcd_block() {
+    const int per = (int)(gridDim.x >> 3);
+    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ int meta_boundary(uint32_t meta) { return (int)((meta >> 8) & 255u) - 1; }
+__device__ __forceinline__ int meta_class(uint32_t meta) { return (int)(meta & 255u); }
 
 // --------------------------------------------------------------------------- x^y and e^x
 // Evaluated in FP64 and rounded once to F32, so the result is the same on any IEEE machine:
@@ -303,12 +314,11 @@ __global__ void k_reset(Dev d, uint64_t now) {
         c->dt = c->local = c->end_local = 0.0f;
         c->active = 0;
         c->slot = 0;
-        c->ticket = 0;
         c->limited_cells = c->limited_faces = 0;
         c->min_scale_bits = __float_as_uint(1.0f);
         c->limit_value[0] = INFINITY;
     }
-    for (int i = tid; i < d.depth_blocks * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i < d.depth_blocks * NHF_WAVES * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
     for (int i = tid; i < d.m.e_num; i += stride) d.donor[i] = 0.0f;
     for (int i = tid; i < d.m.s_num; i += stride) d.qflux[i] = 0.0f;
     if (d.face_qn)
@@ -329,13 +339,16 @@ __global__ void k_set_raining(Dev d, int raining) {
     if (threadIdx.x == 0) d.ctl->raining = raining;
 }
 
+// First slot of a batch: clear the trace rows, then prepare slot 0.
 __global__ void k_prepare(Dev d, int slot) {
+    for (int i = threadIdx.x; i < NHF_MAX_BATCH * NHF_TRACE_COLS; i += blockDim.x) d.ctl->trace[i] = 0u;
+    __syncthreads();
     if (threadIdx.x == 0) prepare_slot(d, slot);
 }
 
 __global__ void k_clear_acc(Dev d) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int i = tid; i < d.depth_blocks * d.terms; i += gridDim.x * blockDim.x)
+    for (int i = tid; i < d.depth_blocks * NHF_WAVES * d.terms; i += gridDim.x * blockDim.x)
         d.acc[i] = make_double2(0.0, 0.0);
 }
 
@@ -381,7 +394,7 @@ __device__ void face_difference(const Dev &d, int cell, int dir, int axis, float
     int n = face_n(cnt, dir);
     for (int j = 0; j < n; ++j) {
         int side = m.isl[start + j];
-        int other = m.side_cells[2 * side] + m.side_cells[2 * side + 1] - cell;
+        int other = m.isl_cell[start + j];
         if (other <= 0 || m.depth[other] <= d.p.min_h) {
             ok = 0;
         } else {
@@ -477,7 +490,7 @@ __device__ __forceinline__ float neighbour_sum(float q, float left, float right,
 
 __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
     if (!d.ctl->active) return;
-    int si = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
+    int si = xcd_block() * NHF_BLOCK + threadIdx.x + 1;
     if (si >= d.m.s_num) return;
     const nhf_mesh &m = d.m;
     const float dt = d.ctl->dt;
@@ -522,13 +535,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
 __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
     Ctl *c = d.ctl;
     if (!c->active) return;
-    __shared__ uint32_t s_count[NHF_WAVES], s_min[NHF_WAVES];
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
-    int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
+    int cell = xcd_block() * NHF_BLOCK + threadIdx.x + 1;
     uint32_t limited = 0, scale_bits = __float_as_uint(1.0f);
     if (cell < m.e_num) {
-        int bi = m.boundary_id[cell];
+        int bi = meta_boundary(m.cell_meta[cell]);
         float discharge = 0.0f;
         bool level = false;
         if (bi >= 0) {
@@ -536,7 +548,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
             if (kind == 1) discharge = d.values[bi] * m.bweight[cell];
             level = kind == 0;
         }
-        float rainq = c->raining ? d.rain_rates[m.rain_index[cell]] : 0.0f;
+        float rainq = c->raining ? d.rain_rates[m.n_pixels > 1 ? m.rain_index[cell] : 0] : 0.0f;
         float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
         float area = m.area[cell];
         float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
@@ -561,22 +573,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
         }
         d.donor[cell] = ds;
     }
-    uint32_t wc = wave_sum_u(limited), wm = wave_min_u(scale_bits);
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        s_count[wave] = wc;
-        s_min[wave] = wm;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tc = 0, tm = __float_as_uint(1.0f);
-        for (int w = 0; w < NHF_WAVES; ++w) {
-            tc += s_count[w];
-            tm = s_min[w] < tm ? s_min[w] : tm;
-        }
-        if (tc) {
-            atomicAdd(&c->limited_cells, tc);
-            atomicMin(&c->min_scale_bits, tm);
+    // limiter counters: one integer atomic per wavefront that limited a cell (rare)
+    if (__any(limited)) {
+        uint32_t wc = wave_sum_u(limited), wm = wave_min_u(scale_bits);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&c->limited_cells, wc);
+            atomicMin(&c->min_scale_bits, wm);
         }
     }
 }
@@ -596,12 +598,10 @@ __device__ __forceinline__ void acc_add(double2 &a, double v) {
 __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
     Ctl *c = d.ctl;
     if (!c->active) return;
-    __shared__ double s_red[NHF_WAVES];
-    __shared__ uint32_t s_faces[NHF_WAVES];
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
     const int b_num = m.b_num;
-    int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
+    int cell = xcd_block() * NHF_BLOCK + threadIdx.x + 1;
     bool live = cell < m.e_num;
     // per-cell budget contributions (solver_compact.py:533, 578-590)
     float t_infil = 0.0f, t_floor = 0.0f, t_level_floor = 0.0f, t_pipe = 0.0f, t_disc = 0.0f;
@@ -609,7 +609,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
     int bi = -1, kind = -1;
     uint32_t faces_limited = 0;
     if (live) {
-        bi = m.boundary_id[cell];
+        const uint32_t meta = m.cell_meta[cell];
+        bi = meta_boundary(meta);
         if (bi >= 0) kind = m.kind[bi];
         float ds = d.donor[cell];
         float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
@@ -621,13 +622,14 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
             t_disc = discharge * dt;
         }
         float ssq = pipe + discharge;
-        float rainq = c->raining ? d.rain_rates[m.rain_index[cell]] : 0.0f;
+        float rainq = c->raining ? d.rain_rates[m.n_pixels > 1 ? m.rain_index[cell] : 0] : 0.0f;
         float cap;
         if (rainq > 0.0f) {
             float rt = m.rain_time[cell] + dt;
             m.rain_time[cell] = rt;
-            float f0 = m.f0[cell], fc = m.fc[cell];
-            cap = fc + (f0 - fc) * det_expf(-m.hk[cell] * rt / 3600.0f);
+            const float *hp = m.class_horton + 3 * meta_class(meta);
+            float f0 = hp[0], fc = hp[1], hk = hp[2];
+            cap = fc + (f0 - fc) * det_expf(-hk * rt / 3600.0f);
             m.capacity[cell] = cap;
         } else {
             cap = m.capacity[cell];
@@ -640,16 +642,16 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
         for (int dir = 0; dir < 4; ++dir) {
             int n = face_n(cnt, dir);
             float acc = 0.0f;
+            const bool upper = dir & 1;  // right/top faces: this cell is the side's lower cell
             for (int j = 0; j < n; ++j, ++k) {
                 int side = m.isl[k];
+                int nb = m.isl_cell[k];
                 float qf = d.qflux[side];
-                int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[side];
-                int donor_cell = qf >= 0.0f ? lh.x : lh.y;
+                int donor_cell = (qf >= 0.0f) == upper ? cell : nb;
                 float scale = d.donor[donor_cell];
                 float qs = qf * scale;
                 acc += qs * m.length[side];
-                int owner = lh.x != 0 ? lh.x : lh.y;
-                if (owner == cell) {
+                if (upper || nb == 0) {  // the side's single writer
                     m.q[side] = qs;
                     if (qf != 0.0f && scale < 1.0f) faces_limited += 1;
                 }
@@ -686,23 +688,25 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
         float hv = fmx(h_new, 0.01f);
         float u = 0.0f, v = 0.0f;
         if (h_new >= d.p.min_h) {
-            u = (ql + qr) / m.height[cell] / hv / 2.0f;
-            v = (qb + qt) / m.width[cell] / hv / 2.0f;
+            const float size = m.size[cell];
+            u = (ql + qr) / size / hv / 2.0f;
+            v = (qb + qt) / size / hv / 2.0f;
         }
         float speed = sqrtf(u * u + v * v);
-        float ms = m.max_speed[cell];
-        m.max_speed[cell] = fmx(ms, speed);
+        if (speed > m.max_speed[cell]) m.max_speed[cell] = speed;
         if (h_new > m.max_depth[cell]) {
             m.max_depth[cell] = h_new;
             m.peak_tick[cell] = c->step_end;
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // limited-face count
-    uint32_t wf = wave_sum_u(faces_limited);
-    if (lane == 0) s_faces[wave] = wf;
-    // FP64 budget: one block reduction per term that is nonzero somewhere in the block
-    double2 *acc = d.acc + (size_t)blockIdx.x * d.terms;
+    if (__any(faces_limited)) {
+        uint32_t wf = wave_sum_u(faces_limited);
+        if (lane == 0) atomicAdd(&c->limited_faces, wf);
+    }
+    // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
+    // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it.
+    double2 *acc = d.acc + ((size_t)blockIdx.x * NHF_WAVES + wave) * d.terms;
     for (int t = 0; t < d.terms; ++t) {
         float v = 0.0f;
         if (t == 0)
@@ -719,50 +723,42 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
         } else if (kind == 1 && bi == t - 4 - 5 * b_num) {
             v = t_disc;
         }
-        if (!__syncthreads_or(v != 0.0f)) continue;
+        if (!__any(v != 0.0f)) continue;
         double w = wave_sum((double)v);
-        if (lane == 0) s_red[wave] = w;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double total = 0.0;
-            for (int i = 0; i < NHF_WAVES; ++i) total += s_red[i];
+        if (lane == 0) {
             double2 a = acc[t];
-            acc_add(a, total);
+            acc_add(a, w);
             acc[t] = a;
         }
-        __syncthreads();
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tf = 0;
-        for (int i = 0; i < NHF_WAVES; ++i) tf += s_faces[i];
-        if (tf) atomicAdd(&c->limited_faces, tf);
     }
 }
 
 // --------------------------------------------------------------------------- (c) CFL dt
 
-// mode 0: standalone compute_cfl (state only). mode 1: inside a batch; the last workgroup also
-// commits `slot` and prepares `slot + 1` when it is below `steps`.
-__global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int slot, int steps) {
+// Per-workgroup CFL minimum over a grid-stride range of sides. The key orders by the F32 bits of
+// the bound (non-negative, so bit order is numeric order), then by the reference side id, which
+// makes the argmin the smallest reference id among ties (solver_compact.py:633-641).
+__global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode) {
     Ctl *c = d.ctl;
     if (mode == 1 && !c->active) return;
     __shared__ uint64_t s_key[NHF_WAVES];
     __shared__ int32_t s_idx[NHF_WAVES];
-    __shared__ int s_last;
     const nhf_mesh &m = d.m;
-    int si = blockIdx.x * NHF_BLOCK + threadIdx.x + 1;
     uint64_t key = ~0ull;
     int32_t idx = 0;
-    if (si < m.s_num) {
+    const float afa = d.p.afa, g = d.p.g;
+    for (int si = xcd_block() * NHF_BLOCK + threadIdx.x + 1; si < m.s_num; si += gridDim.x * NHF_BLOCK) {
         int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
         if (lh.x != 0 && lh.y != 0) {
             float zl = m.bed[lh.x], zh = m.bed[lh.y];
             float head = fmx(zl + m.depth[lh.x], zh + m.depth[lh.y]);
             float h = fmx(head - fmx(fmx(zl, zh), -999.0f), 0.01f);
-            float cfl = d.p.afa * m.length[si] / (sqrtf(d.p.g * h) + fabsf(m.q[si]) / h);
-            key = ((uint64_t)__float_as_uint(cfl) << 32) | m.side_key[si];
-            idx = si;
+            float cfl = afa * m.length[si] / (sqrtf(g * h) + fabsf(m.q[si]) / h);
+            uint64_t k2 = ((uint64_t)__float_as_uint(cfl) << 32) | m.side_key[si];
+            if (k2 < key) {
+                key = k2;
+                idx = si;
+            }
         }
     }
     wave_min_key(key, idx);
@@ -780,38 +776,38 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int slot, in
             }
         d.cfl_key[blockIdx.x] = key;
         d.cfl_idx[blockIdx.x] = idx;
-        __threadfence();
-        uint32_t ticket = atomicAdd(&c->ticket, 1u);
-        s_last = ticket == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!s_last) return;
-    // last workgroup: reduce the per-block partials
-    __threadfence();
-    key = ~0ull;
-    idx = 0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += NHF_BLOCK) {
-        uint64_t k2 = __atomic_load_n(&d.cfl_key[b], __ATOMIC_RELAXED);
-        int32_t i2 = __atomic_load_n(&d.cfl_idx[b], __ATOMIC_RELAXED);
+}
+
+// One workgroup: reduce the CFL partials to ndt and its side; in a batch (mode 1) also commit
+// `slot` (finish_step) and prepare `slot + 1` (prepare_step) when it is below `steps`.
+__global__ __launch_bounds__(1024) void k_finish(Dev d, int mode, int slot, int steps) {
+    Ctl *c = d.ctl;
+    if (mode == 1 && !c->active) return;
+    __shared__ uint64_t s_key[16];
+    __shared__ int32_t s_idx[16];
+    uint64_t key = ~0ull;
+    int32_t idx = 0;
+    for (int b = threadIdx.x; b < d.cfl_blocks; b += blockDim.x) {
+        uint64_t k2 = d.cfl_key[b];
         if (k2 < key) {
             key = k2;
-            idx = i2;
+            idx = d.cfl_idx[b];
         }
     }
     wave_min_key(key, idx);
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {
         s_key[wave] = key;
         s_idx[wave] = idx;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < NHF_WAVES; ++w)
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
             if (s_key[w] < key) {
                 key = s_key[w];
                 idx = s_idx[w];
             }
-        c->ticket = 0;
         if (key == ~0ull) {
             c->ndt = INFINITY;
             c->cfl_arg = 0;
@@ -828,7 +824,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int slot, in
 
 // --------------------------------------------------------------------------- host side
 
-enum { K_PREPARE = 0, K_GRAD, K_FACE, K_FLUX, K_DONOR, K_DEPTH, K_CFL, K_COUNT };
+enum { K_PREPARE = 0, K_GRAD, K_FACE, K_FLUX, K_DONOR, K_DEPTH, K_CFL, K_FINISH, K_COUNT };
 
 struct nhf_ctx {
     Dev d;
@@ -852,13 +848,22 @@ struct Carve {
 
 static int blocks_for(int n) { return (n + NHF_BLOCK - 1) / NHF_BLOCK; }
 
+// Grids of the XCD-mapped kernels are rounded up to a multiple of the 8 XCDs.
+static int blocks_xcd(int n) { return (blocks_for(n) + 7) & ~7; }
+
+// CFL partial count: enough workgroups to fill 256 CUs, few enough for one finishing workgroup.
+static int cfl_grid(int s_num) {
+    int n = blocks_xcd(s_num - 1 > 0 ? s_num - 1 : 1);
+    return n < 2048 ? n : 2048;
+}
+
 static Carve carve(const nhf_mesh *m, const nhf_params *p) {
     Carve c;
     size_t off = 0;
     int b = m->b_num > 0 ? m->b_num : 1;
     int terms = 4 + 6 * m->b_num;
-    int cfl_blocks = blocks_for(m->s_num - 1 > 0 ? m->s_num - 1 : 1);
-    int depth_blocks = blocks_for(m->e_num - 1 > 0 ? m->e_num - 1 : 1);
+    int cfl_blocks = cfl_grid(m->s_num);
+    int depth_blocks = blocks_xcd(m->e_num - 1 > 0 ? m->e_num - 1 : 1);
     bool theta_one = p->theta == 1.0f;
     c.ctl = off; off = align_up(off + sizeof(Ctl));
     c.coeff = off; off = align_up(off + sizeof(float) * 2 * b);
@@ -871,7 +876,7 @@ static Carve carve(const nhf_mesh *m, const nhf_params *p) {
     c.grad = off; off = align_up(off + sizeof(float) * 2 * (m->n_grad > 0 ? m->n_grad : 1));
     c.key = off; off = align_up(off + sizeof(uint64_t) * cfl_blocks);
     c.idx = off; off = align_up(off + sizeof(int32_t) * cfl_blocks);
-    c.acc = off; off = align_up(off + sizeof(double2) * (size_t)depth_blocks * terms);
+    c.acc = off; off = align_up(off + sizeof(double2) * (size_t)depth_blocks * NHF_WAVES * terms);
     c.total = off;
     return c;
 }
@@ -918,8 +923,8 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     d.cfl_idx = (int32_t *)(ws + cv.idx);
     d.acc = (double2 *)(ws + cv.acc);
     d.terms = 4 + 6 * mesh->b_num;
-    d.cfl_blocks = blocks_for(mesh->s_num - 1 > 0 ? mesh->s_num - 1 : 1);
-    d.depth_blocks = blocks_for(mesh->e_num - 1);
+    d.cfl_blocks = cfl_grid(mesh->s_num);
+    d.depth_blocks = blocks_xcd(mesh->e_num - 1);
     d.theta_one = params->theta == 1.0f;
     ctx->stream = (hipStream_t)stream;
     ctx->graph_enabled = 0;
@@ -1011,10 +1016,11 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
     for (int slot = 0; slot < steps; ++slot) {
         if (d.m.n_grad > 0) LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
         if (!d.theta_one) LAUNCH(K_FACE, k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
-        if (sides > 0) LAUNCH(K_FLUX, k_flux, dim3(blocks_for(sides)), dim3(NHF_BLOCK), 0, s, d);
-        LAUNCH(K_DONOR, k_donor, dim3(blocks_for(cells)), dim3(NHF_BLOCK), 0, s, d);
+        if (sides > 0) LAUNCH(K_FLUX, k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d);
+        LAUNCH(K_DONOR, k_donor, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
         LAUNCH(K_DEPTH, k_depth, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
-        LAUNCH(K_CFL, k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, slot, steps);
+        LAUNCH(K_CFL, k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
+        LAUNCH(K_FINISH, k_finish, dim3(1), dim3(1024), 0, s, d, 1, slot, steps);
     }
 #undef LAUNCH
     HIP_TRY(hipGetLastError());
@@ -1058,14 +1064,16 @@ int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
 
 int32_t nhf_compute_cfl(nhf_ctx *ctx) {
     const Dev &d = ctx->d;
-    hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0, 0);
+    hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, ctx->stream, d, 0, 0, 0);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
 int32_t nhf_flush_budget(nhf_ctx *ctx, double *out) {
     const Dev &d = ctx->d;
-    size_t n = (size_t)d.depth_blocks * d.terms;
+    const int owners = d.depth_blocks * NHF_WAVES;
+    size_t n = (size_t)owners * d.terms;
     if (!ctx->host_acc) HIP_TRY(hipHostMalloc((void **)&ctx->host_acc, sizeof(double2) * n, hipHostMallocDefault));
     HIP_TRY(hipMemcpyAsync(ctx->host_acc, d.acc, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1074,7 +1082,7 @@ int32_t nhf_flush_budget(nhf_ctx *ctx, double *out) {
     for (int t = 0; t < d.terms; ++t) {
         // two-sum accumulation of every block's (sum, compensation) pair
         double hi = 0.0, lo = 0.0;
-        for (int b = 0; b < d.depth_blocks; ++b) {
+        for (int b = 0; b < owners; ++b) {
             const double2 a = ctx->host_acc[(size_t)b * d.terms + t];
             const double parts[2] = {a.x, a.y};
             for (double x : parts) {

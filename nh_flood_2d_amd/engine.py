@@ -196,9 +196,9 @@ class Engine:
         self.lib.nhf_set_timing(self.ctx, int(enable))
 
     def kernel_times(self, reset: bool = True):
-        ms = np.zeros(7, dtype=np.float64)
-        count = np.zeros(7, dtype=np.int64)
-        self.lib.nhf_kernel_times(self.ctx, ms.ctypes.data, count.ctypes.data, 7, int(reset))
+        ms = np.zeros(8, dtype=np.float64)
+        count = np.zeros(8, dtype=np.int64)
+        self.lib.nhf_kernel_times(self.ctx, ms.ctypes.data, count.ctypes.data, 8, int(reset))
         return ms, count
 
     def synchronize(self) -> None:

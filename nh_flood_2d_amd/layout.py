@@ -80,6 +80,7 @@ class Layout:
     n_grad: int = 0
     n_hang: int = 0
     theta: float = 1.0
+    cell_f0: np.ndarray | None = None
 
     # --- permutations between the reference numbering and the device order
     def cells_to_device(self, a: np.ndarray) -> np.ndarray:
@@ -130,19 +131,43 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
     packed = (counts[:, 0].astype(np.uint32) | (counts[:, 1].astype(np.uint32) << 8) |
               (counts[:, 2].astype(np.uint32) << 16) | (counts[:, 3].astype(np.uint32) << 24))
 
+    # the cell across each listed side: a cell is the higher cell of its left/bottom sides and the
+    # lower cell of its right/top sides (the reference's face convention, checked here)
+    owner_cell = np.repeat(np.arange(e_num), totals)
+    face_dir = np.concatenate([np.repeat(np.arange(4), c) for c in counts]) if r_len else np.zeros(0, int)
+    ref_sides = mesh.isl[gather] if r_len else np.zeros(0, np.int64)
+    lower_of, higher_of = topo[ref_sides, 1], topo[ref_sides, 2]
+    upper_face = (face_dir & 1) == 1
+    expected_self = np.where(upper_face, lower_of, higher_of)
+    if np.any((ref_sides != 0) & (cperm[owner_cell] != expected_self)):
+        raise ValueError('a side is listed on a face whose cell is not its left/bottom or right/top neighbour')
+    isl_cell = cinv[np.where(upper_face, higher_of, lower_of)].astype(np.int32) if r_len else np.zeros(1, np.int32)
+
     side_cells = np.empty((s_num, 2), dtype=np.int32)
     side_cells[:, 0] = cinv[topo[sperm, 1]]
     side_cells[:, 1] = cinv[topo[sperm, 2]]
     meta = topo[sperm, 0].astype(np.uint8) | (mesh.sbf[sperm] * 4).astype(np.uint8)
 
+    # Horton parameters are per land class (map_land_parameters assigns class constants), so the
+    # cells carry their class and the kernels read a 256-entry table; cell 0 keeps zeros.
+    land = mesh.land.astype(np.int64)
+    horton = np.zeros((256, 3), dtype=np.float32)
+    for cls in np.unique(land[1:]):
+        first = 1 + int(np.flatnonzero(land[1:] == cls)[0])
+        horton[cls] = (f0[first], fc[first], hk[first])
+    if np.any(horton[land[1:], 0] != f0[1:]) or np.any(horton[land[1:], 2] != hk[1:]):
+        raise ValueError('Horton parameters must be constant per land class')
+    bid = forcing.boundary_id.astype(np.int64)
+    if np.any(bid > 253):
+        raise ValueError('at most 254 prescribed boundaries are supported')
+    meta_cells = (land | ((bid + 1) << 8)).astype(np.uint32)
+    meta_cells[0] = 0
     arrays = dict(
-        area=area.astype(np.float32)[cperm], width=widths.astype(np.float32)[cperm],
-        height=heights.astype(np.float32)[cperm],
-        f0=f0[cperm], fc=fc[cperm], hk=hk[cperm],
+        area=area.astype(np.float32)[cperm], size=widths.astype(np.float32)[cperm],
+        cell_meta=meta_cells[cperm], class_horton=horton.ravel(),
         rain_index=forcing.rain_pixel.astype(np.int32)[cperm],
-        boundary_id=forcing.boundary_id.astype(np.int32)[cperm],
         bweight=forcing.bweight.astype(np.float32)[cperm],
-        csr_start=start.astype(np.int32), csr_count=packed.astype(np.uint32), isl=isl,
+        csr_start=start.astype(np.int32), csr_count=packed.astype(np.uint32), isl=isl, isl_cell=isl_cell,
         side_cells=side_cells.ravel(), side_meta=meta,
         length=mesh.length[sperm].astype(np.float32), sdc=sdc[sperm],
         manning=side_n[sperm], side_key=sperm.astype(np.uint32),
@@ -150,6 +175,7 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
     )
     layout = Layout(e_num, s_num, cperm, cinv, sperm, sinv, arrays, forcing.n_pixels,
                     len(forcing.kinds), 0, 0, float(theta))
+    layout.cell_f0 = f0.astype(np.float32)
     if theta != 1.:
         fr = open_face_fractions(forcing.selected_sides, mesh.length, topo, mesh.face_ptr,
                                  mesh.face_count, mesh.isl)
@@ -172,15 +198,9 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
 def initial_state(mesh: Mesh, layout: Layout) -> dict:
     """Cold-start state in original numbering (``initialize``, solver_compact.py:398-410)."""
     e_num, s_num = mesh.e_num, mesh.s_num
-    f0 = map_land_parameters_cached(layout)
+    f0 = layout.cell_f0.copy()
+    f0[0] = 0.
     return dict(depth=np.zeros(e_num, np.float32), bed=mesh.z.astype(np.float32).copy(),
                 rain_time=np.zeros(e_num, np.float32), capacity=f0,
                 max_depth=np.zeros(e_num, np.float32), max_speed=np.zeros(e_num, np.float32),
                 peak_tick=np.zeros(e_num, np.uint64), q=np.zeros(s_num, np.float32))
-
-
-def map_land_parameters_cached(layout: Layout) -> np.ndarray:
-    f0 = layout.cells_from_device(layout.arrays['f0'])
-    f0 = f0.copy()
-    f0[0] = 0.
-    return f0

@@ -171,6 +171,10 @@ size_t nhf_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params) {
     return 0;
 }
 
+static int cell_boundary(const nhf_ctx *c, int ei) { return (int)((c->m.cell_meta[ei] >> 8) & 255u) - 1; }
+static int cell_class(const nhf_ctx *c, int ei) { return (int)(c->m.cell_meta[ei] & 255u); }
+static int cell_pixel(const nhf_ctx *c, int ei) { return c->m.n_pixels > 1 ? c->m.rain_index[ei] : 0; }
+
 static int face_count(const nhf_ctx *c, int ei, int d) {
     return (int)((c->m.csr_count[ei] >> (8 * d)) & 255u);
 }
@@ -624,11 +628,11 @@ static void tick(nhf_ctx *c) {
     real min_scale = c->min_scale;
 #pragma omp parallel for schedule(static) reduction(+ : limited_cells) reduction(min : min_scale)
     for (int ei = 1; ei < e_num; ++ei) {
-        int bi = m->boundary_id[ei];
+        int bi = cell_boundary(c, ei);
         real discharge = 0;
         if (bi >= 0 && m->kind[bi] == 1) discharge = c->values[bi] * (real)m->bweight[ei];
         real rainq = 0;
-        if (c->raining) rainq = (real)c->rain_rates[m->rain_index[ei]];
+        if (c->raining) rainq = (real)c->rain_rates[cell_pixel(c, ei)];
         real pq = m->pipe_q ? (real)m->pipe_q[ei] : (real)0.0;
         real positive = (rmax(pq, 0) + rmax(discharge, 0)) * dt;
         real available = c->depth[ei] * (real)m->area[ei] + positive + rainq * dt * (real)m->area[ei];
@@ -666,7 +670,7 @@ static void tick(nhf_ctx *c) {
     /* sources, Horton capacity and face totals (:525-566) */
 #pragma omp parallel for schedule(static)
     for (int ei = 1; ei < e_num; ++ei) {
-        int bi = m->boundary_id[ei];
+        int bi = cell_boundary(c, ei);
         real ds = c->donor[ei];
         real pq = m->pipe_q ? (real)m->pipe_q[ei] : (real)0.0;
         real pipe = rmax(pq, 0) + rmin(pq, 0) * ds;
@@ -677,11 +681,12 @@ static void tick(nhf_ctx *c) {
         }
         c->ssq[ei] = pipe + discharge;
         real rainq = 0;
-        if (c->raining) rainq = (real)c->rain_rates[m->rain_index[ei]];
+        if (c->raining) rainq = (real)c->rain_rates[cell_pixel(c, ei)];
         if (rainq > 0) {
             c->rain_time[ei] += dt;
-            c->capacity[ei] = (real)m->fc[ei] + ((real)m->f0[ei] - (real)m->fc[ei]) *
-                                                    REXP(-(real)m->hk[ei] * c->rain_time[ei] / (real)3600.0);
+            const float *hp = &m->class_horton[3 * cell_class(c, ei)];
+            real f0 = hp[0], fc = hp[1], hk = hp[2];
+            c->capacity[ei] = fc + (f0 - fc) * REXP(-hk * c->rain_time[ei] / (real)3600.0);
         }
         real f[4];
         for (int d = 0; d < 4; ++d) {
@@ -701,7 +706,7 @@ static void tick(nhf_ctx *c) {
     }
     /* discharge diagnostic in cell order (:533) */
     for (int ei = 1; ei < e_num; ++ei) {
-        int bi = m->boundary_id[ei];
+        int bi = cell_boundary(c, ei);
         if (bi >= 0 && m->kind[bi] == 1) {
             real requested = c->values[bi] * (real)m->bweight[ei];
             real ds = c->donor[ei];
@@ -714,7 +719,7 @@ static void tick(nhf_ctx *c) {
     const uint64_t step_end = c->step_end;
 #pragma omp parallel for schedule(static)
     for (int ei = 1; ei < e_num; ++ei) {
-        int bi = m->boundary_id[ei];
+        int bi = cell_boundary(c, ei);
         real ea = (real)m->area[ei], previous = c->depth[ei];
         real pq = m->pipe_q ? (real)m->pipe_q[ei] : (real)0.0;
         real pipe = rmax(pq, 0) + rmin(pq, 0) * c->donor[ei];
@@ -743,8 +748,8 @@ static void tick(nhf_ctx *c) {
         real h = rmax(c->depth[ei], (real)0.01);
         real u = 0, v = 0;
         if (c->depth[ei] >= min_h) {
-            u = (ql + qr) / (real)m->height[ei] / h / (real)2.0;
-            v = (qb + qt) / (real)m->width[ei] / h / (real)2.0;
+            u = (ql + qr) / (real)m->size[ei] / h / (real)2.0;
+            v = (qb + qt) / (real)m->size[ei] / h / (real)2.0;
         }
         c->max_speed[ei] = rmax(c->max_speed[ei], RSQRT(u * u + v * v));
         if (c->depth[ei] > c->max_depth[ei]) {
@@ -755,7 +760,7 @@ static void tick(nhf_ctx *c) {
     /* budget terms, added per cell in cell order (:578-590) */
     for (int ei = 1; ei < e_num; ++ei) {
         const real *dg = &c->diag[ei * 8];
-        int bi = m->boundary_id[ei];
+        int bi = cell_boundary(c, ei);
         add_diag(c, 0, dg[0]);
         add_diag(c, 3, dg[2]);
         if (bi >= 0 && m->kind[bi] == 0) {
@@ -814,6 +819,8 @@ static void finish_step(nhf_ctx *c) {
 
 int32_t nhf_run_batch(nhf_ctx *c, int32_t steps, uint32_t *trace) {
     if (steps < 1 || steps > c->p.batch_capacity) return fail("steps out of range");
+    c->slot = 0;
+    memset(c->trace, 0, sizeof c->trace);
     for (int s = 0; s < steps; ++s) {
         prepare_step(c);
         if (c->active) {
@@ -932,7 +939,10 @@ int32_t nhf_oracle_pressure(const nhf_mesh *mesh, float min_h, int32_t n_probe,
     p.min_h = min_h;
     p.batch_capacity = 1;
     if (nhf_create(&c, mesh, &p, NULL, 0, NULL)) return -1;
-    load_state(c);
+    for (int i = 0; i < mesh->e_num; ++i) {
+        c->depth[i] = mesh->depth[i];
+        c->bed[i] = mesh->bed[i];
+    }
     update_pressure(c);
     for (int i = 0; i < 2 * mesh->n_grad; ++i) gradients_out[i] = (float)c->gradients[i];
     for (int i = 0; i < n_probe; ++i) {

@@ -316,8 +316,10 @@ def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, 
                 raise ValueError('Checkpoint clock or physical parameters differ from configuration')
             if meta.get('budget', {}).get('device_terms_exact') is None:
                 raise ValueError('Checkpoint budget was not written by this solver')
-    stations = resolve_station_cells(domain_cfg.stations, mesh)
     origin_before_start = meta['clock_origin_before_start_s'] if continuation else domain_cfg.warmup_duration
+    if state is not None and ((state.clock_hi << 32) | state.clock_lo) > seconds_to_ticks(end_time + origin_before_start):
+        raise ValueError('Checkpoint time is later than the requested simulation end')
+    stations = resolve_station_cells(domain_cfg.stations, mesh)
     settings = RunSettings(afa=domain_cfg.afa, sita=domain_cfg.sita, min_h=domain_cfg.min_h, end_time=end_time,
                            warmup_duration=float(domain_cfg.warmup_duration), yield_step=float(domain_cfg.yield_step),
                            start_text=domain_cfg.start.isoformat(), stations=stations)

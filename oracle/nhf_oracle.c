@@ -109,6 +109,7 @@ static float det_expf(float x) { return (float)det_exp2((double)x * 1.4426950408
 #endif
 
 #define TICKS_PER_SECOND 1099511627776.0 /* 2^40 */
+#define OMP_MIN_ITEMS 32768                /* loops shorter than this run on one thread */
 #define SNAP_TICKS 1099512ull            /* ceil(1e-6 * 2^40), clock.py:11 */
 
 static __thread char g_error[256];
@@ -512,7 +513,7 @@ static real gradient(const nhf_ctx *c, int cell, int axis) {
 /* update_pressure / update_cell (solver_compact.py:346-349, pressure.py:108-116) */
 static void update_pressure(nhf_ctx *c) {
     const nhf_mesh *m = &c->m;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (m->n_grad > OMP_MIN_ITEMS)
     for (int slot = 0; slot < m->n_grad; ++slot) {
         int cell = m->grad_cell[slot];
         for (int axis = 0; axis < 2; ++axis) {
@@ -587,14 +588,14 @@ static void tick(nhf_ctx *c) {
 
     /* face averages of the previous discharge (:430-436) */
     if (!theta_one) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (e_num > OMP_MIN_ITEMS)
         for (int ei = 0; ei < e_num; ++ei)
             for (int d = 0; d < 4; ++d)
                 c->face_qn[ei * 4 + d] = face_discharge(c, face_start(c, ei, d), face_count(c, ei, d));
     }
 
     /* momentum on every side (:439-486) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s_num > OMP_MIN_ITEMS)
     for (int si = 1; si < s_num; ++si) {
         real bf = (m->side_meta[si] & NHF_SIDE_EXTERIOR) ? (real)0.0 : (real)1.0;
         int orient = m->side_meta[si] & NHF_SIDE_ORIENT_MASK;
@@ -626,7 +627,7 @@ static void tick(nhf_ctx *c) {
     /* donor scale per cell (:491-514) */
     uint32_t limited_cells = 0;
     real min_scale = c->min_scale;
-#pragma omp parallel for schedule(static) reduction(+ : limited_cells) reduction(min : min_scale)
+#pragma omp parallel for schedule(static) reduction(+ : limited_cells) reduction(min : min_scale) if (e_num > OMP_MIN_ITEMS)
     for (int ei = 1; ei < e_num; ++ei) {
         int bi = cell_boundary(c, ei);
         real discharge = 0;
@@ -658,7 +659,7 @@ static void tick(nhf_ctx *c) {
 
     /* shared-subedge scaling (:516-524) */
     uint32_t limited_faces = 0;
-#pragma omp parallel for schedule(static) reduction(+ : limited_faces)
+#pragma omp parallel for schedule(static) reduction(+ : limited_faces) if (s_num > OMP_MIN_ITEMS)
     for (int si = 1; si < s_num; ++si) {
         int el = m->side_cells[2 * si], eh = m->side_cells[2 * si + 1];
         int donor = c->q[si] >= 0 ? el : eh;
@@ -668,7 +669,7 @@ static void tick(nhf_ctx *c) {
     c->limited_faces += limited_faces;
 
     /* sources, Horton capacity and face totals (:525-566) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (e_num > OMP_MIN_ITEMS)
     for (int ei = 1; ei < e_num; ++ei) {
         int bi = cell_boundary(c, ei);
         real ds = c->donor[ei];
@@ -717,7 +718,7 @@ static void tick(nhf_ctx *c) {
 
     /* depth, positivity, level cells and extrema (:567-610) */
     const uint64_t step_end = c->step_end;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (e_num > OMP_MIN_ITEMS)
     for (int ei = 1; ei < e_num; ++ei) {
         int bi = cell_boundary(c, ei);
         real ea = (real)m->area[ei], previous = c->depth[ei];
@@ -781,7 +782,7 @@ static void compute_cfl(nhf_ctx *c) {
     const nhf_mesh *m = &c->m;
     const real g = (real)c->p.g, afa = (real)c->p.afa;
     real ndt = INFINITY;
-#pragma omp parallel for schedule(static) reduction(min : ndt)
+#pragma omp parallel for schedule(static) reduction(min : ndt) if (m->s_num > OMP_MIN_ITEMS)
     for (int si = 1; si < m->s_num; ++si) {
         int el = m->side_cells[2 * si], eh = m->side_cells[2 * si + 1];
         c->cfl[si] = INFINITY;

@@ -1,0 +1,360 @@
+"""Host-side units: NE/NS ingestion, land mapping, open faces, still water, forcing, outputs, ABI.
+
+Known answers are the reference's (``tests/test_hydraulics.py``, ``tests/test_still_water.py``,
+``tests/test_open_boundary.py:30-61``, ``tests/test_forcing.py``, ``tests/test_output_io.py``).
+"""
+import ctypes
+import json
+from datetime import datetime, timezone
+from types import SimpleNamespace as NS
+
+import numpy as np
+import pytest
+from pydantic import ValidationError
+
+from nh_flood_2d_amd import abi, mesh as meshmod
+from nh_flood_2d_amd.config import DomainConfig, absolute_time
+from nh_flood_2d_amd.forcing import (RainField, TimeSeries, load_boundaries, make_boundary,
+                                     select_boundary_edges)
+from nh_flood_2d_amd.hydraulics import connected_still_water, map_land_parameters, open_face_fractions
+from nh_flood_2d_amd.output import (append_station_samples, list_depth_frames, read_checkpoint, read_depth_frame,
+                                    read_extrema, resolve_station_cells, write_checkpoint, write_depth_frame,
+                                    write_extrema)
+import cases
+
+ORIGIN = datetime(2023, 9, 7, 8, tzinfo=timezone.utc)
+
+
+# ------------------------------------------------------------------------------ ingestion
+
+def test_filter_drops_nodata_cells_and_dangling_sides_and_renumbers():
+    # cells 1, 2 and 3 in a row; cell 2 has z = -9999 and is dropped
+    ne = ['1,1,1,1,1,1,2,8,9,5,5,0.5,1', '2,1,1,1,1,2,3,10,11,15,5,-9999,1', '3,1,1,1,1,3,4,12,13,25,5,0.7,2']
+    ns = ['1,2,0,1,0,0,10,0,5,0,0', '2,2,1,2,0,0,10,10,5,0,0', '3,2,2,3,0,0,10,20,5,0,0',
+          '4,2,3,0,0,0,10,30,5,0,0', '8,1,0,0,0,1,10,5,0,0,0', '9,1,0,0,1,0,10,5,10,0,0',
+          '10,1,0,0,0,2,10,15,0,0,0', '11,1,0,0,2,0,10,15,10,0,0', '12,1,0,0,0,3,10,25,0,0,0',
+          '13,1,0,0,3,0,10,25,10,0,0']
+    ne_rows, ns_rows = meshmod.filter_rows(ne, ns)
+    m = meshmod.compact_mesh(ne_rows, ns_rows)
+    assert m.e_num == 3 and m.s_num == 9   # sides of cell 2's top/bottom are dangling
+    np.testing.assert_array_equal(m.x, [0, 5, 25])
+    np.testing.assert_array_equal(m.land, [0, 1, 2])
+    # old side 3 (between 2 and 3) keeps its old right neighbour 3 -> new 2, left becomes exterior
+    topo = {tuple(t) for t in m.topo[1:].tolist()}
+    assert (2, 0, 2) in topo and (2, 1, 0) in topo
+    np.testing.assert_array_equal(m.sbf[1:], np.ones(8, np.uint8))
+    assert np.all(m.face_count[1:] == 1)
+    for cell in (1, 2):
+        for d in range(4):
+            side = m.face_sides(cell, d)[0]
+            assert cell in m.topo[side, 1:]
+
+
+def test_text_round_trip_preserves_compact_layout(tmp_path):
+    m = cases.boxes_mesh([(0, 8, 0, 8), (8, 12, 0, 4), (8, 12, 4, 8)], bed=[1., 2., 3.])
+    cases.write_mesh(tmp_path, m)
+    back = meshmod.load_text_mesh(tmp_path / 'ne.txt', tmp_path / 'ns.txt')
+    for name in ('x', 'y', 'z', 'land', 'face_count', 'face_ptr', 'isl', 'topo', 'length', 'sx', 'sy'):
+        np.testing.assert_array_equal(getattr(back, name), getattr(m, name), err_msg=name)
+
+
+def test_non_square_cells_are_rejected():
+    m = cases.boxes_mesh([(0, 10, 0, 10), (10, 20, 0, 10)])
+    m.sy[m.sy == 10] = 20
+    with pytest.raises(ValueError, match='square'):
+        m.geometry()
+
+
+# ------------------------------------------------------------------------------ hydraulics
+
+def land(manning_n, infiltration='none'):
+    return dict(manning_n=manning_n, infiltration=infiltration)
+
+
+def test_land_mapping_averages_real_neighbours_and_mirrors_exterior():
+    land_types = np.array([0, 1, 8, 10], dtype=np.uint8)
+    topo = np.array([[0, 0, 0], [2, 1, 2], [1, 2, 3], [2, 0, 1], [1, 3, 0]], dtype=np.int64)
+    table = {1: land(.02, dict(f0_mm_h=76.2, fc_mm_h=2.54, k_per_h=2)), 8: land(.04),
+             10: land(.06, dict(f0_mm_h=20.32, fc_mm_h=.508, k_per_h=10))}
+    side_n, f0, fc, k = map_land_parameters(table, land_types, topo)
+    assert all(v.dtype == np.float32 for v in (side_n, f0, fc, k))
+    np.testing.assert_allclose(side_n, [0., .03, .05, .02, .06], rtol=1e-7, atol=0.)
+    np.testing.assert_allclose(f0 * 3_600_000, [0., 76.2, 0., 20.32], rtol=1e-6)
+    np.testing.assert_allclose(fc * 3_600_000, [0., 2.54, 0., .508], rtol=1e-6)
+    np.testing.assert_array_equal(k, [0., 2., 0., 10.])
+
+
+def test_land_mapping_requires_every_real_land_class():
+    with pytest.raises(ValueError, match=r'land_parameters missing land types: \[8\]'):
+        map_land_parameters({10: land(.03)}, np.array([0, 8, 10], dtype=np.uint8), np.array([[0, 0, 0]]))
+
+
+def test_uniform_manning_preserves_each_side_value():
+    side_n, *_ = map_land_parameters({k: land(.033) for k in (1, 8, 10)}, np.array([0, 1, 8, 10], np.uint8),
+                                     np.array([[0, 0, 0], [2, 1, 2], [1, 2, 3], [2, 0, 1], [1, 3, 0]]))
+    np.testing.assert_array_equal(side_n[1:], np.full(4, .033, dtype=np.float32))
+
+
+def test_open_boundary_fraction_counts_only_selected_exterior_subedges():
+    selected = np.array([False, True, False, False, False])
+    lengths = np.array([0., 2., 3., 4., 1.])
+    topo = np.array([[0, 0, 0], [2, 0, 1], [2, 0, 1], [2, 1, 2], [1, 1, 0]])
+    pointers = np.array([[0, 0, 0, 0], [0, 2, 3, 4], [0, 0, 0, 0]])
+    counts = np.array([[0, 0, 0, 0], [2, 1, 1, 1], [0, 0, 0, 0]])
+    fractions = open_face_fractions(selected, lengths, topo, pointers, counts, np.array([1, 2, 3, 4, 4]))
+    np.testing.assert_array_equal(fractions[[0, 2]], np.zeros((2, 4)))
+    np.testing.assert_allclose(fractions[1], [2 / 5, 0, 0, 0])
+
+
+@pytest.mark.parametrize('selected_id', [-1, 0, 1])
+def test_open_face_fraction_uses_exterior_length_only_for_selected_cells(selected_id):
+    lengths = np.array([0, 2, 3, 1, 3, 2, 2, 4], dtype=np.float32)
+    topo = np.array([[0, 0, 0], [2, 0, 1], [2, 1, 2], [1, 0, 1], [1, 1, 2], [1, 1, 0], [1, 1, 2], [2, 2, 0]])
+    faces = [[[], [], [], []], [[1], [2], [3, 4], [5, 6]], [[2], [7], [4], [7]]]
+    pointers = np.zeros((3, 4), np.int32)
+    counts = np.zeros((3, 4), np.int32)
+    flat = []
+    for c, cf in enumerate(faces):
+        for d, ids in enumerate(cf):
+            pointers[c, d], counts[c, d] = len(flat), len(ids)
+            flat.extend(ids)
+    sel = selected_id >= 0
+    fr = open_face_fractions(np.array([False, sel, False, sel, False, sel, False, False]), lengths, topo, pointers,
+                             counts, np.array(flat))
+    assert fr.shape == (3, 4) and fr.dtype == np.float32
+    np.testing.assert_allclose(fr[1], [1., 0., .25, .5] if sel else [0.] * 4, rtol=0, atol=0)
+    np.testing.assert_array_equal(fr[[0, 2]], np.zeros((2, 4), np.float32))
+
+
+def _boundary(name, cell, level):
+    return NS(name=name, kind='level', cells=np.array([cell]), series=NS(at=lambda t: level))
+
+
+def _fill(bed, levels, crest=None, closed=None):
+    sides = np.array([[0, 0, 0], [2, 0, 1], [2, 1, 2], [2, 2, 3], [2, 3, 4], [2, 4, 0]])
+    return connected_still_water(np.array([0, *bed], dtype='float32'),
+                                 np.full(6, -999.) if crest is None else np.array(crest),
+                                 np.zeros(6, dtype='uint8') if closed is None else np.array(closed),
+                                 sides, levels, np.array([0, 1, 1, 1, 1]))
+
+
+def test_still_water_bed_barrier_is_strict_and_isolated_pool_stays_empty():
+    depth, meta = _fill([-2, -1, 1, -3], [_boundary('sea', 1, 1)])
+    np.testing.assert_array_equal(depth, [0, 3, 2, 0, 0])
+    assert meta['storage_m3'] == 5
+
+
+@pytest.mark.parametrize('obstacle', ['crest', 'closed'])
+def test_still_water_crest_and_closed_side_block_filling(obstacle):
+    kw = {'crest': [-999, -999, -999, 1, -999, -999]} if obstacle == 'crest' else {'closed': [0, 0, 0, 1, 0, 0]}
+    depth, _ = _fill([-2, -1, -2, -3], [_boundary('sea', 1, 1)], **kw)
+    np.testing.assert_array_equal(depth, [0, 3, 2, 0, 0])
+
+
+@pytest.mark.parametrize('reverse', [False, True])
+def test_still_water_connected_different_levels_reject(reverse):
+    levels = [_boundary('sea_A', 1, 1), _boundary('sea_B', 4, .5)]
+    with pytest.raises(ValueError, match='Conflicting initial levels'):
+        _fill([-2, -1, -2, -3], levels[::-1] if reverse else levels)
+
+
+def test_still_water_disconnected_levels_fill_separately_and_equal_levels_share():
+    depth, _ = _fill([-2, 2, -2, -3], [_boundary('A', 1, 1), _boundary('B', 4, .5)])
+    np.testing.assert_array_equal(depth, [0, 3, 0, 2.5, 3.5])
+    depth, meta = _fill([-2, -1, -2, -3], [_boundary('A', 1, 1), _boundary('B', 4, 1)])
+    np.testing.assert_array_equal(depth, [0, 3, 2, 3, 4])
+    assert len(meta['levels']) == 1 and meta['levels'][0]['connected_components'] == 1
+
+
+def test_still_water_compares_level_in_f64():
+    depth, _ = _fill([1, 1, 1, 1], [_boundary('sea', 1, 1 + 1e-8)])
+    np.testing.assert_allclose(depth[1:], 1e-8, rtol=1e-7)
+
+
+# ------------------------------------------------------------------------------ forcing
+
+@pytest.fixture
+def square4():
+    return cases.boxes_mesh([(0, 1, 0, 1), (1, 2, 0, 1), (0, 1, 1, 2), (1, 2, 1, 2)])
+
+
+def _series(tmp_path):
+    p = tmp_path / 'q.csv'
+    p.write_text('time,value\n2023-09-07T16:00:00+08:00,4\n2023-09-07T16:10:00+08:00,8\n')
+    return str(p)
+
+
+def _line(tmp_path, coords, epsg=2326, name='line'):
+    p = tmp_path / f'{name}{len(list(tmp_path.glob("*.geojson")))}.geojson'
+    p.write_text(json.dumps({'type': 'LineString', 'coordinates': coords,
+                             'crs': {'type': 'name', 'properties': {'name': f'EPSG:{epsg}'}}}))
+    return str(p)
+
+
+def _bcfg(tmp_path, coords, name='river', epsg=2326):
+    return NS(name=name, kind='discharge', line=_line(tmp_path, coords, epsg), series=_series(tmp_path))
+
+
+def test_exterior_line_selects_edges_and_shares_discharge(tmp_path, square4):
+    b = load_boundaries([_bcfg(tmp_path, [[0, 0], [0, 2]])], square4, 2326, ORIGIN, 0, 600)[0]
+    np.testing.assert_array_equal(b.cells, [1, 3])
+    assert b.distribute(7.3).sum() == pytest.approx(7.3, abs=1e-14)
+    assert b.series.at(300) == 6
+
+
+def test_corner_touch_and_point_crossings_select_nothing(tmp_path, square4):
+    for coords in ([[-1, -1], [0, 0]], [[-.1, .5], [2.1, .5]]):
+        with pytest.raises(ValueError, match='no exterior edges'):
+            load_boundaries([_bcfg(tmp_path, coords)], square4, 2326, ORIGIN, 0, 600)
+
+
+def test_partial_overlap_selects_whole_side_and_weights_use_selected_edges(tmp_path, square4):
+    b = load_boundaries([_bcfg(tmp_path, [[0, .2], [0, .3]])], square4, 2326, ORIGIN, 0, 600)[0]
+    np.testing.assert_array_equal(b.cells, [1])
+    np.testing.assert_array_equal(b.weights, [1.])
+    b = load_boundaries([_bcfg(tmp_path, [[0, 1], [0, 0], [2, 0]])], square4, 2326, ORIGIN, 0, 600)[0]
+    np.testing.assert_array_equal(b.cells, [1, 2])
+    np.testing.assert_allclose(b.weights, [2 / 3, 1 / 3])
+
+
+def test_invalid_crs_and_overlap_are_rejected(tmp_path, square4):
+    with pytest.raises(ValueError, match='CRS differs'):
+        load_boundaries([_bcfg(tmp_path, [[0, 0], [0, 1]], epsg=4326)], square4, 2326, ORIGIN, 0, 600)
+    with pytest.raises(ValueError, match='overlap'):
+        load_boundaries([_bcfg(tmp_path, [[0, 0], [0, 1]]), _bcfg(tmp_path, [[0, 0], [0, 1]], name='sea')],
+                        square4, 2326, ORIGIN, 0, 600)
+
+
+@pytest.mark.parametrize('begin,end', [(-1, 600), (0, 601)])
+def test_series_must_cover_the_window(tmp_path, begin, end):
+    with pytest.raises(ValueError, match='does not cover'):
+        TimeSeries.load(_series(tmp_path), ORIGIN, begin, end)
+
+
+def test_timestamps_require_a_timezone():
+    with pytest.raises(ValueError, match='timezone'):
+        absolute_time('2023-09-07T08:00:00')
+
+
+def test_rain_mapping_piecewise_and_volume(tmp_path, square4):
+    from nh_flood_2d_amd.geotiff import write_geotiff
+    write_geotiff(tmp_path / 'rain.tif', np.array([[[10, 20], [0, 30]], [[0, 0], [0, 0]]], np.float32),
+                  0., 2., 1., 1., 2326)
+    (tmp_path / 'rain.csv').write_text('interval_start,interval_end\n2023-09-07T08:00:00Z,2023-09-07T08:05:00Z\n'
+                                       '2023-09-07T08:05:00Z,2023-09-07T08:10:00Z\n')
+    cfg = NS(raster=str(tmp_path / 'rain.tif'), intervals=str(tmp_path / 'rain.csv'))
+    r = RainField.load(cfg, square4, 2326, ORIGIN, 0, 600)
+    rates = r.rates(r.index_at(299))[r.pixel_index][1:]
+    # cells in mesh order: (0.5,0.5), (1.5,0.5), (0.5,1.5), (1.5,1.5)
+    np.testing.assert_allclose(rates * 300, [0, .03, .01, .02])
+    assert r.index_at(300) == 1 and np.all(r.rates(1) == 0)
+
+
+@pytest.mark.parametrize('outside', [False, True])
+def test_rain_nodata_or_outside_is_rejected(tmp_path, square4, outside):
+    from nh_flood_2d_amd.geotiff import write_geotiff
+    write_geotiff(tmp_path / 'rain.tif', np.array([[[-9999, 0], [0, 0]], [[0, 0], [0, 0]]], np.float32),
+                  10. if outside else 0., 2., 1., 1., 2326, nodata=-9999)
+    (tmp_path / 'rain.csv').write_text('interval_start,interval_end\n2023-09-07T08:00:00Z,2023-09-07T08:05:00Z\n'
+                                       '2023-09-07T08:05:00Z,2023-09-07T08:10:00Z\n')
+    with pytest.raises(ValueError, match='outside|NoData'):
+        RainField.load(NS(raster=str(tmp_path / 'rain.tif'), intervals=str(tmp_path / 'rain.csv')),
+                       square4, 2326, ORIGIN, 0, 600)
+
+
+@pytest.mark.parametrize('start', [1694054400, '2023-09-07T16:00:00'])
+def test_start_requires_timezone_iso8601(tmp_path, start):
+    (tmp_path / 'ne').write_text('mesh')
+    (tmp_path / 'ns').write_text('mesh')
+    config = dict(ne=str(tmp_path / 'ne'), ns=str(tmp_path / 'ns'), epsg_code=2326, domain_dir=str(tmp_path / 'out'),
+                  start=start, duration=1, warmup_duration=0,
+                  land_parameters={10: dict(manning_n=.03, infiltration='none')})
+    with pytest.raises(ValidationError):
+        DomainConfig.model_validate_json(json.dumps(config))
+
+
+@pytest.mark.parametrize('parameters', [
+    .03, {'manning_n': -.01, 'infiltration': 'none'}, {'manning_n': .03},
+    {'manning_n': .03, 'infiltration': 'invalid'},
+    {'manning_n': .03, 'infiltration': {'f0_mm_h': 1., 'fc_mm_h': 2., 'k_per_h': 1.}},
+    {'manning_n': .03, 'infiltration': {'f0_mm_h': -1., 'fc_mm_h': 0., 'k_per_h': 1.}},
+    {'manning_n': .03, 'infiltration': 'none', 'unexpected': 1}])
+def test_land_config_rejects_invalid_parameter_sets(tmp_path, parameters):
+    (tmp_path / 'ne').write_text('mesh')
+    (tmp_path / 'ns').write_text('mesh')
+    with pytest.raises(ValidationError):
+        DomainConfig(ne=str(tmp_path / 'ne'), ns=str(tmp_path / 'ns'), domain_dir=str(tmp_path / 'out'),
+                     epsg_code=2326, start=ORIGIN, duration=1, warmup_duration=0, land_parameters={10: parameters})
+
+
+def test_clock_range_and_fractional_output_interval_are_rejected(tmp_path):
+    (tmp_path / 'ne').write_text('mesh')
+    (tmp_path / 'ns').write_text('mesh')
+    base = dict(ne=str(tmp_path / 'ne'), ns=str(tmp_path / 'ns'), domain_dir=str(tmp_path / 'out'), epsg_code=2326,
+                start=ORIGIN, land_parameters={10: dict(manning_n=.03, infiltration='none')})
+    with pytest.raises(ValidationError):
+        DomainConfig(**base, duration=2 ** 24, warmup_duration=0)
+    with pytest.raises(ValidationError):
+        DomainConfig(**base, duration=1, warmup_duration=0, yield_step=.5)
+
+
+# ------------------------------------------------------------------------------ outputs
+
+def test_frames_and_checkpoints_keep_exact_f32_and_metadata(tmp_path):
+    write_depth_frame(tmp_path / 'output/frames/depth_900.fdb', [0, .25, .5])
+    write_depth_frame(tmp_path / 'output/frames/depth_0.fdb', [0, 1, 2])
+    assert [s for s, _ in list_depth_frames(tmp_path / 'output')] == [0, 900]
+    depth = read_depth_frame(tmp_path / 'output/frames/depth_900.fdb', expected_count=3)
+    assert depth.dtype == np.float32
+    with pytest.raises(ValueError, match='expected 4'):
+        read_depth_frame(tmp_path / 'output/frames/depth_900.fdb', expected_count=4)
+    d = np.array([0, np.nextafter(np.float32(0), np.float32(1))], dtype=np.float32)
+    meta = '{ "budget_units": ["12345678901234567890"], "phase": "simulation" }'
+    write_checkpoint(tmp_path / 'c.fdb', depth=d, bed=[0, 10000], q=[0, -.25, .5], rain_time=[0, 12],
+                     infiltration=[0, .0001], max_depth=[0, .5], first_time_hi=[0, 2], first_time_lo=[0, 3],
+                     max_speed=[0, 1.25], clock_hi=2, clock_lo=4294967295, gate_open=True, metadata_json=meta)
+    s = read_checkpoint(tmp_path / 'c.fdb', expected_cells=2, expected_sides=3)
+    np.testing.assert_array_equal(s.depth.view(np.uint32), d.view(np.uint32))
+    np.testing.assert_array_equal(s.q, [0, -.25, .5])
+    assert (s.clock_hi, s.clock_lo, s.gate_open) == (2, 4294967295, True)
+    assert s.metadata_json == meta
+    with pytest.raises(ValueError, match='side count'):
+        read_checkpoint(tmp_path / 'c.fdb', expected_sides=4)
+    with pytest.raises(ValueError, match='depth'):
+        write_checkpoint(tmp_path / 'bad.fdb', depth=[0, np.nan], bed=[0, 1], q=[0, 0, 0], rain_time=[0, 0],
+                         infiltration=[0, 0], max_depth=d, first_time_hi=[0, 0], first_time_lo=[0, 0],
+                         max_speed=[0, 0], clock_hi=0, clock_lo=0, gate_open=False, metadata_json='{}')
+    write_extrema(tmp_path / 'e.fdb', [0, .5], [0, 2], [0, 4294967295], [0, 1.25])
+    e = read_extrema(tmp_path / 'e.fdb', expected_count=2)
+    np.testing.assert_array_equal(e.first_time_lo, [0, 4294967295])
+
+
+def test_station_resolution_and_csv_order(tmp_path):
+    m = cases.two_cells(True)
+    assert resolve_station_cells({'joint': (10, 5), 'right': (15, 5)}, m) == {'joint': 1, 'right': 2}
+    with pytest.raises(ValueError, match='outside the mesh'):
+        resolve_station_cells({'missing': (25, 5)}, m)
+    append_station_samples(tmp_path / 's.csv', elapsed_seconds=300, time='t',
+                           samples={'right': dict(cell_id=2, depth=.5, level=20.5, u=1, v=2),
+                                    'joint': dict(cell_id=1, depth=.25, level=10.25, u=3, v=4)})
+    rows = (tmp_path / 's.csv').read_text().splitlines()
+    assert rows[1].split(',')[2] == 'joint' and rows[2].split(',')[2] == 'right'
+
+
+# ------------------------------------------------------------------------------ C ABI
+
+def test_hip_library_loads_and_exports_every_declared_symbol():
+    """The gfx950 library links on a host without a GPU; no compute call is made here."""
+    import re
+    from pathlib import Path
+    header = (Path(__file__).resolve().parents[1] / 'include' / 'nhflood.h').read_text()
+    declared = set(re.findall(r'\b(nhf_[a-z_]+)\s*\(', header))
+    lib = abi.load_hip_library()
+    assert declared == set(abi.EXPORTED)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.nhf_abi_version() == abi.ABI_VERSION
+    m = abi.MeshStruct()
+    m.e_num, m.s_num, m.b_num = 10, 20, 1
+    p = abi.Params(batch_capacity=32, theta=1.0)
+    assert lib.nhf_workspace_bytes(ctypes.byref(m), ctypes.byref(p)) > 0

@@ -11,7 +11,8 @@ the host protocol of the solver. Mesh, state and forcing are resident in HBM bef
 
 Prints one JSON line (rank 0, N = 1 only):
   value        whole-job cell-updates/s = cells x steps / wall seconds of the timed region
-  roofline     the dominant kernel's algorithmic bytes / its mean duration (HIP events)
+  roofline     the dominant phase's algorithmic bytes / its mean duration (HIP events), with
+               its measured HBM bytes (rocprofv3 counters) as `traffic`
   cpu_baseline the CPU oracle (C, OpenMP) on the host cores, on a bounded sample of steps
 """
 from __future__ import annotations
@@ -19,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -46,10 +48,29 @@ def build_workload(seed: int):
     return sc, forcing, depth
 
 
-def kernel_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
-    """Algorithmic (compulsory) bytes per launch of each kernel: every input it needs read once,
-    every output written once (conditional extrema writes excluded). Nc cells, Ns sides, R
-    side-list entries, E hanging edges, G gradient cells."""
+def algorithmic_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
+    """Algorithmic bytes per step and per phase: every per-step input read once, every output
+    written once, with these per-unit figures:
+      per side       q r+w 8, lower/higher 8, length 4, centre distance 4, Manning 4, crest 4,
+                     orient/flag 1                                                   = 33 B
+      per cell       depth r+w 8, bed 4, area 4, Horton 12, rain time r+w 8, capacity r+w 8,
+                     rain index 4, boundary id 4, side-list header 8, donor w+r 8, max depth
+                     r+w 8, max speed r+w 8                                           = 84 B
+      per side-list entry                                                              4 B
+      2:1 meshes     edge index 4 per side, 24 per hanging edge, 24 per gradient cell
+    Phase (a), the face flux, is 33 Ns + 8 Nc (+ the 2:1 terms); phase (b), the cell update, is
+    the rest, 76 Nc + 4 R; the CFL reads only inputs already counted."""
+    nc, ns, r = e_num - 1, s_num - 1, r_len
+    hang = (4 * ns + 24 * layout.n_hang + 24 * layout.n_grad) if layout.n_hang else 0
+    a = 33 * ns + 8 * nc + hang
+    b = 76 * nc + 4 * r
+    return {'step': a + b, 'flux': a, 'cell_update': b}
+
+
+def launch_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
+    """Bytes each launch itself must move in this layout (its inputs read once, its outputs
+    written once, conditional extrema writes excluded): Nc cells, Ns sides, R side-list entries,
+    E hanging edges, G gradient cells."""
     nc, ns, r = e_num - 1, s_num - 1, r_len
     e_hang, g = layout.n_hang, layout.n_grad
     pixel = 4 if layout.n_pixels > 1 else 0
@@ -57,19 +78,66 @@ def kernel_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
         # per side: side_cells 8, meta 1, q 4, sdc 4, manning 4, qflux out 4 (+ edge_index 4);
         # per cell: bed + depth 8; per hanging edge: offsets + slots 16; per gradient cell 8
         'flux': ns * (25 + (4 if e_hang else 0)) + 8 * nc + 16 * e_hang + 8 * g,
-        # per cell: meta 4, area 4, depth 4, csr start + counts 8, donor out 4 (+ rain pixel);
+        # per cell: meta 4, area 4, depth 4, side-list header 8, donor out 4 (+ rain pixel);
         # per side-list entry: side id 4; per side: qflux 4, length 4
         'donor': (24 + pixel) * nc + 4 * r + 8 * ns,
-        # per cell: meta 4, donor 4, rain time r+w 8, capacity r+w 8, area 4, size 4, depth r+w 8,
-        # max speed 4, max depth 4, csr 8 (+ rain pixel); per side-list entry: side id 4,
-        # neighbour id 4; per side: qflux 4, length 4, q out 4
-        'depth': (56 + pixel) * nc + 8 * r + 12 * ns,
+        # per cell: meta 4, donor 4, rain time r+w 8, capacity w 4, area 4, size 4, depth r+w 8,
+        # max speed 4, max depth 4, side-list header 8 (+ rain pixel); per side-list entry: side
+        # id 4, neighbour id 4; per side: qflux 4, length 4, scaled q out 4
+        'depth': (52 + pixel) * nc + 8 * r + 12 * ns,
         # per side: side_cells 8, length 4, q 4, reference id 4; per cell: bed + depth 8
         'cfl': 20 * ns + 8 * nc,
         # per gradient cell: its face neighbours' bed, depth, centre (about 5 x 12 B)
         'gradients': g * 60,
         'face_avg': 0, 'prepare': 0, 'finish': 0,
     }
+
+
+TRAFFIC_SUMMARY = ROOT / 'profiles' / 'hbm_traffic_multires_channel.txt'
+PMC_KERNELS = {'k_gradients': 'gradients', 'k_face_avg': 'face_avg', 'k_flux': 'flux', 'k_donor': 'donor',
+               'k_depth': 'depth', 'k_cfl': 'cfl', 'k_finish': 'finish'}
+
+
+def pmc_summary(fetch_csv: str, write_csv: str) -> str:
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE in
+    KiB), scaled by what each counter reads for the 256 MiB device copy of --calibrate-copy.
+    Lines: 'calibration <counter> ... factor F' and 'kernel <name> ... hbm_bytes B'."""
+    import csv
+    from collections import defaultdict
+    lines, scaled = [], {}
+    copy_bytes = float(64 << 20) * 4
+    for counter, path in (('FETCH_SIZE', fetch_csv), ('WRITE_SIZE', write_csv)):
+        per_kernel, copies = defaultdict(list), []
+        for row in csv.DictReader(open(path)):
+            name = row['Kernel_Name'].split('(')[0].strip()
+            value = float(row['Counter_Value']) * 1024.0
+            if 'copyBuffer' in name and int(row['Grid_Size']) >= 131072:
+                copies.append(value)
+            elif name in PMC_KERNELS:
+                per_kernel[PMC_KERNELS[name]].append(value)
+        if not copies:
+            raise SystemExit(f'{path}: no calibration copy found (run bench.py with --calibrate-copy)')
+        factor = copy_bytes / (sum(copies) / len(copies))
+        lines.append(f'calibration {counter} copy_bytes {copy_bytes:.0f} counter_bytes '
+                     f'{sum(copies) / len(copies):.0f} factor {factor:.4f}')
+        for k, v in per_kernel.items():
+            scaled.setdefault(k, {})[counter] = (sum(v) / len(v) * factor, len(v))
+    for k in sorted(scaled, key=list(PMC_KERNELS.values()).index):
+        f, nf = scaled[k].get('FETCH_SIZE', (0.0, 0))
+        w, _ = scaled[k].get('WRITE_SIZE', (0.0, 0))
+        lines.append(f'kernel {k} dispatches {nf} fetch_bytes {f:.0f} write_bytes {w:.0f} hbm_bytes {f + w:.0f}')
+    return '\n'.join(lines) + '\n'
+
+
+def read_traffic(cells: int) -> dict | None:
+    """HBM bytes per dispatch from the committed counter summary, if it was made on this workload."""
+    if not TRAFFIC_SUMMARY.exists():
+        return None
+    text = TRAFFIC_SUMMARY.read_text()
+    m = re.search(r'^workload cells (\d+)', text, re.M)
+    if not m or int(m.group(1)) != cells:
+        return None
+    return {k: float(v) for k, v in re.findall(r'^kernel (\w+) .* hbm_bytes (\d+)', text, re.M)}
 
 
 def cpu_baseline(sc, forcing, depth, seconds_budget: float = 15.0):
@@ -123,7 +191,17 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='also copy a 256 MiB buffer 3 times (reference dispatch for HBM counters)')
+    ap.add_argument('--pmc-summary', nargs=3, metavar=('FETCH_CSV', 'WRITE_CSV', 'BENCH_JSON'),
+                    help='summarise two rocprofv3 --pmc runs of this script (no GPU needed) and exit')
     args = ap.parse_args()
+    if args.pmc_summary:
+        fetch_csv, write_csv, bench_json = args.pmc_summary
+        with open(bench_json) as f:
+            line = [ln for ln in f if ln.startswith('{')][-1]
+        cfg = json.loads(line)['config']
+        print(f"workload cells {cfg['cells']} sides {cfg['sides']} side_list {cfg['side_list']}")
+        print(pmc_summary(fetch_csv, write_csv), end='')
+        return
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
               '(independent scenarios would shard one per GPU; see DESIGN.md)', file=sys.stderr)
@@ -195,19 +273,31 @@ def main():
     run(args.kernel_steps)
     ms, count = eng.kernel_times(reset=True)
     eng.set_timing(False)
-    nbytes = kernel_bytes(layout, mesh.e_num, mesh.s_num, r_len)
+    nbytes = launch_bytes(layout, mesh.e_num, mesh.s_num, r_len)
+    alg = algorithmic_bytes(layout, mesh.e_num, mesh.s_num, r_len)
+    traffic = read_traffic(cells)
     kernels = {}
     for i, name in enumerate(KERNELS):
         if count[i]:
             avg_ms = ms[i] / count[i]
             kernels[name] = dict(avg_us=avg_ms * 1e3, launches=int(count[i]),
-                                 bytes=int(nbytes[name]),
-                                 gbs=nbytes[name] / (avg_ms * 1e-3) / 1e9 if nbytes[name] else None)
-    dominant = max((k for k in kernels if nbytes[k]), key=lambda k: kernels[k]['avg_us'])
-    dk = kernels[dominant]
-    step_bytes = sum(nbytes[k] for k in ('flux', 'donor', 'depth', 'cfl', 'gradients'))
-    log('kernels: ' + ', '.join(f"{k} {v['avg_us']:.1f} us" + (f" {v['gbs']:.0f} GB/s" if v['gbs'] else '')
-                                for k, v in kernels.items()))
+                                 launch_bytes=int(nbytes[name]),
+                                 launch_gbs=nbytes[name] / (avg_ms * 1e-3) / 1e9 if nbytes[name] else None,
+                                 hbm_bytes=traffic.get(name) if traffic else None)
+    # phases of the step: (a) face flux = k_flux; (b) cell update = k_donor + k_depth (the donor
+    # limiter needs every face of a cell before any face is scaled, hence two launches)
+    phases = {'flux': ['flux'], 'cell_update': ['donor', 'depth']}
+    phase = {}
+    for p, ks in phases.items():
+        us = sum(kernels[k]['avg_us'] for k in ks)
+        hbm = sum(traffic[k] for k in ks) if traffic and all(k in traffic for k in ks) else None
+        phase[p] = dict(kernels=ks, avg_us=us, algorithmic_bytes=alg[p],
+                        achieved_gbs=alg[p] / (us * 1e-6) / 1e9, hbm_bytes=hbm)
+    dominant = max(phase, key=lambda p: phase[p]['avg_us'])
+    dp = phase[dominant]
+    log('kernels: ' + ', '.join(f"{k} {v['avg_us']:.1f} us" for k, v in kernels.items()))
+    log('phases: ' + ', '.join(f"{p} {v['avg_us']:.1f} us {v['achieved_gbs']:.0f} GB/s algorithmic"
+                               for p, v in phase.items()))
     result = {
         'metric': METRIC, 'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': steps,
         'warmup': args.warmup, 'ms_per_step': event_ms / steps, 'higher_is_better': True,
@@ -217,11 +307,15 @@ def main():
                                'tide boundary, wet start, plain single-scenario path',
                    'cells': cells, 'sides': mesh.s_num - 1, 'side_list': r_len,
                    'hanging_edges': layout.n_hang, 'batch_steps': 32, 'launch': 'hipGraph per 32-step batch'},
-        'roofline': {'bound': 'hbm', 'kernel': dominant, 'achieved': dk['gbs'], 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': dk['gbs'] / HBM_PEAK_GBS, 'traffic': None,
-                     'algorithmic_bytes': dk['bytes'], 'avg_us': dk['avg_us']},
+        'roofline': {'bound': 'hbm', 'kernel': f"{dominant} ({' + '.join('k_' + k for k in dp['kernels'])})",
+                     'achieved': dp['achieved_gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': dp['achieved_gbs'] / HBM_PEAK_GBS, 'traffic': dp['hbm_bytes'],
+                     'algorithmic_bytes': dp['algorithmic_bytes'], 'avg_us': dp['avg_us']},
+        'phases': phase,
         'kernels': kernels,
-        'step': {'algorithmic_bytes': step_bytes, 'achieved_gbs': step_bytes / (event_ms / steps * 1e-3) / 1e9},
+        'step': {'algorithmic_bytes': alg['step'],
+                 'achieved_gbs': alg['step'] / (event_ms / steps * 1e-3) / 1e9,
+                 'hbm_bytes': sum(v for v in traffic.values()) if traffic else None},
     }
     if args.calibrate_copy:
         with torch.cuda.stream(device.stream):

@@ -1,0 +1,275 @@
+// Device code shared by the single-mesh step (nhflood.hip) and the scenario-batched step
+// (nhbatch.hip): constants, the clock, deterministic x^y / e^x, wave reductions and the
+// per-context control block. Everything here is static or inline: each translation unit gets
+// its own copy of the device functions.
+#ifndef NHF_DEVICE_H
+#define NHF_DEVICE_H
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "nhflood.h"
+
+#define NHF_BLOCK 256
+#define NHF_WAVES (NHF_BLOCK / 64)
+#define TICKS_PER_SECOND_F 1099511627776.0f  // 2^40 exactly representable in F32
+#define SNAP_TICKS 1099512ull                 // ceil(1e-6 s * 2^40)
+
+// Thread-local text of the last error, defined once in nhflood.hip.
+extern thread_local char nhf_error_text[512];
+
+static inline int32_t set_error(const char *what, hipError_t err) {
+    snprintf(nhf_error_text, sizeof nhf_error_text, "%s: %s", what, hipGetErrorString(err));
+    return -1;
+}
+
+static inline int32_t set_error_text(const char *what) {
+    snprintf(nhf_error_text, sizeof nhf_error_text, "%s", what);
+    return -2;
+}
+
+#define HIP_TRY(expr)                                      \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return set_error(#expr, e_); \
+    } while (0)
+
+// --------------------------------------------------------------------------- control block
+
+// One scenario's clock, step scalars, limiter counters, CFL-limit capture and batch trace.
+struct Ctl {
+    uint64_t now, next, origin, step_end;
+    float dt, local, end_local, ndt;
+    int32_t active, cfl_arg, raining, slot;
+    uint32_t limited_cells, limited_faces, min_scale_bits, pad0;
+    float limit_value[5];
+    int32_t limit_ids[3];
+    uint32_t trace[NHF_MAX_BATCH * NHF_TRACE_COLS];
+};
+
+// Comparisons written as selects so +0/-0 and NaN behave the same in every pass.
+static __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
+static __device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
+
+static __device__ __forceinline__ int face_n(uint32_t counts, int d) { return (int)((counts >> (8 * d)) & 255u); }
+
+// Workgroups are dealt to the 8 XCDs round-robin. For a grid that is a multiple of 8, this maps
+// the hardware order so XCD x runs the contiguous x-th eighth of the logical blocks: Morton-near
+// cells then share one XCD's L2 instead of being spread over all eight.
+static __device__ __forceinline__ int xcd_block() {
+    const int per = (int)(gridDim.x >> 3);
+    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+}
+
+static __device__ __forceinline__ int meta_boundary(uint32_t meta) { return (int)((meta >> 8) & 255u) - 1; }
+static __device__ __forceinline__ int meta_class(uint32_t meta) { return (int)(meta & 255u); }
+
+// --------------------------------------------------------------------------- x^y and e^x
+// Evaluated in FP64 and rounded once to F32, so the result is the same on any IEEE machine:
+// log2 by range reduction to m in [sqrt(1/2), sqrt(2)) and the odd series of
+// ln m = 2 atanh((m-1)/(m+1)); 2^t by t = n + f, |f| <= 1/2, and the Taylor series of
+// e^(f ln 2) (J.-M. Muller, Elementary Functions, ch. 5). Only IEEE add, multiply, divide and
+// explicit fma are used.
+static __device__ __forceinline__ double det_log2(double x) {
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double p = 1.0 / 25.0;
+    p = fma(p, s2, 1.0 / 23.0);
+    p = fma(p, s2, 1.0 / 21.0);
+    p = fma(p, s2, 1.0 / 19.0);
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    p = fma(p, s2, 1.0);
+    double ln_m = 2.0 * s * p;
+    return fma(ln_m, 1.44269504088896340736, (double)e);
+}
+
+static __device__ __forceinline__ double det_exp2(double t) {
+    if (t < -1100.0) return 0.0;
+    if (t > 1100.0) return INFINITY;
+    double n = rint(t);
+    double u = (t - n) * 0.69314718055994530942;
+    double p = 1.0 / 87178291200.0;
+    p = fma(p, u, 1.0 / 6227020800.0);
+    p = fma(p, u, 1.0 / 479001600.0);
+    p = fma(p, u, 1.0 / 39916800.0);
+    p = fma(p, u, 1.0 / 3628800.0);
+    p = fma(p, u, 1.0 / 362880.0);
+    p = fma(p, u, 1.0 / 40320.0);
+    p = fma(p, u, 1.0 / 5040.0);
+    p = fma(p, u, 1.0 / 720.0);
+    p = fma(p, u, 1.0 / 120.0);
+    p = fma(p, u, 1.0 / 24.0);
+    p = fma(p, u, 1.0 / 6.0);
+    p = fma(p, u, 0.5);
+    p = fma(p, u, 1.0);
+    p = fma(p, u, 1.0);
+    return ldexp(p, (int)n);
+}
+
+static __device__ __forceinline__ float det_powf(float x, float y) {
+    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+    if (x == 1.0f) return 1.0f;
+    return (float)det_exp2((double)y * det_log2((double)x));
+}
+
+static __device__ __forceinline__ float det_expf(float x) { return (float)det_exp2((double)x * 1.44269504088896340736); }
+
+// --------------------------------------------------------------------------- block reductions
+
+static __device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+static __device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+static __device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// (key, index) minimum: the key orders by CFL bits, then by the reference side id.
+static __device__ __forceinline__ void wave_min_key(uint64_t &key, int32_t &idx) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t k2 = __shfl_xor((unsigned long long)key, o, 64);
+        int32_t i2 = __shfl_xor(idx, o, 64);
+        if (k2 < key) {
+            key = k2;
+            idx = i2;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- clock
+
+static __device__ __forceinline__ float ticks_local(uint64_t ticks) {
+    uint32_t hi = (uint32_t)(ticks >> 32), lo = (uint32_t)ticks;
+    return (float)hi * (1.0f / 256.0f) + (float)lo * (1.0f / TICKS_PER_SECOND_F);
+}
+
+// Exact F32 seconds -> 2^-40 s ticks, half-even below one tick; 0 when not representable.
+static __device__ __forceinline__ uint64_t f32_ticks(float v) {
+    uint32_t bits = __float_as_uint(v);
+    int e = (int)((bits >> 23) & 255u) - 127;
+    if (!(v > 0.0f) || e < -40 || e > 0) return 0;
+    uint64_t mant = (uint64_t)((bits & 0x7fffffu) | 0x800000u);
+    int shift = e + 17;
+    if (shift >= 0) return mant << shift;
+    int right = -shift;
+    uint64_t t = mant >> right;
+    uint64_t dropped = mant & ((1ull << right) - 1ull);
+    uint64_t half = 1ull << (right - 1);
+    if (dropped > half || (dropped == half && (t & 1ull))) t += 1;
+    return t;
+}
+
+// StepClock.choose (core/clock.py:94-167) with u64 tick arithmetic. Writes the trace row's
+// status / candidate / event-short columns and the step's dt, local times and end tick.
+static __device__ void clock_choose(Ctl *c, float candidate, float min_dt, int slot) {
+    uint32_t *row = &c->trace[slot * NHF_TRACE_COLS];
+    c->active = 0;
+    c->dt = 0.0f;
+    c->local = 0.0f;
+    c->end_local = 0.0f;
+    uint64_t now = c->now, target = c->next;
+    if (!(now < target)) return;
+    if (!(candidate >= min_dt)) {
+        row[3] = NHF_BAD_CFL;
+        return;
+    }
+    uint64_t rem = target - now;
+    float remaining = ticks_local(rem);
+    float capped = fmn(candidate, 1.0f);
+    float chosen = fmn(capped, remaining);
+    row[4] = __float_as_uint(candidate);
+    row[5] = chosen < capped ? 1u : 0u;
+    uint64_t dt_ticks = f32_ticks(chosen);
+    if (dt_ticks == 0) {
+        row[3] = NHF_BAD_DT;
+        return;
+    }
+    bool exceeds = dt_ticks > rem;
+    uint64_t diff = exceeds ? dt_ticks - rem : rem - dt_ticks;
+    uint64_t end;
+    if (diff < SNAP_TICKS) {
+        end = target;
+    } else if (exceeds) {
+        row[3] = NHF_OVERSHOOT;
+        return;
+    } else {
+        end = now + dt_ticks;
+    }
+    c->local = ticks_local(now - c->origin);
+    c->end_local = ticks_local(end - c->origin);
+    c->dt = chosen;
+    c->step_end = end;
+    c->active = 1;
+}
+
+// finish_step + StepClock.commit (solver_compact.py:384-391, core/clock.py:173-181).
+static __device__ void commit_slot(Ctl *c, int slot) {
+    if (!c->active) return;
+    uint32_t *row = &c->trace[slot * NHF_TRACE_COLS];
+    row[6] = c->limited_cells;
+    row[7] = c->limited_faces;
+    row[8] = c->min_scale_bits;
+    c->now = c->step_end;
+    row[0] = __float_as_uint(c->dt);
+    row[1] = (uint32_t)(c->step_end >> 32);
+    row[2] = (uint32_t)c->step_end;
+    row[3] = NHF_COMPLETED;
+}
+
+// Linear continuation of missing theta neighbours at prescribed faces
+// (boundary_neighbor_sum, solver_compact.py:135-149).
+static __device__ __forceinline__ float neighbour_sum(float q, float left, float right, float fl, float fr) {
+    float result = left + right;
+    if (fl > 0.0f || fr > 0.0f) {
+        if (fl == 1.0f || fr == 1.0f) {
+            result = 2.0f * q;
+        } else {
+            float product = fl * fr;
+            result = ((1.0f - fr) * left + (1.0f - fl) * right + 2.0f * q * (fl + fr - 2.0f * product)) /
+                     (1.0f - product);
+        }
+    }
+    return result;
+}
+
+// Neumaier step on a (sum, compensation) pair.
+static __device__ __forceinline__ void acc_add(double2 &a, double v) {
+    double s = a.x + v;
+    if (fabs(a.x) >= fabs(v))
+        a.y += (a.x - s) + v;
+    else
+        a.y += (v - s) + a.x;
+    a.x = s;
+}
+
+#endif  // NHF_DEVICE_H

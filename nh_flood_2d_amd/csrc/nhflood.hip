@@ -5,10 +5,11 @@
 //   k_face_avg    theta < 1 face averages of the previous q         (solver_compact.py:430-436)
 //   k_flux        face flux: theta-weighted discharge, semi-implicit Manning friction, min_h
 //                 gating, exterior zeroing, hanging pressure correction (solver_compact.py:439-486)
-//   k_donor       positivity limiter: outgoing vs available volume   (solver_compact.py:491-514)
-//   k_depth       shared-subedge scaling fused with sources, Horton capacity, face totals,
-//                 continuity, infiltration removal, positivity floor, level cells, extrema and
-//                 the per-block FP64 water budget                      (solver_compact.py:516-613)
+//   k_donor       positivity limiter: outgoing vs available volume, then the shared-subedge
+//                 scaling of every side this cell is the donor of (solver_compact.py:491-524)
+//   k_depth       sources, Horton capacity, face totals, continuity, infiltration removal,
+//                 positivity floor, level cells, extrema and the per-wavefront FP64 water
+//                 budget                                               (solver_compact.py:525-613)
 //   k_cfl         CFL bound per interior side, wave-shuffle + LDS min/argmin, then the last
 //                 workgroup reduces the partials, commits the clock and prepares the next slot
 //                 (solver_compact.py:617-641, 364-391; core/clock.py:94-181)
@@ -17,52 +18,18 @@
 // Layout: structure of arrays, cells and sides renumbered along a Morton curve by the host so a
 // wavefront's neighbour gathers stay within a few cache lines. Cell 0 / side 0 are sentinels.
 // The path is HBM-bandwidth bound (about 0.2 flop/byte); no MFMA.
-#include <hip/hip_runtime.h>
-
-#include <math.h>
-#include <stdint.h>
-#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <vector>
 
-#include "nhflood.h"
+#include "nhf_device.h"
 
-#define NHF_BLOCK 256
-#define NHF_WAVES (NHF_BLOCK / 64)
-#define TICKS_PER_SECOND_F 1099511627776.0f  // 2^40 exactly representable in F32
-#define SNAP_TICKS 1099512ull                 // ceil(1e-6 s * 2^40)
+// --------------------------------------------------------------------------- errors
 
-static thread_local char g_error[512];
-
-static int32_t set_error(const char *what, hipError_t err) {
-    snprintf(g_error, sizeof g_error, "%s: %s", what, hipGetErrorString(err));
-    return -1;
-}
-
-static int32_t set_error_text(const char *what) {
-    snprintf(g_error, sizeof g_error, "%s", what);
-    return -2;
-}
-
-#define HIP_TRY(expr)                                      \
-    do {                                                   \
-        hipError_t e_ = (expr);                            \
-        if (e_ != hipSuccess) return set_error(#expr, e_); \
-    } while (0)
+thread_local char nhf_error_text[512];
 
 // --------------------------------------------------------------------------- device state
-
-struct Ctl {
-    uint64_t now, next, origin, step_end;
-    float dt, local, end_local, ndt;
-    int32_t active, cfl_arg, raining, slot;
-    uint32_t limited_cells, limited_faces, min_scale_bits, pad0;
-    float limit_value[5];
-    int32_t limit_ids[3];
-    uint32_t trace[NHF_MAX_BATCH * NHF_TRACE_COLS];
-};
 
 struct Dev {
     nhf_mesh m;
@@ -75,188 +42,6 @@ struct Dev {
     double2 *acc;  // per depth-block, per term: (sum, compensation)
     int32_t terms, cfl_blocks, depth_blocks, theta_one;
 };
-
-// Comparisons written as selects so +0/-0 and NaN behave the same in every pass.
-__device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
-__device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
-
-__device__ __forceinline__ int face_n(uint32_t counts, int d) { return (int)((counts >> (8 * d)) & 255u); }
-
-// Workgroups are dealt to the 8 XCDs round-robin. For a grid that is a multiple of 8, this maps
-// the hardware order so XCD x runs the contiguous x-th eighth of the logical blocks: Morton-near
-// cells then share one XCD's L2 instead of being spread over all eight.
-__device__ __forceinline__ int xcd_block() {
-    const int per = (int)(gridDim.x >> 3);
-    return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
-}
-
-__device__ __forceinline__ int meta_boundary(uint32_t meta) { return (int)((meta >> 8) & 255u) - 1; }
-__device__ __forceinline__ int meta_class(uint32_t meta) { return (int)(meta & 255u); }
-
-// --------------------------------------------------------------------------- x^y and e^x
-// Evaluated in FP64 and rounded once to F32, so the result is the same on any IEEE machine:
-// log2 by range reduction to m in [sqrt(1/2), sqrt(2)) and the odd series of
-// ln m = 2 atanh((m-1)/(m+1)); 2^t by t = n + f, |f| <= 1/2, and the Taylor series of
-// e^(f ln 2) (J.-M. Muller, Elementary Functions, ch. 5). Only IEEE add, multiply, divide and
-// explicit fma are used.
-__device__ __forceinline__ double det_log2(double x) {
-    int e;
-    double m = frexp(x, &e);
-    if (m < 0.70710678118654752440) {
-        m *= 2.0;
-        e -= 1;
-    }
-    double s = (m - 1.0) / (m + 1.0);
-    double s2 = s * s;
-    double p = 1.0 / 25.0;
-    p = fma(p, s2, 1.0 / 23.0);
-    p = fma(p, s2, 1.0 / 21.0);
-    p = fma(p, s2, 1.0 / 19.0);
-    p = fma(p, s2, 1.0 / 17.0);
-    p = fma(p, s2, 1.0 / 15.0);
-    p = fma(p, s2, 1.0 / 13.0);
-    p = fma(p, s2, 1.0 / 11.0);
-    p = fma(p, s2, 1.0 / 9.0);
-    p = fma(p, s2, 1.0 / 7.0);
-    p = fma(p, s2, 1.0 / 5.0);
-    p = fma(p, s2, 1.0 / 3.0);
-    p = fma(p, s2, 1.0);
-    double ln_m = 2.0 * s * p;
-    return fma(ln_m, 1.44269504088896340736, (double)e);
-}
-
-__device__ __forceinline__ double det_exp2(double t) {
-    if (t < -1100.0) return 0.0;
-    if (t > 1100.0) return INFINITY;
-    double n = rint(t);
-    double u = (t - n) * 0.69314718055994530942;
-    double p = 1.0 / 87178291200.0;
-    p = fma(p, u, 1.0 / 6227020800.0);
-    p = fma(p, u, 1.0 / 479001600.0);
-    p = fma(p, u, 1.0 / 39916800.0);
-    p = fma(p, u, 1.0 / 3628800.0);
-    p = fma(p, u, 1.0 / 362880.0);
-    p = fma(p, u, 1.0 / 40320.0);
-    p = fma(p, u, 1.0 / 5040.0);
-    p = fma(p, u, 1.0 / 720.0);
-    p = fma(p, u, 1.0 / 120.0);
-    p = fma(p, u, 1.0 / 24.0);
-    p = fma(p, u, 1.0 / 6.0);
-    p = fma(p, u, 0.5);
-    p = fma(p, u, 1.0);
-    p = fma(p, u, 1.0);
-    return ldexp(p, (int)n);
-}
-
-__device__ __forceinline__ float det_powf(float x, float y) {
-    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
-    if (x == 1.0f) return 1.0f;
-    return (float)det_exp2((double)y * det_log2((double)x));
-}
-
-__device__ __forceinline__ float det_expf(float x) { return (float)det_exp2((double)x * 1.44269504088896340736); }
-
-// --------------------------------------------------------------------------- block reductions
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-// (key, index) minimum: the key orders by CFL bits, then by the reference side id.
-__device__ __forceinline__ void wave_min_key(uint64_t &key, int32_t &idx) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint64_t k2 = __shfl_xor((unsigned long long)key, o, 64);
-        int32_t i2 = __shfl_xor(idx, o, 64);
-        if (k2 < key) {
-            key = k2;
-            idx = i2;
-        }
-    }
-}
-
-// --------------------------------------------------------------------------- clock
-
-__device__ __forceinline__ float ticks_local(uint64_t ticks) {
-    uint32_t hi = (uint32_t)(ticks >> 32), lo = (uint32_t)ticks;
-    return (float)hi * (1.0f / 256.0f) + (float)lo * (1.0f / TICKS_PER_SECOND_F);
-}
-
-// Exact F32 seconds -> 2^-40 s ticks, half-even below one tick; 0 when not representable.
-__device__ __forceinline__ uint64_t f32_ticks(float v) {
-    uint32_t bits = __float_as_uint(v);
-    int e = (int)((bits >> 23) & 255u) - 127;
-    if (!(v > 0.0f) || e < -40 || e > 0) return 0;
-    uint64_t mant = (uint64_t)((bits & 0x7fffffu) | 0x800000u);
-    int shift = e + 17;
-    if (shift >= 0) return mant << shift;
-    int right = -shift;
-    uint64_t t = mant >> right;
-    uint64_t dropped = mant & ((1ull << right) - 1ull);
-    uint64_t half = 1ull << (right - 1);
-    if (dropped > half || (dropped == half && (t & 1ull))) t += 1;
-    return t;
-}
-
-// StepClock.choose (core/clock.py:94-167) with u64 tick arithmetic. Writes the trace row's
-// status / candidate / event-short columns and the step's dt, local times and end tick.
-__device__ void clock_choose(Ctl *c, float candidate, float min_dt, int slot) {
-    uint32_t *row = &c->trace[slot * NHF_TRACE_COLS];
-    c->active = 0;
-    c->dt = 0.0f;
-    c->local = 0.0f;
-    c->end_local = 0.0f;
-    uint64_t now = c->now, target = c->next;
-    if (!(now < target)) return;
-    if (!(candidate >= min_dt)) {
-        row[3] = NHF_BAD_CFL;
-        return;
-    }
-    uint64_t rem = target - now;
-    float remaining = ticks_local(rem);
-    float capped = fmn(candidate, 1.0f);
-    float chosen = fmn(capped, remaining);
-    row[4] = __float_as_uint(candidate);
-    row[5] = chosen < capped ? 1u : 0u;
-    uint64_t dt_ticks = f32_ticks(chosen);
-    if (dt_ticks == 0) {
-        row[3] = NHF_BAD_DT;
-        return;
-    }
-    bool exceeds = dt_ticks > rem;
-    uint64_t diff = exceeds ? dt_ticks - rem : rem - dt_ticks;
-    uint64_t end;
-    if (diff < SNAP_TICKS) {
-        end = target;
-    } else if (exceeds) {
-        row[3] = NHF_OVERSHOOT;
-        return;
-    } else {
-        end = now + dt_ticks;
-    }
-    c->local = ticks_local(now - c->origin);
-    c->end_local = ticks_local(end - c->origin);
-    c->dt = chosen;
-    c->step_end = end;
-    c->active = 1;
-}
 
 // prepare_step (solver_compact.py:364-382), run by one thread.
 __device__ void prepare_slot(const Dev &d, int slot) {
@@ -287,20 +72,6 @@ __device__ void prepare_slot(const Dev &d, int slot) {
         c->limit_ids[1] = el;
         c->limit_ids[2] = eh;
     }
-}
-
-// finish_step + StepClock.commit (solver_compact.py:384-391, core/clock.py:173-181).
-__device__ void commit_slot(Ctl *c, int slot) {
-    if (!c->active) return;
-    uint32_t *row = &c->trace[slot * NHF_TRACE_COLS];
-    row[6] = c->limited_cells;
-    row[7] = c->limited_faces;
-    row[8] = c->min_scale_bits;
-    c->now = c->step_end;
-    row[0] = __float_as_uint(c->dt);
-    row[1] = (uint32_t)(c->step_end >> 32);
-    row[2] = (uint32_t)c->step_end;
-    row[3] = NHF_COMPLETED;
 }
 
 // --------------------------------------------------------------------------- small kernels
@@ -470,22 +241,6 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_face_avg(Dev d) {
     }
 }
 
-// Linear continuation of missing theta neighbours at prescribed faces
-// (boundary_neighbor_sum, solver_compact.py:135-149).
-__device__ __forceinline__ float neighbour_sum(float q, float left, float right, float fl, float fr) {
-    float result = left + right;
-    if (fl > 0.0f || fr > 0.0f) {
-        if (fl == 1.0f || fr == 1.0f) {
-            result = 2.0f * q;
-        } else {
-            float product = fl * fr;
-            result = ((1.0f - fr) * left + (1.0f - fl) * right + 2.0f * q * (fl + fr - 2.0f * product)) /
-                     (1.0f - product);
-        }
-    }
-    return result;
-}
-
 // --------------------------------------------------------------------------- (a) face flux
 
 __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
@@ -528,6 +283,9 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
     nq = wh < d.p.min_h ? 0.0f : nq;
     nq *= bf;
     d.qflux[si] = nq;
+    // An exterior side's discharge is +-0 (or NaN), which any donor scale in [0, 1] leaves as it
+    // is, so its final value is stored here; k_donor stores every side that has a real donor.
+    if (meta & NHF_SIDE_EXTERIOR) m.q[si] = nq;
 }
 
 // --------------------------------------------------------------------------- donor scale
@@ -538,7 +296,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
     int cell = xcd_block() * NHF_BLOCK + threadIdx.x + 1;
-    uint32_t limited = 0, scale_bits = __float_as_uint(1.0f);
+    uint32_t limited = 0, faces = 0, scale_bits = __float_as_uint(1.0f);
     if (cell < m.e_num) {
         int bi = meta_boundary(m.cell_meta[cell]);
         float discharge = 0.0f;
@@ -571,29 +329,39 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
             limited = 1;
             scale_bits = __float_as_uint(ds);
         }
-        d.donor[cell] = ds;
+        // Shared-subedge scaling (solver_compact.py:516-524): every side takes the scale of its
+        // donor, the lower cell when q >= 0 and the higher cell otherwise. This cell is the
+        // lower cell of its right/top sides, so it is the donor of a side exactly when
+        // (q >= 0) == (face is right/top), and it stores that side's scaled discharge: one
+        // writer per side, and the second pass over the list hits the cache.
+        k = m.csr_start[cell];
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            const bool upper = dir & 1;
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                float qf = d.qflux[side];
+                if ((qf >= 0.0f) == upper) {
+                    m.q[side] = qf * ds;
+                    if (qf != 0.0f && ds < 1.0f) faces += 1;
+                }
+            }
+        }
+        // the depth pass needs the scale itself only for boundary and pipe exchange
+        if (bi >= 0 || m.pipe_q) d.donor[cell] = ds;
     }
-    // limiter counters: one integer atomic per wavefront that limited a cell (rare)
-    if (__any(limited)) {
-        uint32_t wc = wave_sum_u(limited), wm = wave_min_u(scale_bits);
+    // limiter counters: one integer atomic per wavefront that limited a cell or face (rare)
+    if (__any(limited | faces)) {
+        uint32_t wc = wave_sum_u(limited), wf = wave_sum_u(faces), wm = wave_min_u(scale_bits);
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&c->limited_cells, wc);
-            atomicMin(&c->min_scale_bits, wm);
+            if (wc) atomicAdd(&c->limited_cells, wc);
+            if (wf) atomicAdd(&c->limited_faces, wf);
+            if (wc) atomicMin(&c->min_scale_bits, wm);
         }
     }
 }
 
 // --------------------------------------------------------------------------- (b) depth update
-
-// Neumaier step on a (sum, compensation) pair.
-__device__ __forceinline__ void acc_add(double2 &a, double v) {
-    double s = a.x + v;
-    if (fabs(a.x) >= fabs(v))
-        a.y += (a.x - s) + v;
-    else
-        a.y += (v - s) + a.x;
-    a.x = s;
-}
 
 __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
     Ctl *c = d.ctl;
@@ -607,12 +375,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
     float t_infil = 0.0f, t_floor = 0.0f, t_level_floor = 0.0f, t_pipe = 0.0f, t_disc = 0.0f;
     float t_lv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int bi = -1, kind = -1;
-    uint32_t faces_limited = 0;
     if (live) {
         const uint32_t meta = m.cell_meta[cell];
         bi = meta_boundary(meta);
         if (bi >= 0) kind = m.kind[bi];
-        float ds = d.donor[cell];
+        // the cell's own scale, stored by k_donor for boundary and pipe cells only
+        float ds = (bi >= 0 || m.pipe_q) ? d.donor[cell] : 1.0f;
         float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
         float pipe = fmx(pq, 0.0f) + fmn(pq, 0.0f) * ds;
         float discharge = 0.0f;
@@ -634,7 +402,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
         } else {
             cap = m.capacity[cell];
         }
-        // face totals of the donor-scaled discharge; the owner writes the scaled side value
+        // face totals of the donor-scaled discharge (stored in q by k_donor and k_flux)
         float fsum[4];
         uint32_t cnt = m.csr_count[cell];
         int k = m.csr_start[cell];
@@ -642,19 +410,9 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
         for (int dir = 0; dir < 4; ++dir) {
             int n = face_n(cnt, dir);
             float acc = 0.0f;
-            const bool upper = dir & 1;  // right/top faces: this cell is the side's lower cell
             for (int j = 0; j < n; ++j, ++k) {
                 int side = m.isl[k];
-                int nb = m.isl_cell[k];
-                float qf = d.qflux[side];
-                int donor_cell = (qf >= 0.0f) == upper ? cell : nb;
-                float scale = d.donor[donor_cell];
-                float qs = qf * scale;
-                acc += qs * m.length[side];
-                if (upper || nb == 0) {  // the side's single writer
-                    m.q[side] = qs;
-                    if (qf != 0.0f && scale < 1.0f) faces_limited += 1;
-                }
+                acc += m.q[side] * m.length[side];
             }
             fsum[dir] = acc;
         }
@@ -700,10 +458,6 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (__any(faces_limited)) {
-        uint32_t wf = wave_sum_u(faces_limited);
-        if (lane == 0) atomicAdd(&c->limited_faces, wf);
-    }
     // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
     // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it.
     double2 *acc = d.acc + ((size_t)blockIdx.x * NHF_WAVES + wave) * d.terms;
@@ -885,7 +639,7 @@ extern "C" {
 
 int32_t nhf_abi_version(void) { return NHF_ABI_VERSION; }
 
-const char *nhf_last_error(void) { return g_error; }
+const char *nhf_last_error(void) { return nhf_error_text; }
 
 size_t nhf_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params) {
     if (!mesh || !params) return 0;

@@ -180,6 +180,118 @@ def _set_forcing(eng, forcing, t: float):
                     rain.rates(rain.index_at(t)).astype(np.float32), True)
 
 
+def _run_protocol(engines, run_batch, steps: int, horizon: int, now: list, batch: int = 32) -> int:
+    """Advance every engine `steps` steps with the solver's batch protocol (begin, run, read the
+    trace); returns the completed steps summed over the engines."""
+    done, total = 0, 0
+    while done < steps:
+        n = min(batch, steps - done)
+        for i, eng in enumerate(engines):
+            eng.begin(horizon, now[i])
+        trace = run_batch(n)
+        for i in range(len(engines)):
+            rows = trace[i]
+            ok = rows[:, 3] == 1
+            if not ok.all():
+                raise RuntimeError(f'scenario {i}: batch stopped early, status {rows[~ok][0, 3]}')
+            now[i] = (int(rows[-1, 1]) << 32) | int(rows[-1, 2])
+            total += n
+        done += n
+    return total
+
+
+def batched_bench(args, device) -> dict:
+    """Same-mesh scenario batch: `--batch-slots` scenarios (rain peak, Manning factor and start
+    state drawn per scenario) on one shared two-level mesh, against one of them on the plain path."""
+    import torch
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.batch import BatchEngine
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state, relayout
+    t0 = time.time()
+    base = synth.two_level_fractal(args.seed, blocks=args.batch_blocks)
+    variants = synth.scenario_variants(base, args.batch_slots, seed=args.seed + 1)
+    mesh = base.mesh
+    cells = mesh.e_num - 1
+    cf = cell_forcing_arrays(mesh, synth.scenario_forcing(variants[0]))
+    layout0 = build_layout(mesh, variants[0].land_parameters, cf, base.sita)
+    params = make_params(base.afa, base.sita, base.min_h, 32)
+    log(f'batch workload: {len(variants)} scenarios x {cells} cells, {mesh.s_num - 1} sides ({time.time() - t0:.1f} s)')
+    t_start = 1800.
+    start = seconds_to_ticks(t_start)
+    horizon = seconds_to_ticks(t_start + 1e5)
+
+    def states():
+        for v in variants:
+            lay = relayout(layout0, mesh, v.land_parameters, cf)
+            st = initial_state(mesh, lay)
+            if v.initial_depth is not None:
+                st['depth'] = v.initial_depth.copy()
+            yield v, lay, st
+
+    # plain single-scenario path on the first scenario
+    v, lay, st = next(states())
+    plain = Engine(lay, params, st, device=device)
+    _set_forcing(plain, synth.scenario_forcing(v), t_start)
+    plain.reset(start)
+    plain.compute_cfl()
+    plain.set_graph(True)
+    now = [start]
+    one = lambda n: plain.run_batch(n)[None]
+    _run_protocol([plain], one, args.warmup, horizon, now)
+    plain.synchronize()
+    w0 = time.perf_counter()
+    steps_plain = _run_protocol([plain], one, args.steps, horizon, now)
+    plain.synchronize()
+    plain_wall = time.perf_counter() - w0
+    plain_value = cells * steps_plain / plain_wall
+    del plain
+    torch.cuda.synchronize()
+
+    batch = BatchEngine(len(variants), device=device)
+    views = []
+    for v, lay, st in states():
+        view = batch.view()(lay, params, st)
+        _set_forcing(view, synth.scenario_forcing(v), t_start)
+        view.reset(start)
+        views.append(view)
+    batch.compute_cfl()
+    batch.set_graph(True)
+    now = [start] * len(views)
+    _run_protocol(views, batch.run_batch, args.warmup, horizon, now)
+    batch.synchronize()
+    w0 = time.perf_counter()
+    steps_batch = _run_protocol(views, batch.run_batch, args.steps, horizon, now)
+    batch.synchronize()
+    batch_wall = time.perf_counter() - w0
+    batch_value = cells * steps_batch / batch_wall
+    # per-kernel times of the batched step (eager pass, HIP events)
+    batch.set_graph(False)
+    batch.set_timing(True)
+    batch.kernel_times(reset=True)
+    _run_protocol(views, batch.run_batch, args.kernel_steps, horizon, now)
+    ms, count = batch.kernel_times(reset=True)
+    batch.set_timing(False)
+    kernels = {name: dict(avg_us=ms[i] / count[i] * 1e3, launches=int(count[i]))
+               for i, name in enumerate(KERNELS) if count[i]}
+    sim_seconds = [(n - start) / 2 ** 40 for n in now]
+    log(f'batched: {steps_batch} scenario-steps in {batch_wall:.3f} s, {batch_value / 1e9:.2f} G cell-updates/s; '
+        f'plain: {plain_value / 1e9:.2f} G; ratio {batch_value / plain_value:.2f}')
+    log('batch kernels: ' + ', '.join(f"{k} {v['avg_us']:.1f} us" for k, v in kernels.items()))
+    batch.close()
+    return {
+        'workload': f'{len(variants)} scenarios on one shared 2-level (10/5 m) fractal-DEM mesh, {cells} cells, '
+                    f'{mesh.s_num - 1} sides; rain peak U[20,150] mm/h, Manning x lognormal(0.2), dry or 0.3 m '
+                    'pond start, own dt per scenario',
+        'scenarios': len(variants), 'cells': cells, 'value': batch_value, 'unit': 'cell-updates/s',
+        'plain_value': plain_value, 'batched_vs_plain': batch_value / plain_value,
+        'scenario_steps': steps_batch, 'wall_s': batch_wall, 'plain_steps': steps_plain,
+        'simulated_s_min': min(sim_seconds), 'simulated_s_max': max(sim_seconds), 'kernels': kernels,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -191,6 +303,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='also copy a 256 MiB buffer 3 times (reference dispatch for HBM counters)')
+    ap.add_argument('--batch-slots', type=int, default=64, help='scenarios of the same-mesh batch (0: skip)')
+    ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
     ap.add_argument('--pmc-summary', nargs=3, metavar=('FETCH_CSV', 'WRITE_CSV', 'BENCH_JSON'),
                     help='summarise two rocprofv3 --pmc runs of this script (no GPU needed) and exit')
     args = ap.parse_args()
@@ -204,7 +318,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
-              '(independent scenarios would shard one per GPU; see DESIGN.md)', file=sys.stderr)
+              '(independent scenarios would shard whole scenarios per GPU)', file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -325,6 +439,8 @@ def main():
                 dst.copy_(src)
         device.stream.synchronize()
         result['copy_calibration_bytes'] = 2 * src.numel() * 4
+    if args.batch_slots:
+        result['batched'] = batched_bench(args, device)
     if not args.no_cpu:
         del eng
         torch.cuda.synchronize()

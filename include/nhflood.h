@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 1
+#define NHF_ABI_VERSION 2
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -183,6 +183,62 @@ int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, in
 int32_t nhf_clock_probe(const uint64_t *now, const uint64_t *target, const uint64_t *origin,
                         const float *candidate, float min_dt, int32_t n, uint32_t *out_words,
                         uint64_t *out_ticks, float *out_times);
+
+/* ------------------------------------------------------------------------------------------
+ * Scenario batch: up to NHB_MAX_SLOTS scenarios ("slots") on one shared mesh, advanced by the
+ * same launches, each with its own state, forcing, clock, dt, CFL, trace and budget.
+ *
+ * The reference runs one scenario per process and has no batched entry point; each nhb_*
+ * function is the per-slot form of the nhf_* function of the same name above (same semantics,
+ * same reference lines), so a slot behaves exactly like a context of its own. Per-scenario
+ * arrays are laid out [entity][slot] (the slot index innermost, `slots` words per cell or
+ * side); the mesh's static arrays are shared. The state pointers inside nhf_mesh are ignored.
+ * ------------------------------------------------------------------------------------------ */
+#define NHB_MAX_SLOTS 64
+
+typedef struct nhb_state {
+    /* per-scenario state, [e_num * slots] or [s_num * slots] (q), device pointers */
+    float *depth, *rain_time, *capacity, *max_depth, *max_speed;
+    uint64_t *peak_tick;
+    float *q;
+    const float *manning;        /* [s_num * slots] per-scenario side Manning n, or NULL to share
+                                    mesh->manning */
+    /* boundary cells (static): bslot[cell] = index into the list or -1; the list is sorted by
+       boundary, and bcell_start[b] .. bcell_start[b + 1] are boundary b's cells */
+    const int32_t *bslot;        /* [e_num] */
+    const int32_t *bcell_start;  /* [b_num + 1] */
+    int32_t n_bcells, reserved;
+} nhb_state;
+
+typedef struct nhb_ctx nhb_ctx;
+
+/* Workspace bytes for `slots` scenarios (a power of two, 1..NHB_MAX_SLOTS). */
+size_t nhb_workspace_bytes(const nhf_mesh *mesh, const nhb_state *state, const nhf_params *params,
+                           int32_t slots);
+/* nhf_create for a batch; every slot starts idle at tick 0. pipe_q must be NULL. */
+int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, const nhf_params *params,
+                   int32_t slots, void *workspace, size_t workspace_bytes, void *stream);
+int32_t nhb_destroy(nhb_ctx *ctx);
+/* nhf_reset of one slot (clock, budget, limiter capture). */
+int32_t nhb_reset(nhb_ctx *ctx, int32_t slot, uint64_t now_ticks);
+/* nhf_set_forcing / nhf_begin of one slot: staged on the host, applied by the next
+ * nhb_run_batch or nhb_compute_cfl in one upload for all slots. */
+int32_t nhb_set_forcing(nhb_ctx *ctx, int32_t slot, const float *coefficients, const float *rain_rates,
+                        int32_t raining);
+int32_t nhb_begin(nhb_ctx *ctx, int32_t slot, uint64_t target_ticks, uint64_t origin_ticks);
+/* nhf_run_batch for every slot at once: each slot takes up to `steps` steps towards its own
+ * target with its own dt; trace receives slots x steps x 9 words. */
+int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace);
+/* nhf_compute_cfl for every slot. */
+int32_t nhb_compute_cfl(nhb_ctx *ctx);
+/* nhf_flush_budget / nhf_limit_capture / nhf_cfl_state of one slot. */
+int32_t nhb_flush_budget(nhb_ctx *ctx, int32_t slot, double *out);
+int32_t nhb_limit_capture(nhb_ctx *ctx, int32_t slot, float *values, int32_t *ids, int32_t reset);
+int32_t nhb_cfl_state(nhb_ctx *ctx, int32_t slot, float *ndt, int32_t *side);
+int32_t nhb_sync(nhb_ctx *ctx);
+int32_t nhb_set_graph(nhb_ctx *ctx, int32_t enable);
+int32_t nhb_set_timing(nhb_ctx *ctx, int32_t enable);
+int32_t nhb_kernel_times(nhb_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);
 
 #ifdef __cplusplus
 }

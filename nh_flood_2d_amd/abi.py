@@ -5,9 +5,10 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 TRACE_COLS = 9
 MAX_BATCH = 64
+MAX_SLOTS = 64
 DONE, COMPLETED, BAD_CFL, BAD_DT, OVERSHOOT = range(5)
 SIDE_EXTERIOR = 4
 
@@ -43,6 +44,15 @@ class MeshStruct(ctypes.Structure):
     _fields_ = [(name, c_int32 if kind == 'i' else c_void_p) for name, kind in MESH_FIELDS]
 
 
+# nhb_state: per-scenario state of a batch ([entity][slot] device arrays) and the boundary cells
+BATCH_STATE_FIELDS = ['depth', 'rain_time', 'capacity', 'max_depth', 'max_speed', 'peak_tick', 'q',
+                      'manning', 'bslot', 'bcell_start']
+
+
+class BatchState(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in BATCH_STATE_FIELDS] + [('n_bcells', c_int32), ('reserved', c_int32)]
+
+
 def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
     """Declare every entry point of include/nhflood.h on a loaded library."""
     vp = c_void_p
@@ -66,6 +76,25 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
         'nhf_clock_probe': (c_int32, [vp, vp, vp, vp, c_float, c_int32, vp, vp, vp]),
     }
+    if hasattr(lib, 'nhb_create'):
+        sig.update({
+            'nhb_workspace_bytes': (c_size_t, [POINTER(MeshStruct), POINTER(BatchState), POINTER(Params), c_int32]),
+            'nhb_create': (c_int32, [POINTER(vp), POINTER(MeshStruct), POINTER(BatchState), POINTER(Params),
+                                     c_int32, vp, c_size_t, vp]),
+            'nhb_destroy': (c_int32, [vp]),
+            'nhb_reset': (c_int32, [vp, c_int32, c_uint64]),
+            'nhb_set_forcing': (c_int32, [vp, c_int32, vp, vp, c_int32]),
+            'nhb_begin': (c_int32, [vp, c_int32, c_uint64, c_uint64]),
+            'nhb_run_batch': (c_int32, [vp, c_int32, vp]),
+            'nhb_compute_cfl': (c_int32, [vp]),
+            'nhb_flush_budget': (c_int32, [vp, c_int32, vp]),
+            'nhb_limit_capture': (c_int32, [vp, c_int32, vp, vp, c_int32]),
+            'nhb_cfl_state': (c_int32, [vp, c_int32, vp, vp]),
+            'nhb_sync': (c_int32, [vp]),
+            'nhb_set_graph': (c_int32, [vp, c_int32]),
+            'nhb_set_timing': (c_int32, [vp, c_int32]),
+            'nhb_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
+        })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -78,7 +107,10 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
 EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_create',
             'nhf_destroy', 'nhf_reset', 'nhf_set_forcing', 'nhf_begin', 'nhf_run_batch',
             'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
-            'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe']
+            'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe',
+            'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
+            'nhb_run_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
+            'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times']
 
 _hip_lib = None
 
@@ -98,6 +130,6 @@ def check(lib, rc: int, what: str) -> None:
         raise RuntimeError(f'{what} failed ({rc}): {lib.nhf_last_error().decode(errors="replace")}')
 
 
-__all__ = ['Params', 'MeshStruct', 'MESH_FIELDS', 'bind', 'load_hip_library', 'check',
+__all__ = ['Params', 'MeshStruct', 'MESH_FIELDS', 'BatchState', 'BATCH_STATE_FIELDS', 'MAX_SLOTS', 'bind', 'load_hip_library', 'check',
            'TRACE_COLS', 'MAX_BATCH', 'DONE', 'COMPLETED', 'BAD_CFL', 'BAD_DT', 'OVERSHOOT',
            'c_double', 'c_int64', 'c_uint32']

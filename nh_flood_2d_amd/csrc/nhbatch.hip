@@ -1,0 +1,1094 @@
+// libnhflood.so, scenario-batched step: up to 64 scenarios ("slots") on one shared mesh, each
+// with its own state, forcing, clock, dt, CFL, limiter counters, trace and water budget, advanced
+// together by the same launches.
+//
+// Layout: every per-scenario array is [entity][slot] with the slot innermost, so the S slots of
+// one cell or side are S consecutive words: a wavefront (64 lanes = 64 / S entities x S slots)
+// reads state in whole cache lines, and the static mesh arrays (side lists, lengths, ids, bed,
+// area, ...) are fetched once per entity for all of its slots.
+//
+// Each slot runs exactly the single-mesh step (nhflood.hip) with the same arithmetic in the same
+// order, so a scenario gives the same bits alone and in a batch; only the FP64 budget is summed
+// in a different grouping. Slot s is idle in a step when dt[s] == 0 (its clock is at its target,
+// failed, or the slot is unused): its lanes load and store nothing.
+//
+// Kernels per step (reference lines as in nhflood.hip):
+//   kb_gradients, kb_face_avg (2:1 meshes, sita < 1), kb_flux, kb_donor, kb_depth (grid-stride,
+//   per-thread FP64 budget), kb_cfl (grid-stride, per-slot partials), kb_finish (one workgroup
+//   per slot: CFL reduction, boundary budget terms, commit, prepare).
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "nhf_device.h"
+
+struct BDev {
+    nhf_mesh m;  // static mesh arrays (the per-scenario state pointers of nhf_mesh are unused)
+    nhf_params p;
+    int32_t shift, terms, cfl_blocks, depth_blocks, n_bcells, theta_one, b_num, n_pixels;
+    // per-scenario state, [entity][slot]
+    float *depth, *rain_time, *capacity, *max_depth, *max_speed, *q;
+    uint64_t *peak_tick;
+    const float *manning;  // [s_num][slot] or NULL: mesh manning shared by every slot
+    // boundary-cell list (static)
+    const int32_t *bslot, *bcell_start;
+    // scratch
+    Ctl *ctl;                     // [slot]
+    float *dt;                    // [slot]; 0 = the slot does not step
+    int32_t *raining;             // [slot]
+    float *coefficients;          // [slot][b][2]
+    float *values, *level_values; // [slot][b]
+    float *rain_rates;            // [slot][n_pixels]
+    float *bdonor;                // [boundary cell][slot]
+    float *qflux;                 // [s_num][slot]
+    float *face_qn;               // [e_num][4][slot]
+    float *grad;                  // [n_grad][2][slot]
+    uint64_t *cfl_key;            // [cfl_blocks][slot]
+    int32_t *cfl_idx;
+    double2 *acc;                 // [depth_blocks][2][slot]: infiltration, non-level floor
+    float *bvals;                 // [boundary cell][6][slot]
+    double2 *bacc;                // [slot][terms]: boundary-cell terms
+    double2 *flush;               // [terms]
+};
+
+// Host -> device staging of one batch: begin words and forcing per slot, uploaded in one copy.
+struct Stage {
+    uint64_t *target, *origin;  // [slot]
+    int32_t *flags;             // [slot]: see STAGE_*
+    float *coeff;               // [slot][b][2]
+    float *rain;                // [slot][n_pixels]
+};
+enum { STAGE_BEGIN = 1, STAGE_COEFF = 2, STAGE_RAIN = 4, STAGE_RAINING = 8, STAGE_RAINING_ON = 16 };
+
+#define SLOTS (1 << d.shift)
+#define IX(e) ((((size_t)(e)) << d.shift) | (size_t)s)
+
+// --------------------------------------------------------------------------- clock per slot
+
+// prepare_step (solver_compact.py:364-382) for slot s.
+__device__ void prepare_slot_b(const BDev &d, int s, int step) {
+    Ctl *c = &d.ctl[s];
+    clock_choose(c, c->ndt, d.p.min_dt, step);
+    d.dt[s] = c->active ? c->dt : 0.0f;
+    if (!c->active) return;
+    c->limited_cells = 0;
+    c->limited_faces = 0;
+    c->min_scale_bits = __float_as_uint(1.0f);
+    const float dt = c->dt, local = c->local, end_local = c->end_local;
+    for (int bi = 0; bi < d.b_num; ++bi) {
+        float c0 = d.coefficients[(s * d.b_num + bi) * 2], c1 = d.coefficients[(s * d.b_num + bi) * 2 + 1];
+        d.values[s * d.b_num + bi] = c0 + c1 * (local + dt / 2.0f);
+        d.level_values[s * d.b_num + bi] = c0 + c1 * end_local;
+    }
+    if (c->ndt < c->limit_value[0]) {
+        int si = c->cfl_arg;
+        int el = d.m.side_cells[2 * si], eh = d.m.side_cells[2 * si + 1];
+        float zl = d.m.bed[el], zh = d.m.bed[eh], hl = d.depth[IX(el)], hh = d.depth[IX(eh)];
+        float head = fmx(zl + hl, zh + hh);
+        float h = fmx(head - fmx(fmx(zl, zh), -999.0f), 0.01f);
+        c->limit_value[0] = c->ndt;
+        c->limit_value[1] = hl;
+        c->limit_value[2] = hh;
+        c->limit_value[3] = h;
+        c->limit_value[4] = d.q[IX(si)] / fmx(h, 1e-30f);
+        c->limit_ids[0] = si;
+        c->limit_ids[1] = el;
+        c->limit_ids[2] = eh;
+    }
+}
+
+// --------------------------------------------------------------------------- small kernels
+
+__global__ void kb_reset(BDev d, int s, uint64_t now) {
+    const int S = SLOTS;
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int stride = gridDim.x * blockDim.x;
+    if (tid == 0) {
+        Ctl *c = &d.ctl[s];
+        c->now = c->next = c->origin = c->step_end = now;
+        c->dt = c->local = c->end_local = 0.0f;
+        c->active = 0;
+        c->slot = 0;
+        c->limited_cells = c->limited_faces = 0;
+        c->min_scale_bits = __float_as_uint(1.0f);
+        c->limit_value[0] = INFINITY;
+        d.dt[s] = 0.0f;
+    }
+    for (int i = tid; i < d.depth_blocks * 2; i += stride) d.acc[(size_t)i * S + s] = make_double2(0.0, 0.0);
+    for (int i = tid; i < d.terms; i += stride) d.bacc[(size_t)s * d.terms + i] = make_double2(0.0, 0.0);
+}
+
+// Apply the staged begin words and forcing: one workgroup per slot.
+__global__ void kb_apply(BDev d, Stage st) {
+    const int s = blockIdx.x;
+    const int flags = st.flags[s];
+    Ctl *c = &d.ctl[s];
+    if (threadIdx.x == 0) {
+        if (flags & STAGE_BEGIN) {
+            c->next = st.target[s];
+            c->origin = st.origin[s];
+            c->slot = 0;
+        }
+        if (flags & STAGE_RAINING) d.raining[s] = (flags & STAGE_RAINING_ON) ? 1 : 0;
+    }
+    if (flags & STAGE_COEFF)
+        for (int i = threadIdx.x; i < 2 * d.b_num; i += blockDim.x)
+            d.coefficients[s * 2 * d.b_num + i] = st.coeff[s * 2 * d.b_num + i];
+    if (flags & STAGE_RAIN)
+        for (int i = threadIdx.x; i < d.n_pixels; i += blockDim.x)
+            d.rain_rates[(size_t)s * d.n_pixels + i] = st.rain[(size_t)s * d.n_pixels + i];
+}
+
+// First step of a batch: clear the trace rows of every slot (one workgroup per slot), then
+// prepare step 0.
+__global__ void kb_prepare(BDev d, int steps) {
+    const int s = blockIdx.x;
+    for (int i = threadIdx.x; i < steps * NHF_TRACE_COLS; i += blockDim.x) d.ctl[s].trace[i] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) prepare_slot_b(d, s, 0);
+}
+
+__global__ void kb_reset_limit(BDev d, int s) {
+    if (threadIdx.x == 0) d.ctl[s].limit_value[0] = INFINITY;
+}
+
+// --------------------------------------------------------------------------- hanging gradients
+
+__device__ void face_difference_b(const BDev &d, int s, int cell, int dir, int axis, float &value,
+                                  float &distance, int &usable) {
+    const nhf_mesh &m = d.m;
+    float v = 0.0f, dist = 0.0f, total = 0.0f;
+    int ok = 1;
+    float eta = m.bed[cell] + d.depth[IX(cell)];
+    uint32_t cnt = m.csr_count[cell];
+    int start = m.csr_start[cell];
+    for (int k = 0; k < dir; ++k) start += face_n(cnt, k);
+    int n = face_n(cnt, dir);
+    for (int j = 0; j < n; ++j) {
+        int side = m.isl[start + j];
+        int other = m.isl_cell[start + j];
+        if (other <= 0 || d.depth[IX(other)] <= d.p.min_h) {
+            ok = 0;
+        } else {
+            float w = m.length[side];
+            v += w * ((m.bed[other] + d.depth[IX(other)]) - eta);
+            float delta = axis == 1 ? m.y[other] - m.y[cell] : m.x[other] - m.x[cell];
+            dist += w * delta;
+            total += w;
+        }
+    }
+    if (total > 0.0f) {
+        v /= total;
+        dist /= total;
+    } else {
+        ok = 0;
+    }
+    value = v;
+    distance = dist;
+    usable = ok;
+}
+
+__global__ __launch_bounds__(NHF_BLOCK) void kb_gradients(BDev d) {
+    const int S = SLOTS;
+    size_t t = (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x;
+    int g = (int)(t >> d.shift);
+    int s = (int)(t & (size_t)(S - 1));
+    if (g >= d.m.n_grad || d.dt[s] == 0.0f) return;
+    int cell = d.m.grad_cell[g];
+    int axes = d.m.grad_axes[g];
+    float h = d.depth[IX(cell)];
+    for (int axis = 0; axis < 2; ++axis) {
+        if (!(axes & (1 << axis))) continue;
+        float value = 0.0f;
+        if (h > d.p.min_h) {
+            float dm, xm, dp, xp;
+            int okm, okp;
+            face_difference_b(d, s, cell, axis == 1 ? 2 : 0, axis, dm, xm, okm);
+            face_difference_b(d, s, cell, axis == 1 ? 3 : 1, axis, dp, xp, okp);
+            if (okm && okp)
+                value = (dp - dm) / (xp - xm);
+            else if (okm)
+                value = dm / xm;
+            else if (okp)
+                value = dp / xp;
+        }
+        d.grad[IX(2 * g + axis)] = value;
+    }
+}
+
+// --------------------------------------------------------------------------- theta < 1
+
+__global__ __launch_bounds__(NHF_BLOCK) void kb_face_avg(BDev d) {
+    const int S = SLOTS;
+    size_t t = (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x;
+    int cell = (int)(t >> d.shift);
+    int s = (int)(t & (size_t)(S - 1));
+    if (cell >= d.m.e_num || d.dt[s] == 0.0f) return;
+    const nhf_mesh &m = d.m;
+    uint32_t cnt = m.csr_count[cell];
+    int start = m.csr_start[cell];
+    for (int dir = 0; dir < 4; ++dir) {
+        int n = face_n(cnt, dir);
+        float result = 0.0f;
+        if (n == 1) {
+            result = d.q[IX(m.isl[start])];
+        } else if (n > 1) {
+            float total = 0.0f, width = 0.0f;
+            for (int j = 0; j < n; ++j) {
+                int side = m.isl[start + j];
+                total += d.q[IX(side)] * m.length[side];
+                width += m.length[side];
+            }
+            result = total / width;
+        }
+        d.face_qn[IX(4 * cell + dir)] = result;
+        start += n;
+    }
+}
+
+// --------------------------------------------------------------------------- (a) face flux
+
+__global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
+    const int S = SLOTS;
+    size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
+    int si = (int)(t >> d.shift) + 1;
+    int s = (int)(t & (size_t)(S - 1));
+    if (si >= d.m.s_num) return;
+    const float dt = d.dt[s];
+    if (dt == 0.0f) return;
+    const nhf_mesh &m = d.m;
+    const float g = d.p.g;
+    int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
+    uint32_t meta = m.side_meta[si];
+    float bf = (meta & NHF_SIDE_EXTERIOR) ? 0.0f : 1.0f;
+    float zl = m.bed[lh.x], zh = m.bed[lh.y];
+    float dl = d.depth[IX(lh.x)], dh = d.depth[IX(lh.y)];
+    float hl = zl + dl, hr = zh + dh;
+    float q = d.q[IX(si)];
+    float wh = fmx(hl, hr) - fmx(fmx(zl, zh), -999.0f);
+    float difference = hr - hl;
+    if (m.edge_index) {
+        int idx = m.edge_index[si];
+        if (idx >= 0 && dl > d.p.min_h && dh > d.p.min_h) {
+            int axis = (meta & NHF_SIDE_ORIENT_MASK) == 2 ? 1 : 0;
+            float o0 = m.hang_offset[2 * idx], o1 = m.hang_offset[2 * idx + 1];
+            if (o0 != 0.0f) difference -= d.grad[IX(2 * m.hang_slot[2 * idx] + axis)] * o0;
+            if (o1 != 0.0f) difference += d.grad[IX(2 * m.hang_slot[2 * idx + 1] + axis)] * o1;
+        }
+    }
+    float dq = ((-g * difference) / m.sdc[si]) * fmx(wh, 0.0f) * dt;
+    float n = d.manning ? d.manning[IX(si)] : m.manning[si];
+    float f = 1.0f;
+    if (q != 0.0f) f = 1.0f + g * dt * (n * n) * fabsf(q / det_powf(fmx(wh, 0.00001f), 7.0f / 3.0f));
+    float neighbours = 0.0f;
+    if (!d.theta_one) {
+        int vertical = (meta & NHF_SIDE_ORIENT_MASK) == 2;
+        int fa = vertical ? 0 : 2, fb = vertical ? 1 : 3;
+        neighbours = neighbour_sum(q, d.face_qn[IX(4 * lh.x + fa)], d.face_qn[IX(4 * lh.y + fb)],
+                                   m.open_face[4 * lh.x + fa], m.open_face[4 * lh.y + fb]);
+    }
+    float nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / f;
+    nq = wh < d.p.min_h ? 0.0f : nq;
+    nq *= bf;
+    d.qflux[IX(si)] = nq;
+    if (meta & NHF_SIDE_EXTERIOR) d.q[IX(si)] = nq;  // final value, as in k_flux
+}
+
+// --------------------------------------------------------------------------- donor scale
+
+__global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
+    const int S = SLOTS;
+    size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
+    int cell = (int)(t >> d.shift) + 1;
+    int s = (int)(t & (size_t)(S - 1));
+    if (cell >= d.m.e_num) return;
+    const float dt = d.dt[s];
+    if (dt == 0.0f) return;
+    const nhf_mesh &m = d.m;
+    Ctl *c = &d.ctl[s];
+    int bi = meta_boundary(m.cell_meta[cell]);
+    float discharge = 0.0f;
+    bool level = false;
+    if (bi >= 0) {
+        int kind = m.kind[bi];
+        if (kind == 1) discharge = d.values[s * d.b_num + bi] * m.bweight[cell];
+        level = kind == 0;
+    }
+    float rainq = d.raining[s] ? d.rain_rates[(size_t)s * d.n_pixels + (d.n_pixels > 1 ? m.rain_index[cell] : 0)] : 0.0f;
+    float area = m.area[cell];
+    float positive = (fmx(0.0f, 0.0f) + fmx(discharge, 0.0f)) * dt;
+    float available = d.depth[IX(cell)] * area + positive + rainq * dt * area;
+    float outgoing = (fmx(-0.0f, 0.0f) + fmx(-discharge, 0.0f)) * dt;
+    uint32_t cnt = m.csr_count[cell];
+    const int k0 = m.csr_start[cell];
+    int k = k0;
+    for (int dir = 0; dir < 4; ++dir) {
+        int n = face_n(cnt, dir);
+        for (int j = 0; j < n; ++j, ++k) {
+            int side = m.isl[k];
+            float sq = d.qflux[IX(side)] * m.length[side];
+            if (dir == 0 || dir == 2) sq = -sq;
+            outgoing += fmx(sq, 0.0f) * dt;
+        }
+    }
+    float ds = 1.0f;
+    bool limited = false;
+    if (outgoing > available && !level) {
+        ds = available / outgoing;
+        limited = true;
+    }
+    uint32_t faces = 0;
+    k = k0;
+    for (int dir = 0; dir < 4; ++dir) {
+        int n = face_n(cnt, dir);
+        const bool upper = dir & 1;
+        for (int j = 0; j < n; ++j, ++k) {
+            int side = m.isl[k];
+            float qf = d.qflux[IX(side)];
+            if ((qf >= 0.0f) == upper) {
+                d.q[IX(side)] = qf * ds;
+                if (qf != 0.0f && ds < 1.0f) faces += 1;
+            }
+        }
+    }
+    if (bi >= 0) d.bdonor[IX(d.bslot[cell])] = ds;
+    if (limited) {
+        atomicAdd(&c->limited_cells, 1u);
+        atomicMin(&c->min_scale_bits, __float_as_uint(ds));
+    }
+    if (faces) atomicAdd(&c->limited_faces, faces);
+}
+
+// --------------------------------------------------------------------------- (b) depth update
+
+// Grid-stride over (cell, slot): the stride is a multiple of the slot count, so each thread
+// keeps one slot and sums its cells' infiltration and floor terms in FP64 registers; the block
+// then adds them to its own compensated accumulators. Boundary cells leave their terms in bvals
+// for kb_finish.
+__global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
+    const int S = SLOTS;
+    const int s = threadIdx.x & (S - 1);
+    const float dt = d.dt[s];
+    const nhf_mesh &m = d.m;
+    const Ctl *c = &d.ctl[s];
+    double a_infil = 0.0, a_floor = 0.0;
+    if (dt != 0.0f) {
+        const size_t total = (size_t)(m.e_num - 1) << d.shift;
+        const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
+        const int raining = d.raining[s];
+        const uint64_t step_end = c->step_end;
+        for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += stride) {
+            const int cell = (int)(t >> d.shift) + 1;
+            const uint32_t meta = m.cell_meta[cell];
+            const int bi = meta_boundary(meta);
+            const int kind = bi >= 0 ? m.kind[bi] : -1;
+            const int j = bi >= 0 ? d.bslot[cell] : -1;
+            float ds = bi >= 0 ? d.bdonor[IX(j)] : 1.0f;
+            float pipe = fmx(0.0f, 0.0f) + fmn(0.0f, 0.0f) * ds;
+            float discharge = 0.0f;
+            if (kind == 1) {
+                float requested = d.values[s * d.b_num + bi] * m.bweight[cell];
+                discharge = fmx(requested, 0.0f) + fmn(requested, 0.0f) * ds;
+            }
+            float ssq = pipe + discharge;
+            float rainq = raining ? d.rain_rates[(size_t)s * d.n_pixels + (d.n_pixels > 1 ? m.rain_index[cell] : 0)] : 0.0f;
+            float cap;
+            if (rainq > 0.0f) {
+                float rt = d.rain_time[IX(cell)] + dt;
+                d.rain_time[IX(cell)] = rt;
+                const float *hp = m.class_horton + 3 * meta_class(meta);
+                float f0 = hp[0], fc = hp[1], hk = hp[2];
+                cap = fc + (f0 - fc) * det_expf(-hk * rt / 3600.0f);
+                d.capacity[IX(cell)] = cap;
+            } else {
+                cap = d.capacity[IX(cell)];
+            }
+            float fsum[4];
+            uint32_t cnt = m.csr_count[cell];
+            int k = m.csr_start[cell];
+#pragma unroll
+            for (int dir = 0; dir < 4; ++dir) {
+                int n = face_n(cnt, dir);
+                float acc = 0.0f;
+                for (int jj = 0; jj < n; ++jj, ++k) {
+                    int side = m.isl[k];
+                    acc += d.q[IX(side)] * m.length[side];
+                }
+                fsum[dir] = acc;
+            }
+            float ql = fsum[0], qr = fsum[1], qb = fsum[2], qt = fsum[3];
+            float tq = ql - qr + qb - qt + ssq;
+            float area = m.area[cell];
+            float rain_depth = rainq * dt;
+            float increment = tq * dt / area + rain_depth;
+            float previous = d.depth[IX(cell)];
+            float raw = previous + increment;
+            float removed = fmn(cap * dt, fmx(raw, 0.0f));
+            raw -= removed;
+            float floor_v = fmx(-raw, 0.0f);
+            a_infil += (double)(removed * area);
+            float h_new;
+            if (kind == 0) {
+                float target = fmx(d.level_values[s * d.b_num + bi] - m.bed[cell], 0.0f);
+                float *bv = d.bvals;
+                bv[IX(6 * j + 0)] = floor_v * area;
+                bv[IX(6 * j + 1)] = (target - previous) * area;
+                bv[IX(6 * j + 2)] = -(ql - qr + qb - qt) * dt;
+                bv[IX(6 * j + 3)] = -pipe * dt;
+                bv[IX(6 * j + 4)] = -rain_depth * area;
+                bv[IX(6 * j + 5)] = removed * area;
+                h_new = target;
+            } else {
+                a_floor += (double)(floor_v * area);
+                if (kind == 1) d.bvals[IX(6 * j + 0)] = discharge * dt;
+                h_new = fmx(raw, 0.0f);
+            }
+            d.depth[IX(cell)] = h_new;
+            float hv = fmx(h_new, 0.01f);
+            float u = 0.0f, v = 0.0f;
+            if (h_new >= d.p.min_h) {
+                const float size = m.size[cell];
+                u = (ql + qr) / size / hv / 2.0f;
+                v = (qb + qt) / size / hv / 2.0f;
+            }
+            float speed = sqrtf(u * u + v * v);
+            if (speed > d.max_speed[IX(cell)]) d.max_speed[IX(cell)] = speed;
+            if (h_new > d.max_depth[IX(cell)]) {
+                d.max_depth[IX(cell)] = h_new;
+                d.peak_tick[IX(cell)] = step_end;
+            }
+        }
+    }
+    // per-slot block totals: lanes of one slot within a wavefront, then the wavefronts
+    for (int o = 32; o >= S; o >>= 1) {
+        a_infil += __shfl_xor(a_infil, o, 64);
+        a_floor += __shfl_xor(a_floor, o, 64);
+    }
+    __shared__ double s_part[NHF_WAVES][2][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < S) {
+        s_part[wave][0][lane] = a_infil;
+        s_part[wave][1][lane] = a_floor;
+    }
+    __syncthreads();
+    if (threadIdx.x < S && dt != 0.0f) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int w = 0; w < NHF_WAVES; ++w) {
+            t0 += s_part[w][0][threadIdx.x];
+            t1 += s_part[w][1][threadIdx.x];
+        }
+        double2 *a = d.acc + ((size_t)blockIdx.x * 2) * S + s;
+        double2 v0 = a[0], v1 = a[S];
+        acc_add(v0, t0);
+        acc_add(v1, t1);
+        a[0] = v0;
+        a[S] = v1;
+    }
+}
+
+// --------------------------------------------------------------------------- (c) CFL dt
+
+__global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
+    const int S = SLOTS;
+    const int s = threadIdx.x & (S - 1);
+    const bool on = mode == 0 || d.dt[s] != 0.0f;
+    const nhf_mesh &m = d.m;
+    uint64_t key = ~0ull;
+    int32_t idx = 0;
+    if (on) {
+        const float afa = d.p.afa, g = d.p.g;
+        const size_t total = (size_t)(m.s_num - 1) << d.shift;
+        const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
+        for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += stride) {
+            const int si = (int)(t >> d.shift) + 1;
+            int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
+            if (lh.x != 0 && lh.y != 0) {
+                float zl = m.bed[lh.x], zh = m.bed[lh.y];
+                float head = fmx(zl + d.depth[IX(lh.x)], zh + d.depth[IX(lh.y)]);
+                float h = fmx(head - fmx(fmx(zl, zh), -999.0f), 0.01f);
+                float cfl = afa * m.length[si] / (sqrtf(g * h) + fabsf(d.q[IX(si)]) / h);
+                uint64_t k2 = ((uint64_t)__float_as_uint(cfl) << 32) | m.side_key[si];
+                if (k2 < key) {
+                    key = k2;
+                    idx = si;
+                }
+            }
+        }
+    }
+    for (int o = 32; o >= S; o >>= 1) {
+        uint64_t k2 = __shfl_xor((unsigned long long)key, o, 64);
+        int32_t i2 = __shfl_xor(idx, o, 64);
+        if (k2 < key) {
+            key = k2;
+            idx = i2;
+        }
+    }
+    __shared__ uint64_t s_key[NHF_WAVES][64];
+    __shared__ int32_t s_idx[NHF_WAVES][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < S) {
+        s_key[wave][lane] = key;
+        s_idx[wave][lane] = idx;
+    }
+    __syncthreads();
+    if (threadIdx.x < S) {
+        for (int w = 1; w < NHF_WAVES; ++w)
+            if (s_key[w][threadIdx.x] < key) {
+                key = s_key[w][threadIdx.x];
+                idx = s_idx[w][threadIdx.x];
+            }
+        d.cfl_key[(size_t)blockIdx.x * S + s] = key;
+        d.cfl_idx[(size_t)blockIdx.x * S + s] = idx;
+    }
+}
+
+// One workgroup per slot: reduce its CFL partials to ndt and side; in a batch (mode 1) also add
+// the boundary cells' budget terms, commit `step` and prepare `step + 1`.
+__global__ __launch_bounds__(NHF_BLOCK) void kb_finish(BDev d, int mode, int step, int steps) {
+    const int S = SLOTS;
+    const int s = blockIdx.x;
+    Ctl *c = &d.ctl[s];
+    if (mode == 1 && d.dt[s] == 0.0f) return;
+    __shared__ uint64_t s_key[NHF_WAVES];
+    __shared__ int32_t s_idx[NHF_WAVES];
+    uint64_t key = ~0ull;
+    int32_t idx = 0;
+    for (int b = threadIdx.x; b < d.cfl_blocks; b += blockDim.x) {
+        uint64_t k2 = d.cfl_key[(size_t)b * S + s];
+        if (k2 < key) {
+            key = k2;
+            idx = d.cfl_idx[(size_t)b * S + s];
+        }
+    }
+    wave_min_key(key, idx);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_key[wave] = key;
+        s_idx[wave] = idx;
+    }
+    if (mode == 1 && d.n_bcells > 0) {
+        // boundary-cell terms: one job per term, one wavefront per job, lanes over the cells
+        const int b_num = d.b_num;
+        const int jobs = 1 + 6 * b_num;
+        for (int job = wave; job < jobs; job += NHF_WAVES) {
+            int term, comp, lo_b, hi_b, want;
+            if (job == 0) {  // floor at level cells, every level boundary
+                term = 2; comp = 0; lo_b = 0; hi_b = b_num; want = 0;
+            } else if (job <= 5 * b_num) {
+                int tb = (job - 1) / 5;
+                term = 4 + (job - 1); comp = 1 + (job - 1) % 5; lo_b = tb; hi_b = tb + 1; want = 0;
+            } else {
+                int tb = job - 1 - 5 * b_num;
+                term = 4 + 5 * b_num + tb; comp = 0; lo_b = tb; hi_b = tb + 1; want = 1;
+            }
+            double sum = 0.0;
+            for (int bb = lo_b; bb < hi_b; ++bb) {
+                if (d.m.kind[bb] != want) continue;
+                for (int j = d.bcell_start[bb] + lane; j < d.bcell_start[bb + 1]; j += 64)
+                    sum += (double)d.bvals[IX(6 * j + comp)];
+            }
+            sum = wave_sum(sum);
+            if (lane == 0 && sum != 0.0) {
+                double2 a = d.bacc[(size_t)s * d.terms + term];
+                acc_add(a, sum);
+                d.bacc[(size_t)s * d.terms + term] = a;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NHF_WAVES; ++w)
+            if (s_key[w] < key) {
+                key = s_key[w];
+                idx = s_idx[w];
+            }
+        if (key == ~0ull) {
+            c->ndt = INFINITY;
+            c->cfl_arg = 0;
+        } else {
+            c->ndt = __uint_as_float((uint32_t)(key >> 32));
+            c->cfl_arg = idx;
+        }
+        if (mode == 1) {
+            commit_slot(c, step);
+            d.dt[s] = 0.0f;
+            if (step + 1 < steps) prepare_slot_b(d, s, step + 1);
+        }
+    }
+}
+
+// Interval budget of slot s: the depth blocks' (sum, compensation) pairs of terms 0 and 1 are
+// combined in double-double in a fixed order; the boundary terms are copied; all are cleared.
+__global__ __launch_bounds__(NHF_BLOCK) void kb_flush(BDev d, int s) {
+    const int S = SLOTS;
+    __shared__ double s_hi[NHF_BLOCK], s_lo[NHF_BLOCK];
+    for (int term = 0; term < 2; ++term) {
+        double hi = 0.0, lo = 0.0;
+        for (int b = threadIdx.x; b < d.depth_blocks; b += NHF_BLOCK) {
+            double2 *p = d.acc + ((size_t)b * 2 + term) * S + s;
+            const double2 a = *p;
+            *p = make_double2(0.0, 0.0);
+            const double parts[2] = {a.x, a.y};
+            for (double x : parts) {
+                double sum = hi + x;
+                double bp = sum - hi;
+                lo += (hi - (sum - bp)) + (x - bp);
+                hi = sum;
+            }
+        }
+        s_hi[threadIdx.x] = hi;
+        s_lo[threadIdx.x] = lo;
+        __syncthreads();
+        for (int o = NHF_BLOCK / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                double a = s_hi[threadIdx.x], b = s_hi[threadIdx.x + o];
+                double sum = a + b;
+                double bp = sum - a;
+                double err = (a - (sum - bp)) + (b - bp);
+                s_hi[threadIdx.x] = sum;
+                s_lo[threadIdx.x] += s_lo[threadIdx.x + o] + err;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) d.flush[term] = make_double2(s_hi[0], s_lo[0]);
+        __syncthreads();
+    }
+    for (int t = 2 + threadIdx.x; t < d.terms; t += NHF_BLOCK) {
+        d.flush[t] = d.bacc[(size_t)s * d.terms + t];
+        d.bacc[(size_t)s * d.terms + t] = make_double2(0.0, 0.0);
+    }
+}
+
+// --------------------------------------------------------------------------- host side
+
+enum { KB_PREPARE = 0, KB_GRAD, KB_FACE, KB_FLUX, KB_DONOR, KB_DEPTH, KB_CFL, KB_FINISH, KB_COUNT };
+
+struct nhb_ctx {
+    BDev d;
+    int slots;
+    hipStream_t stream;
+    int graph_enabled, timing;
+    hipGraphExec_t exec[NHF_MAX_BATCH + 1];
+    std::vector<hipEvent_t> events;
+    std::vector<int> event_kind;
+    double kernel_ms[KB_COUNT];
+    int64_t kernel_count[KB_COUNT];
+    // staging: host (pinned) and device copies of the same layout
+    char *stage_host, *stage_dev;
+    size_t stage_bytes;
+    Stage st_host, st_dev;
+    Ctl *host_ctl;
+    double2 *host_flush;
+};
+
+static size_t align_b(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static int b_blocks(size_t n) { return (int)((n + NHF_BLOCK - 1) / NHF_BLOCK); }
+
+static int b_blocks_xcd(size_t n) { return (b_blocks(n) + 7) & ~7; }
+
+// Grid of the grid-stride kernels: about 2048 workgroups (8 per CU), fewer on small meshes.
+static int b_stride_grid(size_t n) {
+    int g = b_blocks_xcd(n);
+    return g < 2048 ? g : 2048;
+}
+
+static int slot_shift(int slots) {
+    int sh = 0;
+    while ((1 << sh) < slots) ++sh;
+    return sh;
+}
+
+struct BCarve {
+    size_t ctl, dt, raining, coeff, values, level, rain, bdonor, qflux, face_qn, grad, key, idx, acc, bvals, bacc,
+        flush, total;
+};
+
+static BCarve bcarve(const nhf_mesh *m, const nhb_state *st, const nhf_params *p, int slots) {
+    BCarve c;
+    size_t off = 0;
+    const size_t S = slots;
+    const int b = m->b_num > 0 ? m->b_num : 1;
+    const int terms = 4 + 6 * m->b_num;
+    const int nb = st->n_bcells > 0 ? st->n_bcells : 1;
+    const int cfl_blocks = b_stride_grid(((size_t)(m->s_num > 1 ? m->s_num - 1 : 1)) * S);
+    const int depth_blocks = b_stride_grid(((size_t)(m->e_num - 1)) * S);
+    c.ctl = off; off = align_b(off + sizeof(Ctl) * S);
+    c.dt = off; off = align_b(off + sizeof(float) * S);
+    c.raining = off; off = align_b(off + sizeof(int32_t) * S);
+    c.coeff = off; off = align_b(off + sizeof(float) * 2 * b * S);
+    c.values = off; off = align_b(off + sizeof(float) * b * S);
+    c.level = off; off = align_b(off + sizeof(float) * b * S);
+    c.rain = off; off = align_b(off + sizeof(float) * (m->n_pixels > 0 ? m->n_pixels : 1) * S);
+    c.bdonor = off; off = align_b(off + sizeof(float) * nb * S);
+    c.qflux = off; off = align_b(off + sizeof(float) * (size_t)m->s_num * S);
+    c.face_qn = off; off = align_b(off + (p->theta == 1.0f ? 0 : sizeof(float) * 4 * (size_t)m->e_num * S));
+    c.grad = off; off = align_b(off + sizeof(float) * 2 * (size_t)(m->n_grad > 0 ? m->n_grad : 1) * S);
+    c.key = off; off = align_b(off + sizeof(uint64_t) * cfl_blocks * S);
+    c.idx = off; off = align_b(off + sizeof(int32_t) * cfl_blocks * S);
+    c.acc = off; off = align_b(off + sizeof(double2) * 2 * (size_t)depth_blocks * S);
+    c.bvals = off; off = align_b(off + sizeof(float) * 6 * (size_t)nb * S);
+    c.bacc = off; off = align_b(off + sizeof(double2) * terms * S);
+    c.flush = off; off = align_b(off + sizeof(double2) * terms);
+    c.total = off;
+    return c;
+}
+
+static size_t stage_layout(const BDev &d, int slots, Stage &st, char *base) {
+    size_t off = 0;
+    st.target = (uint64_t *)(base + off); off = align_b(off + sizeof(uint64_t) * slots);
+    st.origin = (uint64_t *)(base + off); off = align_b(off + sizeof(uint64_t) * slots);
+    st.flags = (int32_t *)(base + off); off = align_b(off + sizeof(int32_t) * slots);
+    st.coeff = (float *)(base + off); off = align_b(off + sizeof(float) * 2 * (d.b_num > 0 ? d.b_num : 1) * slots);
+    st.rain = (float *)(base + off); off = align_b(off + sizeof(float) * (d.n_pixels > 0 ? d.n_pixels : 1) * slots);
+    return off;
+}
+
+static hipEvent_t b_timing_event(nhb_ctx *ctx, size_t i) {
+    while (ctx->events.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->events.push_back(e);
+    }
+    return ctx->events[i];
+}
+
+static int32_t b_launch_batch(nhb_ctx *ctx, int steps, bool timed) {
+    const BDev &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    const size_t S = ctx->slots;
+    size_t ev = 0;
+    ctx->event_kind.clear();
+#define BLAUNCH(kind, ...)                                                  \
+    do {                                                                    \
+        if (timed) {                                                        \
+            hipEvent_t e0 = b_timing_event(ctx, ev++);                      \
+            if (!e0) return set_error_text("hipEventCreate failed");        \
+            HIP_TRY(hipEventRecord(e0, s));                                 \
+        }                                                                   \
+        hipLaunchKernelGGL(__VA_ARGS__);                                    \
+        if (timed) {                                                        \
+            hipEvent_t e1 = b_timing_event(ctx, ev++);                      \
+            if (!e1) return set_error_text("hipEventCreate failed");        \
+            HIP_TRY(hipEventRecord(e1, s));                                 \
+            ctx->event_kind.push_back(kind);                                \
+        }                                                                   \
+    } while (0)
+    const size_t cells = (size_t)(d.m.e_num - 1), sides = (size_t)(d.m.s_num - 1);
+    BLAUNCH(KB_PREPARE, kb_prepare, dim3((unsigned)S), dim3(64), 0, s, d, steps);
+    for (int step = 0; step < steps; ++step) {
+        if (d.m.n_grad > 0)
+            BLAUNCH(KB_GRAD, kb_gradients, dim3(b_blocks((size_t)d.m.n_grad * S)), dim3(NHF_BLOCK), 0, s, d);
+        if (!d.theta_one)
+            BLAUNCH(KB_FACE, kb_face_avg, dim3(b_blocks((size_t)d.m.e_num * S)), dim3(NHF_BLOCK), 0, s, d);
+        if (sides > 0) BLAUNCH(KB_FLUX, kb_flux, dim3(b_blocks_xcd(sides * S)), dim3(NHF_BLOCK), 0, s, d);
+        BLAUNCH(KB_DONOR, kb_donor, dim3(b_blocks_xcd(cells * S)), dim3(NHF_BLOCK), 0, s, d);
+        BLAUNCH(KB_DEPTH, kb_depth, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
+        BLAUNCH(KB_CFL, kb_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
+        BLAUNCH(KB_FINISH, kb_finish, dim3((unsigned)S), dim3(NHF_BLOCK), 0, s, d, 1, step, steps);
+    }
+#undef BLAUNCH
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" {
+
+size_t nhb_workspace_bytes(const nhf_mesh *mesh, const nhb_state *state, const nhf_params *params,
+                           int32_t slots) {
+    if (!mesh || !state || !params || slots < 1 || slots > NHB_MAX_SLOTS) return 0;
+    return bcarve(mesh, state, params, 1 << slot_shift(slots)).total;
+}
+
+int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, const nhf_params *params,
+                   int32_t slots, void *workspace, size_t workspace_bytes, void *stream) {
+    if (!out || !mesh || !state || !params || !workspace) return set_error_text("nhb_create: null argument");
+    if (slots < 1 || slots > NHB_MAX_SLOTS || (slots & (slots - 1)))
+        return set_error_text("nhb_create: slots must be a power of two in 1..64");
+    if (params->batch_capacity < 1 || params->batch_capacity > NHF_MAX_BATCH)
+        return set_error_text("nhb_create: batch_capacity out of range");
+    if (mesh->e_num < 2 || mesh->s_num < 1) return set_error_text("nhb_create: empty mesh");
+    if (mesh->pipe_q) return set_error_text("nhb_create: pipe exchange is not supported in a batch");
+    if (!state->depth || !state->rain_time || !state->capacity || !state->max_depth || !state->max_speed ||
+        !state->peak_tick || !state->q)
+        return set_error_text("nhb_create: state array missing");
+    if (mesh->b_num > 0 && (!state->bslot || !state->bcell_start))
+        return set_error_text("nhb_create: boundary-cell list missing");
+    if ((params->theta != 1.0f) && !mesh->open_face)
+        return set_error_text("nhb_create: open_face is required when sita != 1");
+    if (mesh->n_hang > 0 && (!mesh->edge_index || !mesh->x || !mesh->y || !mesh->grad_cell))
+        return set_error_text("nhb_create: hanging reconstruction arrays missing");
+    if (const char *name = missing_mesh_array(mesh, false, state->manning == nullptr)) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "nhb_create: mesh array %s is NULL", name);
+        return -2;
+    }
+    BCarve cv = bcarve(mesh, state, params, slots);
+    if (workspace_bytes < cv.total) return set_error_text("nhb_create: workspace too small");
+    nhb_ctx *ctx = new nhb_ctx();
+    char *ws = (char *)workspace;
+    BDev &d = ctx->d;
+    d.m = *mesh;
+    if (mesh->n_hang == 0) d.m.edge_index = nullptr;
+    d.p = *params;
+    d.shift = slot_shift(slots);
+    d.terms = 4 + 6 * mesh->b_num;
+    d.cfl_blocks = b_stride_grid(((size_t)(mesh->s_num > 1 ? mesh->s_num - 1 : 1)) * slots);
+    d.depth_blocks = b_stride_grid(((size_t)(mesh->e_num - 1)) * slots);
+    d.n_bcells = state->n_bcells;
+    d.theta_one = params->theta == 1.0f;
+    d.b_num = mesh->b_num;
+    d.n_pixels = mesh->n_pixels > 0 ? mesh->n_pixels : 1;
+    d.depth = state->depth;
+    d.rain_time = state->rain_time;
+    d.capacity = state->capacity;
+    d.max_depth = state->max_depth;
+    d.max_speed = state->max_speed;
+    d.peak_tick = state->peak_tick;
+    d.q = state->q;
+    d.manning = state->manning;
+    d.bslot = state->bslot;
+    d.bcell_start = state->bcell_start;
+    d.ctl = (Ctl *)(ws + cv.ctl);
+    d.dt = (float *)(ws + cv.dt);
+    d.raining = (int32_t *)(ws + cv.raining);
+    d.coefficients = (float *)(ws + cv.coeff);
+    d.values = (float *)(ws + cv.values);
+    d.level_values = (float *)(ws + cv.level);
+    d.rain_rates = (float *)(ws + cv.rain);
+    d.bdonor = (float *)(ws + cv.bdonor);
+    d.qflux = (float *)(ws + cv.qflux);
+    d.face_qn = d.theta_one ? nullptr : (float *)(ws + cv.face_qn);
+    d.grad = (float *)(ws + cv.grad);
+    d.cfl_key = (uint64_t *)(ws + cv.key);
+    d.cfl_idx = (int32_t *)(ws + cv.idx);
+    d.acc = (double2 *)(ws + cv.acc);
+    d.bvals = (float *)(ws + cv.bvals);
+    d.bacc = (double2 *)(ws + cv.bacc);
+    d.flush = (double2 *)(ws + cv.flush);
+    ctx->slots = slots;
+    ctx->stream = (hipStream_t)stream;
+    ctx->graph_enabled = 0;
+    ctx->timing = 0;
+    for (int i = 0; i <= NHF_MAX_BATCH; ++i) ctx->exec[i] = nullptr;
+    memset(ctx->kernel_ms, 0, sizeof ctx->kernel_ms);
+    memset(ctx->kernel_count, 0, sizeof ctx->kernel_count);
+    Stage probe;
+    ctx->stage_bytes = stage_layout(d, slots, probe, nullptr);
+    ctx->stage_host = ctx->stage_dev = nullptr;
+    ctx->host_ctl = nullptr;
+    ctx->host_flush = nullptr;
+    hipError_t err;
+    if ((err = hipHostMalloc((void **)&ctx->stage_host, ctx->stage_bytes, hipHostMallocDefault)) != hipSuccess ||
+        (err = hipMalloc((void **)&ctx->stage_dev, ctx->stage_bytes)) != hipSuccess ||
+        (err = hipHostMalloc((void **)&ctx->host_ctl, sizeof(Ctl) * slots, hipHostMallocDefault)) != hipSuccess ||
+        (err = hipHostMalloc((void **)&ctx->host_flush, sizeof(double2) * d.terms, hipHostMallocDefault)) !=
+            hipSuccess) {
+        if (ctx->stage_host) hipHostFree(ctx->stage_host);
+        if (ctx->stage_dev) hipFree(ctx->stage_dev);
+        if (ctx->host_ctl) hipHostFree(ctx->host_ctl);
+        delete ctx;
+        return set_error("nhb_create: staging allocation", err);
+    }
+    memset(ctx->stage_host, 0, ctx->stage_bytes);
+    stage_layout(d, slots, ctx->st_host, ctx->stage_host);
+    stage_layout(d, slots, ctx->st_dev, ctx->stage_dev);
+    // every slot starts idle with a clean clock
+    for (int s = 0; s < slots; ++s)
+        hipLaunchKernelGGL(kb_reset, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, s, (uint64_t)0);
+    if ((err = hipGetLastError()) != hipSuccess) {
+        hipHostFree(ctx->stage_host);
+        hipFree(ctx->stage_dev);
+        hipHostFree(ctx->host_ctl);
+        hipHostFree(ctx->host_flush);
+        delete ctx;
+        return set_error("nhb_create: reset", err);
+    }
+    *out = ctx;
+    return 0;
+}
+
+int32_t nhb_destroy(nhb_ctx *ctx) {
+    if (!ctx) return 0;
+    hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i <= NHF_MAX_BATCH; ++i)
+        if (ctx->exec[i]) hipGraphExecDestroy(ctx->exec[i]);
+    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+    hipHostFree(ctx->stage_host);
+    hipFree(ctx->stage_dev);
+    hipHostFree(ctx->host_ctl);
+    hipHostFree(ctx->host_flush);
+    delete ctx;
+    return 0;
+}
+
+static int32_t check_slot(const nhb_ctx *ctx, int32_t slot) {
+    if (slot < 0 || slot >= ctx->slots) return set_error_text("nhb: slot out of range");
+    return 0;
+}
+
+int32_t nhb_reset(nhb_ctx *ctx, int32_t slot, uint64_t now_ticks) {
+    if (check_slot(ctx, slot)) return -2;
+    ctx->st_host.flags[slot] = 0;
+    hipLaunchKernelGGL(kb_reset, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, (int)slot, now_ticks);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhb_set_forcing(nhb_ctx *ctx, int32_t slot, const float *coefficients, const float *rain_rates,
+                        int32_t raining) {
+    if (check_slot(ctx, slot)) return -2;
+    const BDev &d = ctx->d;
+    Stage &st = ctx->st_host;
+    int32_t flags = st.flags[slot] & ~(STAGE_RAINING | STAGE_RAINING_ON);
+    if (d.b_num > 0 && coefficients) {
+        memcpy(st.coeff + (size_t)slot * 2 * d.b_num, coefficients, sizeof(float) * 2 * d.b_num);
+        flags |= STAGE_COEFF;
+    }
+    if (rain_rates) {
+        memcpy(st.rain + (size_t)slot * d.n_pixels, rain_rates, sizeof(float) * d.n_pixels);
+        flags |= STAGE_RAIN;
+    }
+    flags |= STAGE_RAINING | (raining ? STAGE_RAINING_ON : 0);
+    st.flags[slot] = flags;
+    return 0;
+}
+
+int32_t nhb_begin(nhb_ctx *ctx, int32_t slot, uint64_t target_ticks, uint64_t origin_ticks) {
+    if (check_slot(ctx, slot)) return -2;
+    ctx->st_host.target[slot] = target_ticks;
+    ctx->st_host.origin[slot] = origin_ticks;
+    ctx->st_host.flags[slot] |= STAGE_BEGIN;
+    return 0;
+}
+
+// Upload and apply the staged words of every slot; the host copy is reused after the next sync.
+static int32_t apply_stage(nhb_ctx *ctx) {
+    bool any = false;
+    for (int s = 0; s < ctx->slots; ++s) any |= ctx->st_host.flags[s] != 0;
+    if (!any) return 0;
+    HIP_TRY(hipMemcpyAsync(ctx->stage_dev, ctx->stage_host, ctx->stage_bytes, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(kb_apply, dim3(ctx->slots), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, ctx->st_dev);
+    HIP_TRY(hipGetLastError());
+    // the copy reads the pinned staging asynchronously: wait before the flags are reused
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < ctx->slots; ++s) ctx->st_host.flags[s] = 0;
+    return 0;
+}
+
+int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace) {
+    if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhb_run_batch: steps out of range");
+    int32_t rc = apply_stage(ctx);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    if (ctx->graph_enabled && !ctx->timing && s != nullptr) {
+        if (!ctx->exec[steps]) {
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            rc = b_launch_batch(ctx, steps, false);
+            hipError_t end = hipStreamEndCapture(s, &graph);
+            if (rc) return rc;
+            if (end != hipSuccess) return set_error("hipStreamEndCapture", end);
+            hipError_t inst = hipGraphInstantiate(&ctx->exec[steps], graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (inst != hipSuccess) return set_error("hipGraphInstantiate", inst);
+        }
+        HIP_TRY(hipGraphLaunch(ctx->exec[steps], s));
+    } else {
+        rc = b_launch_batch(ctx, steps, ctx->timing != 0);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctl, ctx->d.ctl, sizeof(Ctl) * ctx->slots, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (ctx->timing) {
+        for (size_t i = 0; i < ctx->event_kind.size(); ++i) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->events[2 * i], ctx->events[2 * i + 1]));
+            ctx->kernel_ms[ctx->event_kind[i]] += ms;
+            ctx->kernel_count[ctx->event_kind[i]] += 1;
+        }
+    }
+    if (trace)
+        for (int sl = 0; sl < ctx->slots; ++sl)
+            memcpy(trace + (size_t)sl * steps * NHF_TRACE_COLS, ctx->host_ctl[sl].trace,
+                   sizeof(uint32_t) * NHF_TRACE_COLS * steps);
+    return 0;
+}
+
+int32_t nhb_compute_cfl(nhb_ctx *ctx) {
+    const BDev &d = ctx->d;
+    int32_t rc = apply_stage(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kb_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+    hipLaunchKernelGGL(kb_finish, dim3(ctx->slots), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0, 0);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhb_flush_budget(nhb_ctx *ctx, int32_t slot, double *out) {
+    if (check_slot(ctx, slot)) return -2;
+    const BDev &d = ctx->d;
+    hipLaunchKernelGGL(kb_flush, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, d, (int)slot);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx->host_flush, d.flush, sizeof(double2) * d.terms, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int nonfinite = 0;
+    for (int t = 0; t < d.terms; ++t) {
+        double hi = t == 3 ? 0.0 : ctx->host_flush[t].x, lo = t == 3 ? 0.0 : ctx->host_flush[t].y;
+        if (!isfinite(hi) || !isfinite(lo)) nonfinite = 1;
+        out[2 * t] = hi;
+        out[2 * t + 1] = lo;
+    }
+    return nonfinite;
+}
+
+int32_t nhb_limit_capture(nhb_ctx *ctx, int32_t slot, float *values, int32_t *ids, int32_t reset) {
+    if (check_slot(ctx, slot)) return -2;
+    Ctl *ctl = ctx->d.ctl + slot;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (values) HIP_TRY(hipMemcpy(values, ctl->limit_value, sizeof(float) * 5, hipMemcpyDeviceToHost));
+    if (ids) HIP_TRY(hipMemcpy(ids, ctl->limit_ids, sizeof(int32_t) * 3, hipMemcpyDeviceToHost));
+    if (reset) {
+        hipLaunchKernelGGL(kb_reset_limit, dim3(1), dim3(64), 0, ctx->stream, ctx->d, (int)slot);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+int32_t nhb_cfl_state(nhb_ctx *ctx, int32_t slot, float *ndt, int32_t *side) {
+    if (check_slot(ctx, slot)) return -2;
+    Ctl *ctl = ctx->d.ctl + slot;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ndt) HIP_TRY(hipMemcpy(ndt, &ctl->ndt, sizeof(float), hipMemcpyDeviceToHost));
+    if (side) HIP_TRY(hipMemcpy(side, &ctl->cfl_arg, sizeof(int32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int32_t nhb_sync(nhb_ctx *ctx) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int32_t nhb_set_graph(nhb_ctx *ctx, int32_t enable) {
+    ctx->graph_enabled = enable != 0;
+    return 0;
+}
+
+int32_t nhb_set_timing(nhb_ctx *ctx, int32_t enable) {
+    ctx->timing = enable != 0;
+    return 0;
+}
+
+int32_t nhb_kernel_times(nhb_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset) {
+    for (int i = 0; i < n && i < KB_COUNT; ++i) {
+        ms[i] = ctx->kernel_ms[i];
+        count[i] = ctx->kernel_count[i];
+    }
+    if (reset) {
+        memset(ctx->kernel_ms, 0, sizeof ctx->kernel_ms);
+        memset(ctx->kernel_count, 0, sizeof ctx->kernel_count);
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -37,6 +37,32 @@ static inline int32_t set_error_text(const char *what) {
         if (e_ != hipSuccess) return set_error(#expr, e_); \
     } while (0)
 
+// Name of the first required mesh array that is NULL, or NULL when all are present. `state`
+// also checks the per-cell and per-side state arrays (the single-mesh context reads them from
+// the mesh; a batch has its own).
+static inline const char *missing_mesh_array(const nhf_mesh *m, bool state, bool manning) {
+    struct Field {
+        const void *p;
+        const char *name;
+    } fields[] = {{m->bed, "bed"}, {m->area, "area"}, {m->size, "size"}, {m->cell_meta, "cell_meta"},
+                  {m->class_horton, "class_horton"}, {m->bweight, "bweight"}, {m->csr_start, "csr_start"},
+                  {m->csr_count, "csr_count"}, {m->isl, "isl"}, {m->isl_cell, "isl_cell"},
+                  {m->side_cells, "side_cells"}, {m->side_meta, "side_meta"}, {m->length, "length"},
+                  {m->sdc, "sdc"}, {m->side_key, "side_key"}, {m->kind, "kind"}};
+    for (const Field &f : fields)
+        if (!f.p) return f.name;
+    if (m->n_pixels > 1 && !m->rain_index) return "rain_index";
+    if (manning && !m->manning) return "manning";
+    if (state) {
+        const Field st[] = {{m->depth, "depth"}, {m->rain_time, "rain_time"}, {m->capacity, "capacity"},
+                            {m->max_depth, "max_depth"}, {m->max_speed, "max_speed"}, {m->peak_tick, "peak_tick"},
+                            {m->q, "q"}};
+        for (const Field &f : st)
+            if (!f.p) return f.name;
+    }
+    return nullptr;
+}
+
 // --------------------------------------------------------------------------- control block
 
 // One scenario's clock, step scalars, limiter counters, CFL-limit capture and batch trace.

@@ -658,6 +658,10 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
         return set_error_text("nhf_create: open_face is required when sita != 1");
     if (mesh->n_hang > 0 && (!mesh->edge_index || !mesh->x || !mesh->y || !mesh->grad_cell))
         return set_error_text("nhf_create: hanging reconstruction arrays missing");
+    if (const char *name = missing_mesh_array(mesh, true, true)) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "nhf_create: mesh array %s is NULL", name);
+        return -2;
+    }
     nhf_ctx *ctx = new nhf_ctx();
     char *ws = (char *)workspace;
     Dev &d = ctx->d;

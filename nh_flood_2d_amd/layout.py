@@ -18,6 +18,7 @@ in the same order as ``solver_compact.py`` and results do not depend on the renu
 """
 from __future__ import annotations
 
+import copy
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -96,6 +97,40 @@ class Layout:
         return np.ascontiguousarray(a[self.side_inv])
 
 
+def horton_table(land: np.ndarray, f0: np.ndarray, fc: np.ndarray, hk: np.ndarray) -> np.ndarray:
+    """Horton parameters are per land class (map_land_parameters assigns class constants), so the
+    cells carry their class and the kernels read a 256-entry table; cell 0 keeps zeros."""
+    horton = np.zeros((256, 3), dtype=np.float32)
+    for cls in np.unique(land[1:]):
+        first = 1 + int(np.flatnonzero(land[1:] == cls)[0])
+        horton[cls] = (f0[first], fc[first], hk[first])
+    if np.any(horton[land[1:], 0] != f0[1:]) or np.any(horton[land[1:], 2] != hk[1:]):
+        raise ValueError('Horton parameters must be constant per land class')
+    return horton
+
+
+def relayout(base: Layout, mesh: Mesh, land_parameters: dict, forcing: Forcing) -> Layout:
+    """The layout of another scenario on the mesh and forcing geometry of `base`: only the arrays
+    that depend on the land table (side Manning n, the Horton class table) are recomputed."""
+    if (mesh.e_num, mesh.s_num) != (base.e_num, base.s_num):
+        raise ValueError('relayout needs the mesh of the base layout')
+    bid = forcing.boundary_id.astype(np.int64)
+    same = (np.array_equal(((base.arrays['cell_meta'].astype(np.int64) >> 8) - 1)[1:], bid[base.cell_perm][1:])
+            and np.array_equal(base.arrays['bweight'], forcing.bweight.astype(np.float32)[base.cell_perm])
+            and np.array_equal(base.arrays['rain_index'], forcing.rain_pixel.astype(np.int32)[base.cell_perm])
+            and base.n_pixels == forcing.n_pixels and base.b_num == len(forcing.kinds))
+    if not same:
+        raise ValueError('relayout needs the forcing geometry of the base layout')
+    side_n, f0, fc, hk = map_land_parameters(land_parameters, mesh.land, mesh.topo)
+    arrays = dict(base.arrays)
+    arrays['manning'] = side_n[base.side_perm]
+    arrays['class_horton'] = horton_table(mesh.land.astype(np.int64), f0, fc, hk).ravel()
+    layout = copy.copy(base)
+    layout.arrays = arrays
+    layout.cell_f0 = f0.astype(np.float32)
+    return layout
+
+
 def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: float,
                  order: str = 'morton') -> Layout:
     """Derive every static kernel array from the mesh, land table and forcing geometry."""
@@ -148,15 +183,8 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
     side_cells[:, 1] = cinv[topo[sperm, 2]]
     meta = topo[sperm, 0].astype(np.uint8) | (mesh.sbf[sperm] * 4).astype(np.uint8)
 
-    # Horton parameters are per land class (map_land_parameters assigns class constants), so the
-    # cells carry their class and the kernels read a 256-entry table; cell 0 keeps zeros.
     land = mesh.land.astype(np.int64)
-    horton = np.zeros((256, 3), dtype=np.float32)
-    for cls in np.unique(land[1:]):
-        first = 1 + int(np.flatnonzero(land[1:] == cls)[0])
-        horton[cls] = (f0[first], fc[first], hk[first])
-    if np.any(horton[land[1:], 0] != f0[1:]) or np.any(horton[land[1:], 2] != hk[1:]):
-        raise ValueError('Horton parameters must be constant per land class')
+    horton = horton_table(land, f0, fc, hk)
     bid = forcing.boundary_id.astype(np.int64)
     if np.any(bid > 253):
         raise ValueError('at most 254 prescribed boundaries are supported')

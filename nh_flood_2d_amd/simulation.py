@@ -21,7 +21,7 @@ from .clock import TICKS_PER_SECOND, f32_from_bits, row_end_ticks, seconds_to_ti
 from .engine import Engine, make_params
 from .forcing import Forcing, cell_forcing_arrays
 from .hydraulics import connected_still_water
-from .layout import build_layout, initial_state
+from .layout import build_layout, initial_state, relayout
 from .mesh import Mesh
 
 MIN_DT = 1e-4
@@ -159,7 +159,8 @@ class Simulation:
     """One scenario advanced by the device step with the reference's host protocol."""
 
     def __init__(self, mesh: Mesh, land_parameters: dict, forcing: Forcing, settings: RunSettings,
-                 checkpoint=None, engine_factory=None, outputs=None, record: dict | None = None):
+                 checkpoint=None, engine_factory=None, outputs=None, record: dict | None = None,
+                 initial_depth: np.ndarray | None = None, base_layout=None):
         self.mesh = mesh
         self.forcing = forcing
         self.settings = s = settings
@@ -186,7 +187,11 @@ class Simulation:
         self.host_now_ticks = seconds_to_ticks(initial_clock)
 
         cell_forcing = cell_forcing_arrays(mesh, forcing)
-        self.layout = build_layout(mesh, land_parameters, cell_forcing, s.sita, s.order)
+        if base_layout is not None:
+            # another scenario on the same mesh: reuse its ordering and lists
+            self.layout = relayout(base_layout, mesh, land_parameters, cell_forcing)
+        else:
+            self.layout = build_layout(mesh, land_parameters, cell_forcing, s.sita, s.order)
         self.params = make_params(s.afa, s.sita, s.min_h, s.batch_size)
         _, _, _, area = mesh.geometry()
         self.area = area
@@ -205,6 +210,13 @@ class Simulation:
                                                 mesh.sbf, mesh.topo, self.boundaries, area)
             state['depth'] = depth
             self.record['initialization'] = dict(method='connected_still_water', **info)
+        elif initial_depth is not None:
+            depth = np.asarray(initial_depth, dtype=np.float32).copy()
+            if depth.shape != (mesh.e_num,) or not np.all(depth >= 0):
+                raise ValueError('initial depth must be a non-negative array over the cells')
+            depth[0] = 0.
+            state['depth'] = depth
+            self.record['initialization'] = dict(method='given_depth')
         else:
             self.record['initialization'] = dict(method='dry')
         self.initial_bed = state['bed'].copy()
@@ -365,69 +377,94 @@ class Simulation:
         return ndt
 
     # ------------------------------------------------------------------ main loop
+    # The host loop of solver_compact.py:930-985, split at the device call so that a batch runner
+    # can advance many scenarios with one launch sequence: start(); while plan(): consume(rows);
+    # finish(). run() is that loop for a scenario on its own.
     def run(self):
-        s = self.settings
-        end_time = self.end_time
-        ob = self.origin_before_start
-        elapsed = self.elapsed
+        self.start()
+        while self.plan():
+            t0 = perf_counter()
+            rows = self.engine.run_batch(self.settings.batch_size)
+            self.device_seconds += perf_counter() - t0
+            self.consume(rows)
+        return self.finish()
+
+    def start(self) -> None:
+        """Write the initial output row and frame."""
+        elapsed, end_time = self.elapsed, self.end_time
         self.write_output(elapsed, 'warmup' if elapsed < 0 or end_time == 0 else 'simulation',
                           frame=True, final=elapsed == end_time)
-        while elapsed < end_time:
-            warming = elapsed < 0.
-            target = min(self.next_output, self.next_frame, 0. if warming else end_time)
-            if not warming:
-                now_tick = seconds_to_ticks(self.host_now_seconds)
-                i = np.searchsorted(self.event_ticks, np.uint64(now_tick), side='right')
-                if i < len(self.event_ticks):
-                    target = min(target, int(self.event_ticks[i]) / TICKS_PER_SECOND - ob)
-            self.upload_forcing(elapsed, warming)
-            target_ticks = seconds_to_ticks(target + ob)
-            if target_ticks < self.host_now_ticks:
-                raise ValueError('Target precedes the current clock')
-            self.engine.begin(target_ticks, self.host_now_ticks)
-            t0 = perf_counter()
-            rows = self.engine.run_batch(s.batch_size)
-            self.device_seconds += perf_counter() - t0
-            self.batches += 1
-            completed = []
-            for row in rows:
-                status = int(row[3])
-                if status == abi.DONE:
-                    break
-                if status != abi.COMPLETED:
-                    if status == abi.BAD_CFL:
-                        self.supported_dt()
-                    raise RuntimeError(f'Device clock status {status} at {elapsed}s')
-                completed.append(row[0])
-                self.host_now_ticks = row_end_ticks(row)
-                scale = f32_from_bits(row[8])
-                for activity in (self.interval_activity, self.total_activity):
-                    activity[0] += int(row[6])
-                    activity[1] += int(row[7])
-                    activity[2] = min(activity[2], scale)
-                dt = f32_from_bits(row[0])
-                for bi, b in enumerate(self.boundaries):
-                    self.q_exact[bi] = b.series.at(0. if warming else elapsed + dt / 2.)
-                self.budget.add_forcing(dt, 0. if warming else self.rain_volume_rate, self.q_exact)
-                elapsed = self.host_now_ticks / TICKS_PER_SECOND - ob
-                self.steps += 1
-                self.warmup_steps += int(warming)
-                self.dt_stats = [min(self.dt_stats[0], dt), max(self.dt_stats[1], dt), self.dt_stats[2] + dt]
-                np.maximum(self.discharge_peaks, self.q_exact, out=self.discharge_peaks)
-                (self.interval_event_dts if row[5] else self.interval_dts).append(dt)
-            self.dt_hasher.update(np.array(completed, dtype=np.uint32).tobytes())
-            self.elapsed = elapsed
-            is_output = abs(elapsed - self.next_output) < 1e-9 or elapsed == end_time
-            is_frame = abs(elapsed - self.next_frame) < 1e-9 or elapsed == end_time
-            if is_output or is_frame:
-                if warming and elapsed == 0. and end_time > 0.:
-                    self.reset_extrema()
-                self.write_output(elapsed, 'warmup' if elapsed < 0 or end_time == 0 else 'simulation',
-                                  frame=is_frame, final=elapsed == end_time)
-                if is_output:
-                    self.next_output = min(self.next_output + s.yield_step, 0. if elapsed < 0 else end_time)
-                if is_frame:
-                    self.next_frame = min(self.next_frame + FRAME_INTERVAL_S, 0. if elapsed < 0 else end_time)
+
+    def plan(self) -> bool:
+        """Choose the next target, upload forcing and begin a batch; False when the run is done."""
+        s = self.settings
+        elapsed, end_time, ob = self.elapsed, self.end_time, self.origin_before_start
+        if not elapsed < end_time:
+            return False
+        warming = elapsed < 0.
+        target = min(self.next_output, self.next_frame, 0. if warming else end_time)
+        if not warming:
+            now_tick = seconds_to_ticks(self.host_now_seconds)
+            i = np.searchsorted(self.event_ticks, np.uint64(now_tick), side='right')
+            if i < len(self.event_ticks):
+                target = min(target, int(self.event_ticks[i]) / TICKS_PER_SECOND - ob)
+        self.upload_forcing(elapsed, warming)
+        target_ticks = seconds_to_ticks(target + ob)
+        if target_ticks < self.host_now_ticks:
+            raise ValueError('Target precedes the current clock')
+        self.engine.begin(target_ticks, self.host_now_ticks)
+        self._warming = warming
+        return True
+
+    def consume(self, rows: np.ndarray) -> None:
+        """Read one batch's trace rows (clock, forcing integration, statistics) and write outputs."""
+        s = self.settings
+        end_time, ob = self.end_time, self.origin_before_start
+        warming = self._warming
+        elapsed = self.elapsed
+        self.batches += 1
+        completed = []
+        for row in rows:
+            status = int(row[3])
+            if status == abi.DONE:
+                break
+            if status != abi.COMPLETED:
+                if status == abi.BAD_CFL:
+                    self.supported_dt()
+                raise RuntimeError(f'Device clock status {status} at {elapsed}s')
+            completed.append(row[0])
+            self.host_now_ticks = row_end_ticks(row)
+            scale = f32_from_bits(row[8])
+            for activity in (self.interval_activity, self.total_activity):
+                activity[0] += int(row[6])
+                activity[1] += int(row[7])
+                activity[2] = min(activity[2], scale)
+            dt = f32_from_bits(row[0])
+            for bi, b in enumerate(self.boundaries):
+                self.q_exact[bi] = b.series.at(0. if warming else elapsed + dt / 2.)
+            self.budget.add_forcing(dt, 0. if warming else self.rain_volume_rate, self.q_exact)
+            elapsed = self.host_now_ticks / TICKS_PER_SECOND - ob
+            self.steps += 1
+            self.warmup_steps += int(warming)
+            self.dt_stats = [min(self.dt_stats[0], dt), max(self.dt_stats[1], dt), self.dt_stats[2] + dt]
+            np.maximum(self.discharge_peaks, self.q_exact, out=self.discharge_peaks)
+            (self.interval_event_dts if row[5] else self.interval_dts).append(dt)
+        self.dt_hasher.update(np.array(completed, dtype=np.uint32).tobytes())
+        self.elapsed = elapsed
+        is_output = abs(elapsed - self.next_output) < 1e-9 or elapsed == end_time
+        is_frame = abs(elapsed - self.next_frame) < 1e-9 or elapsed == end_time
+        if is_output or is_frame:
+            if warming and elapsed == 0. and end_time > 0.:
+                self.reset_extrema()
+            self.write_output(elapsed, 'warmup' if elapsed < 0 or end_time == 0 else 'simulation',
+                              frame=is_frame, final=elapsed == end_time)
+            if is_output:
+                self.next_output = min(self.next_output + s.yield_step, 0. if elapsed < 0 else end_time)
+            if is_frame:
+                self.next_frame = min(self.next_frame + FRAME_INTERVAL_S, 0. if elapsed < 0 else end_time)
+
+    def finish(self):
+        """Record the run's totals and close the outputs; returns the last mass-balance row."""
         self.record.update(
             steps=self.steps, warmup_steps=self.warmup_steps, mass_balance_final=self.budget.last_row,
             dt_seconds={'min': self.dt_stats[0] if self.steps else 0., 'max': self.dt_stats[1],

@@ -454,3 +454,69 @@ def small_multires(seed: int = 1, blocks: int = 24, block_size: float = 8., band
         depth[1:] = np.where(mesh.z[1:] < 1.0, pond, 0.).astype(np.float32)
     return Scenario(f'small_multires_{mesh.e_num - 1}', mesh, land, starts, ends, mm[:, None],
                     np.zeros(mesh.e_num, dtype=np.int32), boundaries, duration, 0.5, theta, 0.02, depth)
+
+
+def two_level_fractal(seed: int = 0, blocks: int = 790, block_size: float = 10., refine: float = 0.2,
+                      duration: float = 7200.) -> Scenario:
+    """One shared mesh for scenario batches: a 2-level (block_size / block_size/2) quadtree over a
+    seeded diamond-square DEM (amplitude 20 m), with the lowest `refine` fraction of the blocks
+    refined; about blocks^2 x (1 + 3 refine) cells (one million at the defaults). One land class
+    with Manning n 0.03 and Horton (60, 10, 2) mm/h, 1/h; closed boundaries."""
+    rng = np.random.default_rng(seed)
+    extent = blocks * block_size
+    power = int(np.ceil(np.log2(blocks))) + 1
+    fractal = diamond_square(min(power, 11), 0.6, rng)
+    bc = (np.arange(blocks) + 0.5) * block_size
+    BX, BY = np.meshgrid(bc, bc)
+    zb = sample_grid(fractal, extent, BX.ravel(), BY.ravel()).reshape(BX.shape)
+    levels = np.where(zb <= np.quantile(zb, refine), 1, 0).astype(np.int64)
+    levels = balance_levels(levels)
+
+    def bed_fn(x, y):
+        return (20.0 * sample_grid(fractal, extent, x, y) + 0.0005 * x).astype(np.float32)
+
+    def land_fn(x, y):
+        return np.ones(len(x), dtype=np.uint8)
+
+    mesh = quadtree_mesh(levels, block_size, bed_fn, land_fn)
+    land = {1: dict(manning_n=0.03, infiltration=dict(f0_mm_h=60, fc_mm_h=10, k_per_h=2))}
+    starts, ends, mm = triangular_hyetograph(80., duration / 3, duration, 300.)
+    return Scenario(f'two_level_{mesh.e_num - 1}_cells', mesh, land, starts, ends, mm[:, None],
+                    np.zeros(mesh.e_num, dtype=np.int32), [], duration, 0.5, 1.0, 0.02)
+
+
+def scenario_variants(base: Scenario, count: int, seed: int = 0, peak_range=(20., 150.),
+                      manning_sigma: float = 0.2, pond: float = 0.3) -> list[Scenario]:
+    """`count` scenarios on the mesh of `base`: rain peak uniform in `peak_range` (mm/h, same
+    triangular shape), every land class's Manning n times one lognormal(0, manning_sigma) factor,
+    and a dry start or a `pond` m pond on the lowest quarter of the cells, each drawn per scenario."""
+    rng = np.random.default_rng(seed)
+    out = []
+    low = base.mesh.z <= np.quantile(base.mesh.z[1:], 0.25)
+    for i in range(count):
+        peak = float(rng.uniform(*peak_range))
+        factor = float(np.exp(rng.normal(0., manning_sigma)))
+        wet = bool(rng.integers(0, 2))
+        starts, ends, mm = triangular_hyetograph(peak, base.duration / 3, base.duration,
+                                                 float(base.rain_ends[0] - base.rain_starts[0]))
+        land = {k: dict(v, manning_n=float(np.float32(v['manning_n'] * factor)))
+                for k, v in base.land_parameters.items()}
+        depth = None
+        if wet:
+            depth = np.where(low, np.float32(pond), np.float32(0.)).astype(np.float32)
+            depth[0] = 0.
+        out.append(Scenario(f'{base.name}_v{i}', base.mesh, land, starts, ends, mm[:, None], base.rain_pixel,
+                            list(base.boundaries), base.duration, base.afa, base.sita, base.min_h, depth))
+    return out
+
+
+def simulation(sc: Scenario, engine_factory=None, end_time: float | None = None, yield_step: float = 300.,
+               batch_size: int = 32, base_layout=None, outputs=None):
+    """A Simulation of a synthetic scenario (no files); `base_layout` reuses another scenario's
+    layout of the same mesh."""
+    from .simulation import RunSettings, Simulation
+    settings = RunSettings(afa=sc.afa, sita=sc.sita, min_h=sc.min_h,
+                           end_time=sc.duration if end_time is None else end_time,
+                           yield_step=yield_step, batch_size=batch_size)
+    return Simulation(sc.mesh, sc.land_parameters, scenario_forcing(sc), settings, engine_factory=engine_factory,
+                      initial_depth=sc.initial_depth, base_layout=base_layout, outputs=outputs)

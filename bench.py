@@ -292,6 +292,66 @@ def batched_bench(args, device) -> dict:
     }
 
 
+def served_bench(args) -> dict:
+    """Serving load: Poisson arrivals (rate --serve-rate per second) of --serve-requests solves, each a
+    seeded two-level mesh with log-uniform cell count in [--serve-min-cells, --serve-max-cells],
+    simulated duration uniform in [--serve-min-h, --serve-max-h] hours, a triangular hyetograph
+    with peak U[20, 150] mm/h and, for half of them, a sinusoidal tide boundary; outputs every
+    300 s leave through the frame stream. Run once served (continuous batching, per-solve streams)
+    and once one solve at a time on the plain path."""
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout
+    from nh_flood_2d_amd.serve import FloodServer, FrameStream, Request, footprint_bytes
+    rng = np.random.default_rng(args.seed + 11)
+    t0 = time.time()
+    specs = []
+    arrival = 0.
+    for i in range(args.serve_requests):
+        cells = float(np.exp(rng.uniform(np.log(args.serve_min_cells), np.log(args.serve_max_cells))))
+        blocks = max(8, int(round(np.sqrt(cells / 1.6))))
+        duration = float(np.round(rng.uniform(args.serve_min_h, args.serve_max_h) * 3600. / 300.) * 300.)
+        base = synth.two_level_fractal(args.seed + 100 + i, blocks=blocks, duration=duration,
+                                       tide=1.0 if rng.uniform() < 0.5 else 0.)
+        sc = synth.scenario_variants(base, 1, seed=args.seed + 200 + i)[0]
+        layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, synth.scenario_forcing(sc)),
+                              sc.sita)
+        arrival += float(rng.exponential(1. / args.serve_rate)) if i else 0.
+        specs.append((i, sc, layout, arrival))
+    log(f'serving workload: {len(specs)} solves, cells {[s[1].mesh.e_num - 1 for s in specs]}, '
+        f'hours {[round(s[1].duration / 3600, 2) for s in specs]} ({time.time() - t0:.1f} s)')
+
+    def requests(stream):
+        out = []
+        for i, sc, layout, arr in specs:
+            def build(factory, sc=sc, layout=layout, i=i):
+                return synth.simulation(sc, engine_factory=factory, yield_step=300., base_layout=layout,
+                                        outputs=stream.sink(i))
+            out.append(Request(i, build, arr, sc.mesh.e_num - 1, footprint_bytes(layout)))
+        return out
+
+    stream = FrameStream(None, every_output=True)
+    served = FloodServer(batch_size=32).serve(requests(stream), realtime=True)
+    stream.close()
+    s = served.summary()
+    log(f"served: {s['requests']} solves, p50 {s['p50_latency_s']:.2f} s, p99 {s['p99_latency_s']:.2f} s, "
+        f"{s['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s, peak live {s['peak_live']}")
+    stream = FrameStream(None, every_output=True)
+    alone = FloodServer(batch_size=32, max_live=1).serve(requests(stream), realtime=False)
+    stream.close()
+    a = alone.summary()
+    log(f"one at a time: {a['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s")
+    return {
+        'workload': f'{len(specs)} solves, Poisson arrivals {args.serve_rate}/s, two-level meshes '
+                    f'{args.serve_min_cells}-{args.serve_max_cells} cells (log-uniform), '
+                    f'{args.serve_min_h}-{args.serve_max_h} h simulated, rain peak U[20,150] mm/h, tide on half, '
+                    'outputs every 300 s',
+        'value': s['cell_updates_per_s'], 'unit': 'cell-updates/s', 'p50_latency_s': s['p50_latency_s'],
+        'p99_latency_s': s['p99_latency_s'], 'plain_value': a['cell_updates_per_s'],
+        'served_vs_plain': s['cell_updates_per_s'] / a['cell_updates_per_s'], 'served': s, 'one_at_a_time': a,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -305,6 +365,12 @@ def main():
                     help='also copy a 256 MiB buffer 3 times (reference dispatch for HBM counters)')
     ap.add_argument('--batch-slots', type=int, default=64, help='scenarios of the same-mesh batch (0: skip)')
     ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
+    ap.add_argument('--serve-requests', type=int, default=12, help='solves of the serving load (0: skip)')
+    ap.add_argument('--serve-rate', type=float, default=2.0)
+    ap.add_argument('--serve-min-cells', type=int, default=50_000)
+    ap.add_argument('--serve-max-cells', type=int, default=1_000_000)
+    ap.add_argument('--serve-min-h', type=float, default=1.0)
+    ap.add_argument('--serve-max-h', type=float, default=6.0)
     ap.add_argument('--pmc-summary', nargs=3, metavar=('FETCH_CSV', 'WRITE_CSV', 'BENCH_JSON'),
                     help='summarise two rocprofv3 --pmc runs of this script (no GPU needed) and exit')
     args = ap.parse_args()
@@ -441,6 +507,8 @@ def main():
         result['copy_calibration_bytes'] = 2 * src.numel() * 4
     if args.batch_slots:
         result['batched'] = batched_bench(args, device)
+    if args.serve_requests:
+        result['served'] = served_bench(args)
     if not args.no_cpu:
         del eng
         torch.cuda.synchronize()

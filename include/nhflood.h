@@ -146,6 +146,12 @@ int32_t nhf_begin(nhf_ctx *ctx, uint64_t target_ticks, uint64_t origin_ticks);
  * physics (clock.py:94-167). */
 int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace);
 
+/* nhf_run_batch in two halves, so one host thread can keep several contexts (each on its own
+ * stream) busy at once: nhf_launch_batch enqueues the batch and its trace copy and returns;
+ * nhf_wait_batch waits for that batch and copies its trace. One batch in flight per context. */
+int32_t nhf_launch_batch(nhf_ctx *ctx, int32_t steps);
+int32_t nhf_wait_batch(nhf_ctx *ctx, uint32_t *trace);
+
 /* Standalone compute_cfl on the current state (solver_compact.py:617-641, called once at
  * :889 before the first batch). */
 int32_t nhf_compute_cfl(nhf_ctx *ctx);
@@ -229,6 +235,9 @@ int32_t nhb_begin(nhb_ctx *ctx, int32_t slot, uint64_t target_ticks, uint64_t or
 /* nhf_run_batch for every slot at once: each slot takes up to `steps` steps towards its own
  * target with its own dt; trace receives slots x steps x 9 words. */
 int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace);
+/* nhb_run_batch in two halves, as nhf_launch_batch / nhf_wait_batch. */
+int32_t nhb_launch_batch(nhb_ctx *ctx, int32_t steps);
+int32_t nhb_wait_batch(nhb_ctx *ctx, uint32_t *trace);
 /* nhf_compute_cfl for every slot. */
 int32_t nhb_compute_cfl(nhb_ctx *ctx);
 /* nhf_flush_budget / nhf_limit_capture / nhf_cfl_state of one slot. */

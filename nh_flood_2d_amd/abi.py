@@ -66,6 +66,8 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_set_forcing': (c_int32, [vp, vp, vp, c_int32]),
         'nhf_begin': (c_int32, [vp, c_uint64, c_uint64]),
         'nhf_run_batch': (c_int32, [vp, c_int32, vp]),
+        'nhf_launch_batch': (c_int32, [vp, c_int32]),
+        'nhf_wait_batch': (c_int32, [vp, vp]),
         'nhf_compute_cfl': (c_int32, [vp]),
         'nhf_flush_budget': (c_int32, [vp, vp]),
         'nhf_limit_capture': (c_int32, [vp, vp, vp, c_int32]),
@@ -86,6 +88,8 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
             'nhb_set_forcing': (c_int32, [vp, c_int32, vp, vp, c_int32]),
             'nhb_begin': (c_int32, [vp, c_int32, c_uint64, c_uint64]),
             'nhb_run_batch': (c_int32, [vp, c_int32, vp]),
+            'nhb_launch_batch': (c_int32, [vp, c_int32]),
+            'nhb_wait_batch': (c_int32, [vp, vp]),
             'nhb_compute_cfl': (c_int32, [vp]),
             'nhb_flush_budget': (c_int32, [vp, c_int32, vp]),
             'nhb_limit_capture': (c_int32, [vp, c_int32, vp, vp, c_int32]),
@@ -106,10 +110,11 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
 
 EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_create',
             'nhf_destroy', 'nhf_reset', 'nhf_set_forcing', 'nhf_begin', 'nhf_run_batch',
+            'nhf_launch_batch', 'nhf_wait_batch',
             'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
             'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe',
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
-            'nhb_run_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
+            'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
             'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times']
 
 _hip_lib = None

@@ -119,7 +119,7 @@ class BatchEngine:
         abi.check(self.lib, rc, 'nhb_create')
         self.ctx = handle
         self.terms = 4 + 6 * layout.b_num
-        self._trace = np.zeros((S, abi.MAX_BATCH, abi.TRACE_COLS), dtype=np.uint32)
+        self._pending = 0
 
     def _check_compatible(self, layout: Layout, params: abi.Params) -> None:
         if (layout.e_num, layout.s_num) != (self.layout.e_num, self.layout.s_num):
@@ -168,11 +168,21 @@ class BatchEngine:
     # ------------------------------------------------------------------ batch protocol
     def run_batch(self, steps: int) -> np.ndarray:
         """Advance every slot with a begun target; returns the [slots, steps, 9] trace."""
-        out = self._trace[:, :steps]
         buf = np.zeros((self.slots, steps, abi.TRACE_COLS), dtype=np.uint32)
         rc = self.lib.nhb_run_batch(self.ctx, int(steps), buf.ctypes.data)
         abi.check(self.lib, rc, 'nhb_run_batch')
-        out[:] = buf
+        self.batches += 1
+        return buf
+
+    def launch_batch(self, steps: int) -> None:
+        """Enqueue one batch for every slot with a begun target and return at once."""
+        abi.check(self.lib, self.lib.nhb_launch_batch(self.ctx, int(steps)), 'nhb_launch_batch')
+        self._pending = int(steps)
+
+    def wait_batch(self) -> np.ndarray:
+        """Wait for the launched batch; returns the [slots, steps, 9] trace."""
+        buf = np.zeros((self.slots, self._pending, abi.TRACE_COLS), dtype=np.uint32)
+        abi.check(self.lib, self.lib.nhb_wait_batch(self.ctx, buf.ctypes.data), 'nhb_wait_batch')
         self.batches += 1
         return buf
 

@@ -61,13 +61,22 @@ struct Stage {
 };
 enum { STAGE_BEGIN = 1, STAGE_COEFF = 2, STAGE_RAIN = 4, STAGE_RAINING = 8, STAGE_RAINING_ON = 16 };
 
-#define SLOTS (1 << d.shift)
-#define IX(e) ((((size_t)(e)) << d.shift) | (size_t)s)
+// Index of entity e, slot s in a per-scenario array; `sh` is log2 of the slot count in scope.
+#define IX(e) ((((size_t)(e)) << sh) | (size_t)s)
+
+// Entity of flat thread index t. With 64 slots a wavefront is exactly one entity; readfirstlane
+// makes that explicit, so the entity index and everything derived from it (static mesh loads,
+// loop bounds over side lists, array bases) are wave-uniform and live in scalar registers.
+template <bool W>
+__device__ __forceinline__ int entity_of(size_t t, int sh) {
+    return W ? __builtin_amdgcn_readfirstlane((int)(t >> 6)) : (int)(t >> sh);
+}
 
 // --------------------------------------------------------------------------- clock per slot
 
 // prepare_step (solver_compact.py:364-382) for slot s.
 __device__ void prepare_slot_b(const BDev &d, int s, int step) {
+    const int sh = d.shift;
     Ctl *c = &d.ctl[s];
     clock_choose(c, c->ndt, d.p.min_dt, step);
     d.dt[s] = c->active ? c->dt : 0.0f;
@@ -101,7 +110,7 @@ __device__ void prepare_slot_b(const BDev &d, int s, int step) {
 // --------------------------------------------------------------------------- small kernels
 
 __global__ void kb_reset(BDev d, int s, uint64_t now) {
-    const int S = SLOTS;
+    const int S = 1 << d.shift;
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     int stride = gridDim.x * blockDim.x;
     if (tid == 0) {
@@ -155,8 +164,8 @@ __global__ void kb_reset_limit(BDev d, int s) {
 
 // --------------------------------------------------------------------------- hanging gradients
 
-__device__ void face_difference_b(const BDev &d, int s, int cell, int dir, int axis, float &value,
-                                  float &distance, int &usable) {
+__device__ __forceinline__ void face_difference_b(const BDev &d, int sh, int s, int cell, int dir, int axis,
+                                                  float &value, float &distance, int &usable) {
     const nhf_mesh &m = d.m;
     float v = 0.0f, dist = 0.0f, total = 0.0f;
     int ok = 1;
@@ -189,11 +198,12 @@ __device__ void face_difference_b(const BDev &d, int s, int cell, int dir, int a
     usable = ok;
 }
 
+template <bool W>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_gradients(BDev d) {
-    const int S = SLOTS;
+    const int sh = W ? 6 : d.shift;
     size_t t = (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x;
-    int g = (int)(t >> d.shift);
-    int s = (int)(t & (size_t)(S - 1));
+    int g = entity_of<W>(t, sh);
+    int s = (int)(t & ((1u << sh) - 1));
     if (g >= d.m.n_grad || d.dt[s] == 0.0f) return;
     int cell = d.m.grad_cell[g];
     int axes = d.m.grad_axes[g];
@@ -204,8 +214,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_gradients(BDev d) {
         if (h > d.p.min_h) {
             float dm, xm, dp, xp;
             int okm, okp;
-            face_difference_b(d, s, cell, axis == 1 ? 2 : 0, axis, dm, xm, okm);
-            face_difference_b(d, s, cell, axis == 1 ? 3 : 1, axis, dp, xp, okp);
+            face_difference_b(d, sh, s, cell, axis == 1 ? 2 : 0, axis, dm, xm, okm);
+            face_difference_b(d, sh, s, cell, axis == 1 ? 3 : 1, axis, dp, xp, okp);
             if (okm && okp)
                 value = (dp - dm) / (xp - xm);
             else if (okm)
@@ -219,11 +229,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_gradients(BDev d) {
 
 // --------------------------------------------------------------------------- theta < 1
 
+template <bool W>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_face_avg(BDev d) {
-    const int S = SLOTS;
+    const int sh = W ? 6 : d.shift;
     size_t t = (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x;
-    int cell = (int)(t >> d.shift);
-    int s = (int)(t & (size_t)(S - 1));
+    int cell = entity_of<W>(t, sh);
+    int s = (int)(t & ((1u << sh) - 1));
     if (cell >= d.m.e_num || d.dt[s] == 0.0f) return;
     const nhf_mesh &m = d.m;
     uint32_t cnt = m.csr_count[cell];
@@ -249,11 +260,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_face_avg(BDev d) {
 
 // --------------------------------------------------------------------------- (a) face flux
 
+template <bool W>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
-    const int S = SLOTS;
+    const int sh = W ? 6 : d.shift;
     size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
-    int si = (int)(t >> d.shift) + 1;
-    int s = (int)(t & (size_t)(S - 1));
+    int si = entity_of<W>(t, sh) + 1;
+    int s = (int)(t & ((1u << sh) - 1));
     if (si >= d.m.s_num) return;
     const float dt = d.dt[s];
     if (dt == 0.0f) return;
@@ -297,11 +309,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
 
 // --------------------------------------------------------------------------- donor scale
 
+template <bool W>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
-    const int S = SLOTS;
+    const int sh = W ? 6 : d.shift;
     size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
-    int cell = (int)(t >> d.shift) + 1;
-    int s = (int)(t & (size_t)(S - 1));
+    int cell = entity_of<W>(t, sh) + 1;
+    int s = (int)(t & ((1u << sh) - 1));
     if (cell >= d.m.e_num) return;
     const float dt = d.dt[s];
     if (dt == 0.0f) return;
@@ -366,20 +379,22 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
 // keeps one slot and sums its cells' infiltration and floor terms in FP64 registers; the block
 // then adds them to its own compensated accumulators. Boundary cells leave their terms in bvals
 // for kb_finish.
+template <bool W>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
-    const int S = SLOTS;
+    const int sh = W ? 6 : d.shift;
+    const int S = 1 << sh;
     const int s = threadIdx.x & (S - 1);
     const float dt = d.dt[s];
     const nhf_mesh &m = d.m;
     const Ctl *c = &d.ctl[s];
     double a_infil = 0.0, a_floor = 0.0;
     if (dt != 0.0f) {
-        const size_t total = (size_t)(m.e_num - 1) << d.shift;
+        const size_t total = (size_t)(m.e_num - 1) << sh;
         const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
         const int raining = d.raining[s];
         const uint64_t step_end = c->step_end;
         for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += stride) {
-            const int cell = (int)(t >> d.shift) + 1;
+            const int cell = entity_of<W>(t, sh) + 1;
             const uint32_t meta = m.cell_meta[cell];
             const int bi = meta_boundary(meta);
             const int kind = bi >= 0 ? m.kind[bi] : -1;
@@ -489,8 +504,10 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
 
 // --------------------------------------------------------------------------- (c) CFL dt
 
+template <bool W>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
-    const int S = SLOTS;
+    const int sh = W ? 6 : d.shift;
+    const int S = 1 << sh;
     const int s = threadIdx.x & (S - 1);
     const bool on = mode == 0 || d.dt[s] != 0.0f;
     const nhf_mesh &m = d.m;
@@ -498,10 +515,10 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
     int32_t idx = 0;
     if (on) {
         const float afa = d.p.afa, g = d.p.g;
-        const size_t total = (size_t)(m.s_num - 1) << d.shift;
+        const size_t total = (size_t)(m.s_num - 1) << sh;
         const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
         for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += stride) {
-            const int si = (int)(t >> d.shift) + 1;
+            const int si = entity_of<W>(t, sh) + 1;
             int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
             if (lh.x != 0 && lh.y != 0) {
                 float zl = m.bed[lh.x], zh = m.bed[lh.y];
@@ -546,7 +563,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
 // One workgroup per slot: reduce its CFL partials to ndt and side; in a batch (mode 1) also add
 // the boundary cells' budget terms, commit `step` and prepare `step + 1`.
 __global__ __launch_bounds__(NHF_BLOCK) void kb_finish(BDev d, int mode, int step, int steps) {
-    const int S = SLOTS;
+    const int sh = d.shift;
+    const int S = 1 << sh;
     const int s = blockIdx.x;
     Ctl *c = &d.ctl[s];
     if (mode == 1 && d.dt[s] == 0.0f) return;
@@ -621,7 +639,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_finish(BDev d, int mode, int ste
 // Interval budget of slot s: the depth blocks' (sum, compensation) pairs of terms 0 and 1 are
 // combined in double-double in a fixed order; the boundary terms are copied; all are cleared.
 __global__ __launch_bounds__(NHF_BLOCK) void kb_flush(BDev d, int s) {
-    const int S = SLOTS;
+    const int S = 1 << d.shift;
     __shared__ double s_hi[NHF_BLOCK], s_lo[NHF_BLOCK];
     for (int term = 0; term < 2; ++term) {
         double hi = 0.0, lo = 0.0;
@@ -680,6 +698,8 @@ struct nhb_ctx {
     Stage st_host, st_dev;
     Ctl *host_ctl;
     double2 *host_flush;
+    hipEvent_t done, stage_done;  // batch trace copied; staged words copied
+    int pending_steps, stage_inflight;
 };
 
 static size_t align_b(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -777,21 +797,29 @@ static int32_t b_launch_batch(nhb_ctx *ctx, int steps, bool timed) {
     } while (0)
     const size_t cells = (size_t)(d.m.e_num - 1), sides = (size_t)(d.m.s_num - 1);
     BLAUNCH(KB_PREPARE, kb_prepare, dim3((unsigned)S), dim3(64), 0, s, d, steps);
+    const bool w = S == 64;
     for (int step = 0; step < steps; ++step) {
         if (d.m.n_grad > 0)
-            BLAUNCH(KB_GRAD, kb_gradients, dim3(b_blocks((size_t)d.m.n_grad * S)), dim3(NHF_BLOCK), 0, s, d);
+            BLAUNCH(KB_GRAD, w ? kb_gradients<true> : kb_gradients<false>, dim3(b_blocks((size_t)d.m.n_grad * S)),
+                    dim3(NHF_BLOCK), 0, s, d);
         if (!d.theta_one)
-            BLAUNCH(KB_FACE, kb_face_avg, dim3(b_blocks((size_t)d.m.e_num * S)), dim3(NHF_BLOCK), 0, s, d);
-        if (sides > 0) BLAUNCH(KB_FLUX, kb_flux, dim3(b_blocks_xcd(sides * S)), dim3(NHF_BLOCK), 0, s, d);
-        BLAUNCH(KB_DONOR, kb_donor, dim3(b_blocks_xcd(cells * S)), dim3(NHF_BLOCK), 0, s, d);
-        BLAUNCH(KB_DEPTH, kb_depth, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
-        BLAUNCH(KB_CFL, kb_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
+            BLAUNCH(KB_FACE, w ? kb_face_avg<true> : kb_face_avg<false>, dim3(b_blocks((size_t)d.m.e_num * S)),
+                    dim3(NHF_BLOCK), 0, s, d);
+        if (sides > 0)
+            BLAUNCH(KB_FLUX, w ? kb_flux<true> : kb_flux<false>, dim3(b_blocks_xcd(sides * S)), dim3(NHF_BLOCK), 0, s,
+                    d);
+        BLAUNCH(KB_DONOR, w ? kb_donor<true> : kb_donor<false>, dim3(b_blocks_xcd(cells * S)), dim3(NHF_BLOCK), 0, s,
+                d);
+        BLAUNCH(KB_DEPTH, w ? kb_depth<true> : kb_depth<false>, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
+        BLAUNCH(KB_CFL, w ? kb_cfl<true> : kb_cfl<false>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
         BLAUNCH(KB_FINISH, kb_finish, dim3((unsigned)S), dim3(NHF_BLOCK), 0, s, d, 1, step, steps);
     }
 #undef BLAUNCH
     HIP_TRY(hipGetLastError());
     return 0;
 }
+
+static int32_t settle_stage(nhb_ctx *ctx);
 
 extern "C" {
 
@@ -891,6 +919,16 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
         return set_error("nhb_create: staging allocation", err);
     }
     memset(ctx->stage_host, 0, ctx->stage_bytes);
+    ctx->pending_steps = ctx->stage_inflight = 0;
+    if ((err = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming)) != hipSuccess ||
+        (err = hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming)) != hipSuccess) {
+        hipHostFree(ctx->stage_host);
+        hipFree(ctx->stage_dev);
+        hipHostFree(ctx->host_ctl);
+        hipHostFree(ctx->host_flush);
+        delete ctx;
+        return set_error("nhb_create: events", err);
+    }
     stage_layout(d, slots, ctx->st_host, ctx->stage_host);
     stage_layout(d, slots, ctx->st_dev, ctx->stage_dev);
     // every slot starts idle with a clean clock
@@ -914,6 +952,8 @@ int32_t nhb_destroy(nhb_ctx *ctx) {
     for (int i = 0; i <= NHF_MAX_BATCH; ++i)
         if (ctx->exec[i]) hipGraphExecDestroy(ctx->exec[i]);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+    hipEventDestroy(ctx->done);
+    hipEventDestroy(ctx->stage_done);
     hipHostFree(ctx->stage_host);
     hipFree(ctx->stage_dev);
     hipHostFree(ctx->host_ctl);
@@ -929,6 +969,7 @@ static int32_t check_slot(const nhb_ctx *ctx, int32_t slot) {
 
 int32_t nhb_reset(nhb_ctx *ctx, int32_t slot, uint64_t now_ticks) {
     if (check_slot(ctx, slot)) return -2;
+    if (int32_t rc = settle_stage(ctx)) return rc;
     ctx->st_host.flags[slot] = 0;
     hipLaunchKernelGGL(kb_reset, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, (int)slot, now_ticks);
     HIP_TRY(hipGetLastError());
@@ -938,6 +979,7 @@ int32_t nhb_reset(nhb_ctx *ctx, int32_t slot, uint64_t now_ticks) {
 int32_t nhb_set_forcing(nhb_ctx *ctx, int32_t slot, const float *coefficients, const float *rain_rates,
                         int32_t raining) {
     if (check_slot(ctx, slot)) return -2;
+    if (int32_t rc = settle_stage(ctx)) return rc;
     const BDev &d = ctx->d;
     Stage &st = ctx->st_host;
     int32_t flags = st.flags[slot] & ~(STAGE_RAINING | STAGE_RAINING_ON);
@@ -956,28 +998,41 @@ int32_t nhb_set_forcing(nhb_ctx *ctx, int32_t slot, const float *coefficients, c
 
 int32_t nhb_begin(nhb_ctx *ctx, int32_t slot, uint64_t target_ticks, uint64_t origin_ticks) {
     if (check_slot(ctx, slot)) return -2;
+    if (int32_t rc = settle_stage(ctx)) return rc;
     ctx->st_host.target[slot] = target_ticks;
     ctx->st_host.origin[slot] = origin_ticks;
     ctx->st_host.flags[slot] |= STAGE_BEGIN;
     return 0;
 }
 
-// Upload and apply the staged words of every slot; the host copy is reused after the next sync.
+// The staged words go to the device with an asynchronous copy from pinned memory; the host
+// copy is cleared (and may be written again) only once that copy has completed.
+static int32_t settle_stage(nhb_ctx *ctx) {
+    if (!ctx->stage_inflight) return 0;
+    HIP_TRY(hipEventSynchronize(ctx->stage_done));
+    for (int s = 0; s < ctx->slots; ++s) ctx->st_host.flags[s] = 0;
+    ctx->stage_inflight = 0;
+    return 0;
+}
+
+// Upload and apply the staged words of every slot.
 static int32_t apply_stage(nhb_ctx *ctx) {
+    int32_t rc = settle_stage(ctx);
+    if (rc) return rc;
     bool any = false;
     for (int s = 0; s < ctx->slots; ++s) any |= ctx->st_host.flags[s] != 0;
     if (!any) return 0;
     HIP_TRY(hipMemcpyAsync(ctx->stage_dev, ctx->stage_host, ctx->stage_bytes, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(kb_apply, dim3(ctx->slots), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, ctx->st_dev);
     HIP_TRY(hipGetLastError());
-    // the copy reads the pinned staging asynchronously: wait before the flags are reused
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (int s = 0; s < ctx->slots; ++s) ctx->st_host.flags[s] = 0;
+    HIP_TRY(hipEventRecord(ctx->stage_done, ctx->stream));
+    ctx->stage_inflight = 1;
     return 0;
 }
 
-int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace) {
-    if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhb_run_batch: steps out of range");
+int32_t nhb_launch_batch(nhb_ctx *ctx, int32_t steps) {
+    if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhb_launch_batch: steps out of range");
+    if (ctx->pending_steps) return set_error_text("nhb_launch_batch: the previous batch was not waited for");
     int32_t rc = apply_stage(ctx);
     if (rc) return rc;
     hipStream_t s = ctx->stream;
@@ -999,7 +1054,18 @@ int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace) {
         if (rc) return rc;
     }
     HIP_TRY(hipMemcpyAsync(ctx->host_ctl, ctx->d.ctl, sizeof(Ctl) * ctx->slots, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipEventRecord(ctx->done, s));
+    ctx->pending_steps = steps;
+    return 0;
+}
+
+int32_t nhb_wait_batch(nhb_ctx *ctx, uint32_t *trace) {
+    const int steps = ctx->pending_steps;
+    if (!steps) return set_error_text("nhb_wait_batch: no batch in flight");
+    ctx->pending_steps = 0;
+    HIP_TRY(hipEventSynchronize(ctx->done));
+    int32_t rc = settle_stage(ctx);
+    if (rc) return rc;
     if (ctx->timing) {
         for (size_t i = 0; i < ctx->event_kind.size(); ++i) {
             float ms = 0.0f;
@@ -1015,11 +1081,20 @@ int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace) {
     return 0;
 }
 
+int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace) {
+    int32_t rc = nhb_launch_batch(ctx, steps);
+    if (rc) return rc;
+    return nhb_wait_batch(ctx, trace);
+}
+
 int32_t nhb_compute_cfl(nhb_ctx *ctx) {
     const BDev &d = ctx->d;
     int32_t rc = apply_stage(ctx);
     if (rc) return rc;
-    hipLaunchKernelGGL(kb_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+    if (ctx->slots == 64)
+        hipLaunchKernelGGL(kb_cfl<true>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+    else
+        hipLaunchKernelGGL(kb_cfl<false>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
     hipLaunchKernelGGL(kb_finish, dim3(ctx->slots), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0, 0);
     HIP_TRY(hipGetLastError());
     return 0;

@@ -108,13 +108,7 @@ static __device__ __forceinline__ double det_log2(double x) {
     }
     double s = (m - 1.0) / (m + 1.0);
     double s2 = s * s;
-    double p = 1.0 / 25.0;
-    p = fma(p, s2, 1.0 / 23.0);
-    p = fma(p, s2, 1.0 / 21.0);
-    p = fma(p, s2, 1.0 / 19.0);
-    p = fma(p, s2, 1.0 / 17.0);
-    p = fma(p, s2, 1.0 / 15.0);
-    p = fma(p, s2, 1.0 / 13.0);
+    double p = 1.0 / 13.0;  // |s| < 0.1716: the first omitted term is below 2^-37 of ln m
     p = fma(p, s2, 1.0 / 11.0);
     p = fma(p, s2, 1.0 / 9.0);
     p = fma(p, s2, 1.0 / 7.0);
@@ -130,11 +124,7 @@ static __device__ __forceinline__ double det_exp2(double t) {
     if (t > 1100.0) return INFINITY;
     double n = rint(t);
     double u = (t - n) * 0.69314718055994530942;
-    double p = 1.0 / 87178291200.0;
-    p = fma(p, u, 1.0 / 6227020800.0);
-    p = fma(p, u, 1.0 / 479001600.0);
-    p = fma(p, u, 1.0 / 39916800.0);
-    p = fma(p, u, 1.0 / 3628800.0);
+    double p = 1.0 / 3628800.0;  // 1/10!; |u| <= ln(2)/2: the first omitted term is below 2^-36
     p = fma(p, u, 1.0 / 362880.0);
     p = fma(p, u, 1.0 / 40320.0);
     p = fma(p, u, 1.0 / 5040.0);

@@ -592,6 +592,8 @@ struct nhf_ctx {
     uint32_t *host_trace;
     double2 *host_acc;
     Ctl *probe_scratch;
+    hipEvent_t done;    // recorded after a launched batch's trace copy
+    int pending_steps;  // steps of the batch in flight, 0 when none
 };
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -696,6 +698,13 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     }
     ctx->host_acc = nullptr;
     ctx->probe_scratch = nullptr;
+    ctx->pending_steps = 0;
+    err = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming);
+    if (err != hipSuccess) {
+        hipHostFree(ctx->host_trace);
+        delete ctx;
+        return set_error("hipEventCreate", err);
+    }
     memset(ctx->kernel_ms, 0, sizeof ctx->kernel_ms);
     memset(ctx->kernel_count, 0, sizeof ctx->kernel_count);
     *out = ctx;
@@ -708,6 +717,7 @@ int32_t nhf_destroy(nhf_ctx *ctx) {
     for (int i = 0; i <= NHF_MAX_BATCH; ++i)
         if (ctx->exec[i]) hipGraphExecDestroy(ctx->exec[i]);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+    hipEventDestroy(ctx->done);
     if (ctx->host_trace) hipHostFree(ctx->host_trace);
     if (ctx->host_acc) hipHostFree(ctx->host_acc);
     delete ctx;
@@ -785,8 +795,9 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
     return 0;
 }
 
-int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
-    if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhf_run_batch: steps out of range");
+int32_t nhf_launch_batch(nhf_ctx *ctx, int32_t steps) {
+    if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhf_launch_batch: steps out of range");
+    if (ctx->pending_steps) return set_error_text("nhf_launch_batch: the previous batch was not waited for");
     hipStream_t s = ctx->stream;
     if (ctx->graph_enabled && !ctx->timing && s != nullptr) {
         if (!ctx->exec[steps]) {
@@ -807,7 +818,16 @@ int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
     }
     HIP_TRY(hipMemcpyAsync(ctx->host_trace, ctx->d.ctl->trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps,
                            hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipEventRecord(ctx->done, s));
+    ctx->pending_steps = steps;
+    return 0;
+}
+
+int32_t nhf_wait_batch(nhf_ctx *ctx, uint32_t *trace) {
+    const int steps = ctx->pending_steps;
+    if (!steps) return set_error_text("nhf_wait_batch: no batch in flight");
+    ctx->pending_steps = 0;
+    HIP_TRY(hipEventSynchronize(ctx->done));
     if (ctx->timing) {
         for (size_t i = 0; i < ctx->event_kind.size(); ++i) {
             float ms = 0.0f;
@@ -818,6 +838,12 @@ int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
     }
     if (trace) memcpy(trace, ctx->host_trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps);
     return 0;
+}
+
+int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
+    int32_t rc = nhf_launch_batch(ctx, steps);
+    if (rc) return rc;
+    return nhf_wait_batch(ctx, trace);
 }
 
 int32_t nhf_compute_cfl(nhf_ctx *ctx) {

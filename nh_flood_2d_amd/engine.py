@@ -162,6 +162,17 @@ class Engine:
         abi.check(self.lib, rc, 'nhf_run_batch')
         return self._trace[:steps].copy()
 
+    def launch_batch(self, steps: int) -> None:
+        """Enqueue a batch on this engine's stream and return at once (see wait_batch)."""
+        abi.check(self.lib, self.lib.nhf_launch_batch(self.ctx, int(steps)), 'nhf_launch_batch')
+        self._pending = int(steps)
+
+    def wait_batch(self) -> np.ndarray:
+        """Wait for the launched batch and return its trace rows."""
+        rc = self.lib.nhf_wait_batch(self.ctx, self._trace.ctypes.data)
+        abi.check(self.lib, rc, 'nhf_wait_batch')
+        return self._trace[:self._pending].copy()
+
     def compute_cfl(self) -> None:
         abi.check(self.lib, self.lib.nhf_compute_cfl(self.ctx), 'nhf_compute_cfl')
 

@@ -1,0 +1,214 @@
+"""Serving: a request loop that runs flood solves of uneven size and duration on one GPU.
+
+Requests arrive over time. Each is admitted when its device footprint fits the HBM budget; it then
+keeps its mesh and state resident and runs the unchanged host protocol of
+``solver_compact.py:930-985`` (:class:`~nh_flood_2d_amd.simulation.Simulation`). Every loop
+iteration plans all live solves, enqueues one batch for each on its own HIP stream (so the
+device overlaps small meshes instead of running them one after another) and then consumes the
+traces in turn. A solve that finishes frees its memory and its slot for the queue at once
+(continuous batching). Requests that share a mesh can share a :class:`BatchEngine`, one slot each.
+
+Depth frames leave the loop through :class:`FrameStream`, whose file writes run on a background
+thread. The reference serves one solve per process; this loop has no counterpart there, and each
+solve gives the same results alone and in the mix (same kernels, same arithmetic per solve).
+"""
+from __future__ import annotations
+
+import queue
+import threading
+from dataclasses import dataclass, field
+from pathlib import Path
+from time import perf_counter, sleep
+from typing import Callable
+
+import numpy as np
+
+from .engine import Engine, TorchDevice
+
+
+def footprint_bytes(layout, slots: int = 1) -> int:
+    """Device bytes of one solve (or a batch of `slots`): static arrays, state, scratch."""
+    static = sum(int(np.asarray(v).nbytes) for k, v in layout.arrays.items() if not k.startswith('_'))
+    e, s = layout.e_num, layout.s_num
+    state = slots * (e * (5 * 4 + 8) + s * 4) + e * 4
+    scratch = slots * (e * 4 + s * 4 + (16 * e if layout.theta != 1. else 0)) + (1 << 20)
+    return static + state + scratch
+
+
+@dataclass
+class Request:
+    """One flood solve: `build(engine_factory)` returns its Simulation; `arrival` is in seconds
+    after the server starts; `cells` counts cell-updates; `footprint` is its device bytes."""
+    key: object
+    build: Callable
+    arrival: float
+    cells: int
+    footprint: int
+    group: object = None       # requests with the same group share a BatchEngine (same mesh)
+
+
+@dataclass
+class ServeReport:
+    latency: dict = field(default_factory=dict)      # key -> arrival to final output, s
+    queued: dict = field(default_factory=dict)       # key -> arrival to admission, s
+    results: dict = field(default_factory=dict)      # key -> final mass-balance row
+    steps: dict = field(default_factory=dict)
+    cell_updates: int = 0
+    wall: float = 0.
+    peak_live: int = 0
+    peak_bytes: int = 0
+    batches: int = 0
+
+    def percentile(self, p: float) -> float:
+        values = np.array(sorted(self.latency.values()))
+        return float(np.percentile(values, p)) if len(values) else float('nan')
+
+    def summary(self) -> dict:
+        return dict(requests=len(self.latency), p50_latency_s=self.percentile(50),
+                    p99_latency_s=self.percentile(99), max_latency_s=max(self.latency.values(), default=0.),
+                    mean_queue_s=float(np.mean(list(self.queued.values()))) if self.queued else 0.,
+                    cell_updates=self.cell_updates, wall_s=self.wall,
+                    cell_updates_per_s=self.cell_updates / self.wall if self.wall else 0.,
+                    peak_live=self.peak_live, peak_device_bytes=self.peak_bytes, loop_iterations=self.batches)
+
+
+class FrameStream:
+    """Output sink for served solves: mass-balance rows are kept, depth frames are written as FDB
+    frame tables by a background thread (or only counted when `root` is None)."""
+
+    def __init__(self, root: Path | None = None, every_output: bool = False):
+        self.root = Path(root) if root is not None else None
+        self.every_output = every_output
+        self.rows = {}
+        self.frames = {}
+        self._queue = queue.Queue()
+        self._thread = threading.Thread(target=self._writer, daemon=True)
+        self._thread.start()
+
+    def _writer(self):
+        from .output import write_depth_frame
+        while True:
+            item = self._queue.get()
+            if item is None:
+                break
+            path, depth = item
+            write_depth_frame(path, depth)
+
+    def sink(self, key):
+        stream = self
+
+        class _Sink:
+            def emit(self, sim, seconds, phase, frame, final, depth, row, stats):
+                stream.rows.setdefault(key, []).append(row)
+                if frame or stream.every_output:
+                    stream.frames[key] = stream.frames.get(key, 0) + 1
+                    if stream.root is not None:
+                        stream._queue.put((stream.root / str(key) / f'depth_{int(seconds)}.fdb', depth.copy()))
+
+            def finish(self, sim):
+                pass
+        return _Sink()
+
+    def close(self):
+        self._queue.put(None)
+        self._thread.join()
+
+
+class FloodServer:
+    """Continuous batching of flood solves on one GPU, admitted by device footprint."""
+
+    def __init__(self, budget_bytes: int | None = None, batch_size: int = 32, device: str = 'cuda:0',
+                 max_live: int = 64, group_slots: int = 8, device_factory: Callable | None = None):
+        if budget_bytes is None:
+            import torch
+            total = torch.cuda.get_device_properties(device).total_memory
+            budget_bytes = int(0.85 * total)
+        self.budget = budget_bytes
+        self.batch_size = batch_size
+        self.device_factory = device_factory or (lambda: TorchDevice(device))
+        self.max_live = max_live
+        self.group_slots = group_slots
+
+    def _engine_factory(self, stream_device):
+        def make(layout, params, state, pipe_q=None):
+            return Engine(layout, params, state, device=stream_device, pipe_q=pipe_q)
+        return make
+
+    def serve(self, requests: list[Request], realtime: bool = True) -> ServeReport:
+        """Run every request; arrivals are honoured on the wall clock when `realtime`."""
+        from .batch import BatchEngine
+        report = ServeReport()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        waiting: list[Request] = []
+        live = {}          # key -> (sim, request, owner): owner is the Engine or the BatchEngine
+        groups = {}        # group -> BatchEngine
+        group_use = {}     # group -> live members
+        used = 0
+        t0 = perf_counter()
+        while pending or waiting or live:
+            now = perf_counter() - t0
+            while pending and (pending[0].arrival <= now or not realtime):
+                waiting.append(pending.pop(0))
+            # admission, first come first served, by footprint
+            while waiting and len(live) < self.max_live:
+                r = waiting[0]
+                if used + r.footprint > self.budget and live:
+                    break
+                waiting.pop(0)
+                if r.group is not None:
+                    batch = groups.get(r.group)
+                    if batch is None or not batch.free_slots():
+                        batch = BatchEngine(self.group_slots, device=self.device_factory())
+                        groups[r.group] = batch
+                        group_use[id(batch)] = 0
+                    sim = r.build(batch.view())
+                    owner = batch
+                    group_use[id(batch)] += 1
+                else:
+                    sim = r.build(self._engine_factory(self.device_factory()))
+                    owner = sim.engine
+                if sim.settings.batch_size != self.batch_size:
+                    raise ValueError('every served solve uses the server batch size')
+                sim.start()
+                live[r.key] = (sim, r, owner)
+                used += r.footprint
+                report.queued[r.key] = perf_counter() - t0 - r.arrival
+            report.peak_live = max(report.peak_live, len(live))
+            report.peak_bytes = max(report.peak_bytes, used)
+            # plan every live solve; finish the ones that are done
+            planned = []
+            for key, (sim, r, owner) in list(live.items()):
+                if sim.plan():
+                    planned.append((key, sim, r, owner))
+                    continue
+                report.results[key] = sim.finish()
+                report.latency[key] = perf_counter() - t0 - r.arrival
+                report.steps[key] = sim.steps
+                report.cell_updates += r.cells * sim.steps
+                sim.engine.close()
+                if owner is not sim.engine:
+                    group_use[id(owner)] -= 1
+                    if group_use[id(owner)] == 0:
+                        for g, b in list(groups.items()):
+                            if b is owner:
+                                del groups[g]
+                        owner.close()
+                del live[key]
+                used -= r.footprint
+            if not planned:
+                if not live and not waiting and pending and realtime:
+                    sleep(max(0., min(0.05, pending[0].arrival - (perf_counter() - t0))))
+                continue
+            # one batch per engine, all enqueued before any is waited for
+            launched = {}
+            for key, sim, r, owner in planned:
+                if id(owner) not in launched:
+                    owner.launch_batch(self.batch_size)
+                    launched[id(owner)] = owner
+            traces = {oid: owner.wait_batch() for oid, owner in launched.items()}
+            for key, sim, r, owner in planned:
+                trace = traces[id(owner)]
+                sim.consume(trace[sim.engine.slot] if owner is not sim.engine else trace)
+            report.batches += 1
+        report.wall = perf_counter() - t0
+        return report

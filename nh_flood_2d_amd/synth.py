@@ -294,7 +294,7 @@ class Scenario:
 
 def triangular_hyetograph(peak_mm_h: float, peak_s: float, total_s: float, band_s: float):
     """Band starts/ends and mm per band of a triangular rain-rate hyetograph."""
-    edges = np.arange(0., total_s + band_s / 2, band_s)
+    edges = np.append(np.arange(0., total_s, band_s), total_s)  # the last band may be shorter
     starts, ends = edges[:-1], edges[1:]
     # exact band average of the piecewise-linear rate (mm/h), times band hours
     def cumulative(t):
@@ -457,11 +457,12 @@ def small_multires(seed: int = 1, blocks: int = 24, block_size: float = 8., band
 
 
 def two_level_fractal(seed: int = 0, blocks: int = 790, block_size: float = 10., refine: float = 0.2,
-                      duration: float = 7200.) -> Scenario:
+                      duration: float = 7200., tide: float = 0.) -> Scenario:
     """One shared mesh for scenario batches: a 2-level (block_size / block_size/2) quadtree over a
     seeded diamond-square DEM (amplitude 20 m), with the lowest `refine` fraction of the blocks
     refined; about blocks^2 x (1 + 3 refine) cells (one million at the defaults). One land class
-    with Manning n 0.03 and Horton (60, 10, 2) mm/h, 1/h; closed boundaries."""
+    with Manning n 0.03 and Horton (60, 10, 2) mm/h, 1/h; closed boundaries, or with `tide` > 0 a
+    level boundary on the left edge at tide x sin(2 pi t / 44640 s)."""
     rng = np.random.default_rng(seed)
     extent = blocks * block_size
     power = int(np.ceil(np.log2(blocks))) + 1
@@ -481,8 +482,14 @@ def two_level_fractal(seed: int = 0, blocks: int = 790, block_size: float = 10.,
     mesh = quadtree_mesh(levels, block_size, bed_fn, land_fn)
     land = {1: dict(manning_n=0.03, infiltration=dict(f0_mm_h=60, fc_mm_h=10, k_per_h=2))}
     starts, ends, mm = triangular_hyetograph(80., duration / 3, duration, 300.)
+    boundaries = []
+    if tide > 0:
+        side_ids, _, faces, _, _ = mesh.exterior_edges()
+        knots = np.arange(0., duration + 300., 300.)
+        boundaries.append(BoundarySpec('tide', 'level', np.sort(side_ids[faces == 0]),
+                                       SeriesSpec(knots, tide * np.sin(2 * np.pi * knots / 44640.))))
     return Scenario(f'two_level_{mesh.e_num - 1}_cells', mesh, land, starts, ends, mm[:, None],
-                    np.zeros(mesh.e_num, dtype=np.int32), [], duration, 0.5, 1.0, 0.02)
+                    np.zeros(mesh.e_num, dtype=np.int32), boundaries, duration, 0.5, 1.0, 0.02)
 
 
 def scenario_variants(base: Scenario, count: int, seed: int = 0, peak_range=(20., 150.),

@@ -48,13 +48,7 @@ static double det_log2(double x) {
     }
     double s = (m - 1.0) / (m + 1.0);
     double s2 = s * s;
-    double p = 1.0 / 25.0;
-    p = fma(p, s2, 1.0 / 23.0);
-    p = fma(p, s2, 1.0 / 21.0);
-    p = fma(p, s2, 1.0 / 19.0);
-    p = fma(p, s2, 1.0 / 17.0);
-    p = fma(p, s2, 1.0 / 15.0);
-    p = fma(p, s2, 1.0 / 13.0);
+    double p = 1.0 / 13.0;  // |s| < 0.1716: the first omitted term is below 2^-37 of ln m
     p = fma(p, s2, 1.0 / 11.0);
     p = fma(p, s2, 1.0 / 9.0);
     p = fma(p, s2, 1.0 / 7.0);
@@ -70,11 +64,7 @@ static double det_exp2(double t) {
     if (t > 1100.0) return INFINITY;
     double n = rint(t);
     double u = (t - n) * 0.69314718055994530942;
-    double p = 1.0 / 87178291200.0; /* 1/14! */
-    p = fma(p, u, 1.0 / 6227020800.0);
-    p = fma(p, u, 1.0 / 479001600.0);
-    p = fma(p, u, 1.0 / 39916800.0);
-    p = fma(p, u, 1.0 / 3628800.0);
+    double p = 1.0 / 3628800.0;  // 1/10!; |u| <= ln(2)/2: the first omitted term is below 2^-36
     p = fma(p, u, 1.0 / 362880.0);
     p = fma(p, u, 1.0 / 40320.0);
     p = fma(p, u, 1.0 / 5040.0);
@@ -145,6 +135,7 @@ struct nhf_ctx {
     int active;
     uint32_t trace[NHF_MAX_BATCH * NHF_TRACE_COLS];
     int slot;
+    int pending_steps; /* steps of the batch run by nhf_launch_batch, until nhf_wait_batch */
     /* CFL and limiter state */
     real ndt;
     int32_t cfl_arg;
@@ -816,6 +807,20 @@ static void finish_step(nhf_ctx *c) {
     }
     clock_commit(c, c->slot);
     c->slot += 1;
+}
+
+int32_t nhf_launch_batch(nhf_ctx *c, int32_t steps) {
+    if (c->pending_steps) return fail("the previous batch was not waited for");
+    int32_t rc = nhf_run_batch(c, steps, NULL);
+    if (rc == 0) c->pending_steps = steps;
+    return rc;
+}
+
+int32_t nhf_wait_batch(nhf_ctx *c, uint32_t *trace) {
+    if (!c->pending_steps) return fail("no batch in flight");
+    if (trace) memcpy(trace, c->trace, sizeof(uint32_t) * NHF_TRACE_COLS * c->pending_steps);
+    c->pending_steps = 0;
+    return 0;
 }
 
 int32_t nhf_run_batch(nhf_ctx *c, int32_t steps, uint32_t *trace) {

@@ -327,3 +327,28 @@ def test_split_face_reconstruction_table(lib):
                                        grads.ctypes.data, diffs.ctypes.data) == 0
         assert grads[1] == pytest.approx(gradient, abs=2e-7), depths
         np.testing.assert_allclose(diffs, difference, atol=3e-7, rtol=0)
+
+
+def test_deterministic_pow_and_exp_round_like_the_correct_f32_result():
+    """x^(7/3) (friction) and e^x (Horton) of the oracle, which the HIP kernels evaluate with the same
+    FP64 operations: against the F32 rounding of the FP64 libm result on 200k samples each, every
+    result within 1 ulp and at most 1 in 10^4 not the correctly rounded one."""
+    lib = oracle.load('f32')
+    lib.nhf_oracle_math.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(731)
+    n = 200_000
+    x = (np.float32(10.0) ** rng.uniform(-5.0, 2.4, n)).astype(np.float32)
+    x[:4] = [1e-5, 1.0, 0.0, 2.0]
+    y = np.full(n, np.float32(7.0) / np.float32(3.0), dtype=np.float32)
+    pow_out = np.zeros(n, np.float32)
+    exp_out = np.zeros(n, np.float32)
+    lib.nhf_oracle_math(x.ctypes.data, y.ctypes.data, n, pow_out.ctypes.data, exp_out.ctypes.data)
+    ref = np.array([math.pow(float(a), float(y[0])) for a in x]).astype(np.float32)
+    ulp = np.abs(pow_out.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1 and np.count_nonzero(ulp) <= n // 10_000
+    assert pow_out[1] == 1.0 and pow_out[2] == 0.0
+    z = (-rng.uniform(0.0, 40.0, n)).astype(np.float32)
+    lib.nhf_oracle_math(x.ctypes.data, z.ctypes.data, n, pow_out.ctypes.data, exp_out.ctypes.data)
+    ref = np.array([math.exp(float(a)) for a in z]).astype(np.float32)
+    ulp = np.abs(exp_out.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1 and np.count_nonzero(ulp) <= n // 10_000

@@ -326,18 +326,18 @@ def served_bench(args) -> dict:
         for i, sc, layout, arr in specs:
             def build(factory, sc=sc, layout=layout, i=i):
                 return synth.simulation(sc, engine_factory=factory, yield_step=300., base_layout=layout,
-                                        outputs=stream.sink(i))
+                                        batch_size=64, outputs=stream.sink(i))
             out.append(Request(i, build, arr, sc.mesh.e_num - 1, footprint_bytes(layout)))
         return out
 
     stream = FrameStream(None, every_output=True)
-    served = FloodServer(batch_size=32).serve(requests(stream), realtime=True)
+    served = FloodServer(batch_size=64).serve(requests(stream), realtime=True)
     stream.close()
     s = served.summary()
     log(f"served: {s['requests']} solves, p50 {s['p50_latency_s']:.2f} s, p99 {s['p99_latency_s']:.2f} s, "
         f"{s['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s, peak live {s['peak_live']}")
     stream = FrameStream(None, every_output=True)
-    alone = FloodServer(batch_size=32, max_live=1).serve(requests(stream), realtime=False)
+    alone = FloodServer(batch_size=64, max_live=1).serve(requests(stream), realtime=False)
     stream.close()
     a = alone.summary()
     log(f"one at a time: {a['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s")

@@ -118,7 +118,8 @@ class FloodServer:
     """Continuous batching of flood solves on one GPU, admitted by device footprint."""
 
     def __init__(self, budget_bytes: int | None = None, batch_size: int = 32, device: str = 'cuda:0',
-                 max_live: int = 64, group_slots: int = 8, device_factory: Callable | None = None):
+                 max_live: int = 64, group_slots: int = 8, device_factory: Callable | None = None,
+                 graph: bool = True):
         if budget_bytes is None:
             import torch
             total = torch.cuda.get_device_properties(device).total_memory
@@ -128,6 +129,7 @@ class FloodServer:
         self.device_factory = device_factory or (lambda: TorchDevice(device))
         self.max_live = max_live
         self.group_slots = group_slots
+        self.graph = graph
 
     def _engine_factory(self, stream_device):
         def make(layout, params, state, pipe_q=None):
@@ -169,6 +171,8 @@ class FloodServer:
                     owner = sim.engine
                 if sim.settings.batch_size != self.batch_size:
                     raise ValueError('every served solve uses the server batch size')
+                if self.graph:
+                    sim.engine.set_graph(True)   # one captured graph launch per batch
                 sim.start()
                 live[r.key] = (sim, r, owner)
                 used += r.footprint

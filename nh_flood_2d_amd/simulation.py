@@ -417,39 +417,25 @@ class Simulation:
         return True
 
     def consume(self, rows: np.ndarray) -> None:
-        """Read one batch's trace rows (clock, forcing integration, statistics) and write outputs."""
+        """Read one batch's trace rows (clock, forcing integration, statistics) and write outputs.
+
+        The rows are handled as arrays, with every F64 running total accumulated in step order
+        (np.add.accumulate is sequential), so the totals equal those of a step-by-step loop."""
         s = self.settings
         end_time, ob = self.end_time, self.origin_before_start
         warming = self._warming
         elapsed = self.elapsed
         self.batches += 1
-        completed = []
-        for row in rows:
-            status = int(row[3])
-            if status == abi.DONE:
-                break
-            if status != abi.COMPLETED:
-                if status == abi.BAD_CFL:
-                    self.supported_dt()
-                raise RuntimeError(f'Device clock status {status} at {elapsed}s')
-            completed.append(row[0])
-            self.host_now_ticks = row_end_ticks(row)
-            scale = f32_from_bits(row[8])
-            for activity in (self.interval_activity, self.total_activity):
-                activity[0] += int(row[6])
-                activity[1] += int(row[7])
-                activity[2] = min(activity[2], scale)
-            dt = f32_from_bits(row[0])
-            for bi, b in enumerate(self.boundaries):
-                self.q_exact[bi] = b.series.at(0. if warming else elapsed + dt / 2.)
-            self.budget.add_forcing(dt, 0. if warming else self.rain_volume_rate, self.q_exact)
-            elapsed = self.host_now_ticks / TICKS_PER_SECOND - ob
-            self.steps += 1
-            self.warmup_steps += int(warming)
-            self.dt_stats = [min(self.dt_stats[0], dt), max(self.dt_stats[1], dt), self.dt_stats[2] + dt]
-            np.maximum(self.discharge_peaks, self.q_exact, out=self.discharge_peaks)
-            (self.interval_event_dts if row[5] else self.interval_dts).append(dt)
-        self.dt_hasher.update(np.array(completed, dtype=np.uint32).tobytes())
+        status = rows[:, 3].astype(np.int64)
+        stop = np.flatnonzero(status != abi.COMPLETED)
+        n = int(stop[0]) if len(stop) else len(rows)
+        if n < len(rows) and int(status[n]) != abi.DONE:
+            # the steps before the failure still count, as in a step-by-step loop
+            self.elapsed = self._consume_rows(rows[:n], warming, elapsed)
+            if int(status[n]) == abi.BAD_CFL:
+                self.supported_dt()
+            raise RuntimeError(f'Device clock status {int(status[n])} at {self.elapsed}s')
+        elapsed = self._consume_rows(rows[:n], warming, elapsed)
         self.elapsed = elapsed
         is_output = abs(elapsed - self.next_output) < 1e-9 or elapsed == end_time
         is_frame = abs(elapsed - self.next_frame) < 1e-9 or elapsed == end_time
@@ -462,6 +448,43 @@ class Simulation:
                 self.next_output = min(self.next_output + s.yield_step, 0. if elapsed < 0 else end_time)
             if is_frame:
                 self.next_frame = min(self.next_frame + FRAME_INTERVAL_S, 0. if elapsed < 0 else end_time)
+
+    def _consume_rows(self, r: np.ndarray, warming: bool, elapsed: float) -> float:
+        """Account completed trace rows `r`; returns the elapsed seconds after the last one."""
+        n = len(r)
+        if n == 0:
+            return elapsed
+        ob = self.origin_before_start
+        words = np.ascontiguousarray(r[:, 0], dtype=np.uint32)
+        dt = words.view(np.float32).astype(np.float64)
+        ends = (r[:, 1].astype(np.uint64) << np.uint64(32)) | r[:, 2].astype(np.uint64)
+        self.host_now_ticks = int(ends[-1])
+        scales = np.ascontiguousarray(r[:, 8], dtype=np.uint32).view(np.float32)
+        for activity in (self.interval_activity, self.total_activity):
+            activity[0] += int(r[:, 6].astype(np.int64).sum())
+            activity[1] += int(r[:, 7].astype(np.int64).sum())
+            activity[2] = min(activity[2], float(scales.min()))
+        after = ends.astype(np.float64) / TICKS_PER_SECOND - ob
+        before = np.concatenate([[elapsed], after[:-1]])
+        mid = np.zeros(n) if warming else before + dt / 2.
+        rain_rate = 0. if warming else self.rain_volume_rate
+        b = self.budget
+        b.rain = np.add.accumulate(np.concatenate([[b.rain], dt * rain_rate]))[-1]
+        for bi, bd in enumerate(self.boundaries):
+            values = np.interp(mid, bd.series.times, bd.series.values)
+            if bd.kind == 'discharge':
+                b.discharge[bi] = np.add.accumulate(np.concatenate([[b.discharge[bi]], dt * values]))[-1]
+            self.discharge_peaks[bi] = max(self.discharge_peaks[bi], float(values.max()))
+            self.q_exact[bi] = values[-1]
+        self.steps += n
+        self.warmup_steps += n if warming else 0
+        self.dt_stats = [min(self.dt_stats[0], float(dt.min())), max(self.dt_stats[1], float(dt.max())),
+                         float(np.add.accumulate(np.concatenate([[self.dt_stats[2]], dt]))[-1])]
+        short = r[:, 5] != 0
+        self.interval_event_dts.extend(dt[short].tolist())
+        self.interval_dts.extend(dt[~short].tolist())
+        self.dt_hasher.update(words.tobytes())
+        return float(after[-1])
 
     def finish(self):
         """Record the run's totals and close the outputs; returns the last mass-balance row."""

@@ -63,6 +63,23 @@ static inline const char *missing_mesh_array(const nhf_mesh *m, bool state, bool
     return nullptr;
 }
 
+// --------------------------------------------------------------------------- side lists
+
+// On a 2:1 quadtree a cell lists at most 8 sides (two per face). The cell passes load such a list
+// with all of its loads issued together (ids, then every side's values) instead of one dependent
+// load chain per entry; longer lists take a plain loop.
+#define NHF_LIST_MAX 8
+
+static __device__ __forceinline__ void load_list(const int32_t *isl, int k0, int total, int ids[NHF_LIST_MAX]) {
+#pragma unroll
+    for (int j = 0; j < NHF_LIST_MAX; ++j) ids[j] = j < total ? isl[k0 + j] : 0;
+}
+
+// Face (0 left, 1 right, 2 bottom, 3 top) of list entry j, from the running face counts.
+static __device__ __forceinline__ int entry_face(int j, int n0, int n01, int n012) {
+    return j < n0 ? 0 : (j < n01 ? 1 : (j < n012 ? 2 : 3));
+}
+
 // --------------------------------------------------------------------------- control block
 
 // One scenario's clock, step scalars, limiter counters, CFL-limit capture and batch trace.

@@ -312,15 +312,39 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
         float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
         float available = m.depth[cell] * area + positive + rainq * dt * area;
         float outgoing = (fmx(-pq, 0.0f) + fmx(-discharge, 0.0f)) * dt;
-        uint32_t cnt = m.csr_count[cell];
-        int k = m.csr_start[cell];
-        for (int dir = 0; dir < 4; ++dir) {
-            int n = face_n(cnt, dir);
-            for (int j = 0; j < n; ++j, ++k) {
-                int side = m.isl[k];
-                float sq = d.qflux[side] * m.length[side];
-                if (dir == 0 || dir == 2) sq = -sq;
-                outgoing += fmx(sq, 0.0f) * dt;
+        const uint32_t cnt = m.csr_count[cell];
+        const int k0 = m.csr_start[cell];
+        const int n0 = face_n(cnt, 0), n01 = n0 + face_n(cnt, 1), n012 = n01 + face_n(cnt, 2);
+        const int total = n012 + face_n(cnt, 3);
+        const bool short_list = total <= NHF_LIST_MAX;
+        int ids[NHF_LIST_MAX];
+        float qf[NHF_LIST_MAX];
+        if (short_list) {
+            float len[NHF_LIST_MAX];
+            load_list(m.isl, k0, total, ids);
+#pragma unroll
+            for (int j = 0; j < NHF_LIST_MAX; ++j) {
+                qf[j] = j < total ? d.qflux[ids[j]] : 0.0f;
+                len[j] = j < total ? m.length[ids[j]] : 0.0f;
+            }
+#pragma unroll
+            for (int j = 0; j < NHF_LIST_MAX; ++j) {
+                if (j < total) {
+                    float sq = qf[j] * len[j];
+                    if (!(entry_face(j, n0, n01, n012) & 1)) sq = -sq;  // left and bottom faces
+                    outgoing += fmx(sq, 0.0f) * dt;
+                }
+            }
+        } else {
+            int k = k0;
+            for (int dir = 0; dir < 4; ++dir) {
+                int n = face_n(cnt, dir);
+                for (int j = 0; j < n; ++j, ++k) {
+                    int side = m.isl[k];
+                    float sq = d.qflux[side] * m.length[side];
+                    if (dir == 0 || dir == 2) sq = -sq;
+                    outgoing += fmx(sq, 0.0f) * dt;
+                }
             }
         }
         float ds = 1.0f;
@@ -333,17 +357,27 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
         // donor, the lower cell when q >= 0 and the higher cell otherwise. This cell is the
         // lower cell of its right/top sides, so it is the donor of a side exactly when
         // (q >= 0) == (face is right/top), and it stores that side's scaled discharge: one
-        // writer per side, and the second pass over the list hits the cache.
-        k = m.csr_start[cell];
-        for (int dir = 0; dir < 4; ++dir) {
-            int n = face_n(cnt, dir);
-            const bool upper = dir & 1;
-            for (int j = 0; j < n; ++j, ++k) {
-                int side = m.isl[k];
-                float qf = d.qflux[side];
-                if ((qf >= 0.0f) == upper) {
-                    m.q[side] = qf * ds;
-                    if (qf != 0.0f && ds < 1.0f) faces += 1;
+        // writer per side.
+        if (short_list) {
+#pragma unroll
+            for (int j = 0; j < NHF_LIST_MAX; ++j) {
+                if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
+                    m.q[ids[j]] = qf[j] * ds;
+                    if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+                }
+            }
+        } else {
+            int k = k0;
+            for (int dir = 0; dir < 4; ++dir) {
+                int n = face_n(cnt, dir);
+                const bool upper = dir & 1;
+                for (int j = 0; j < n; ++j, ++k) {
+                    int side = m.isl[k];
+                    float q1 = d.qflux[side];
+                    if ((q1 >= 0.0f) == upper) {
+                        m.q[side] = q1 * ds;
+                        if (q1 != 0.0f && ds < 1.0f) faces += 1;
+                    }
                 }
             }
         }
@@ -403,20 +437,52 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
             cap = m.capacity[cell];
         }
         // face totals of the donor-scaled discharge (stored in q by k_donor and k_flux)
-        float fsum[4];
-        uint32_t cnt = m.csr_count[cell];
-        int k = m.csr_start[cell];
+        float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
+        const uint32_t cnt = m.csr_count[cell];
+        const int k0 = m.csr_start[cell];
+        const int n0 = face_n(cnt, 0), n01 = n0 + face_n(cnt, 1), n012 = n01 + face_n(cnt, 2);
+        const int total = n012 + face_n(cnt, 3);
+        if (total <= NHF_LIST_MAX) {
+            int ids[NHF_LIST_MAX];
+            float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
+            load_list(m.isl, k0, total, ids);
 #pragma unroll
-        for (int dir = 0; dir < 4; ++dir) {
-            int n = face_n(cnt, dir);
-            float acc = 0.0f;
-            for (int j = 0; j < n; ++j, ++k) {
-                int side = m.isl[k];
-                acc += m.q[side] * m.length[side];
+            for (int j = 0; j < NHF_LIST_MAX; ++j) {
+                qv[j] = j < total ? m.q[ids[j]] : 0.0f;
+                len[j] = j < total ? m.length[ids[j]] : 0.0f;
             }
-            fsum[dir] = acc;
+#pragma unroll
+            for (int j = 0; j < NHF_LIST_MAX; ++j) {
+                if (j < total) {
+                    const float p = qv[j] * len[j];
+                    const int dir = entry_face(j, n0, n01, n012);
+                    if (dir == 0)
+                        ql += p;
+                    else if (dir == 1)
+                        qr += p;
+                    else if (dir == 2)
+                        qb += p;
+                    else
+                        qt += p;
+                }
+            }
+        } else {
+            float fsum[4];
+            int k = k0;
+            for (int dir = 0; dir < 4; ++dir) {
+                int n = face_n(cnt, dir);
+                float acc = 0.0f;
+                for (int j = 0; j < n; ++j, ++k) {
+                    int side = m.isl[k];
+                    acc += m.q[side] * m.length[side];
+                }
+                fsum[dir] = acc;
+            }
+            ql = fsum[0];
+            qr = fsum[1];
+            qb = fsum[2];
+            qt = fsum[3];
         }
-        float ql = fsum[0], qr = fsum[1], qb = fsum[2], qt = fsum[3];
         float tq = ql - qr + qb - qt + ssq;
         float area = m.area[cell];
         float rain_depth = rainq * dt;

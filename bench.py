@@ -259,11 +259,18 @@ def batched_bench(args, device) -> dict:
         views.append(view)
     batch.compute_cfl()
     batch.set_graph(True)
-    now = [start] * len(views)
-    _run_protocol(views, batch.run_batch, args.warmup, horizon, now)
+    btune, per_graph = None, 32
+    if args.tune:
+        from nh_flood_2d_amd.tune import tune_batch
+        btune = tune_batch(batch, views, steps=32, rounds=1)
+        per_graph = btune['steps_per_graph']
+        log(f"batch tuned: depth grid {btune['depth_blocks']}, CFL grid {btune['cfl_blocks']}, "
+            f"{per_graph} steps per graph, {btune['us_per_step']:.0f} us/step")
+    now = [start] * len(views)   # forcing origins; the slots' clocks continue from the tuning steps
+    _run_protocol(views, batch.run_batch, args.warmup, horizon, now, per_graph)
     batch.synchronize()
     w0 = time.perf_counter()
-    steps_batch = _run_protocol(views, batch.run_batch, args.steps, horizon, now)
+    steps_batch = _run_protocol(views, batch.run_batch, args.steps, horizon, now, per_graph)
     batch.synchronize()
     batch_wall = time.perf_counter() - w0
     batch_value = cells * steps_batch / batch_wall
@@ -271,7 +278,7 @@ def batched_bench(args, device) -> dict:
     batch.set_graph(False)
     batch.set_timing(True)
     batch.kernel_times(reset=True)
-    _run_protocol(views, batch.run_batch, args.kernel_steps, horizon, now)
+    _run_protocol(views, batch.run_batch, args.kernel_steps, horizon, now, per_graph)
     ms, count = batch.kernel_times(reset=True)
     batch.set_timing(False)
     kernels = {name: dict(avg_us=ms[i] / count[i] * 1e3, launches=int(count[i]))
@@ -289,6 +296,7 @@ def batched_bench(args, device) -> dict:
         'plain_value': plain_value, 'batched_vs_plain': batch_value / plain_value,
         'scenario_steps': steps_batch, 'wall_s': batch_wall, 'plain_steps': steps_plain,
         'simulated_s_min': min(sim_seconds), 'simulated_s_max': max(sim_seconds), 'kernels': kernels,
+        'tuning': btune, 'batch_steps': per_graph,
     }
 
 
@@ -363,6 +371,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='also copy a 256 MiB buffer 3 times (reference dispatch for HBM counters)')
+    ap.add_argument('--no-tune', dest='tune', action='store_false',
+                    help='skip the autotuner (default launch grids, 32 steps per graph)')
     ap.add_argument('--batch-slots', type=int, default=64, help='scenarios of the same-mesh batch (0: skip)')
     ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
     ap.add_argument('--serve-requests', type=int, default=12, help='solves of the serving load (0: skip)')
@@ -403,7 +413,22 @@ def main():
     state = initial_state(mesh, layout)
     state['depth'] = depth
     device = TorchDevice()
-    eng = Engine(layout, make_params(sc.afa, sc.sita, sc.min_h, 32), state, device=device)
+    params = make_params(sc.afa, sc.sita, sc.min_h, 64)
+    tuning = None
+    per_graph = 32
+    if args.tune:
+        from nh_flood_2d_amd.tune import tuned_engine_parameters
+        c = np.zeros((max(1, len(forcing.boundaries)), 2), dtype=np.float32)
+        tuning = tuned_engine_parameters(layout, params, state, device,
+                                         forcing=(c.ravel() if forcing.boundaries else np.zeros(0, np.float32),
+                                                  forcing.rain.rates(forcing.rain.index_at(3600.)).astype(np.float32),
+                                                  True), steps=96)
+        per_graph = tuning['steps_per_graph']
+        log(f"tuned: CFL grid {tuning['cfl_blocks']}, {per_graph} steps per graph, "
+            f"{tuning['us_per_step']:.1f} us/step (candidates {tuning['table']})")
+    eng = Engine(layout, params, state, device=device)
+    if tuning:
+        eng.set_tuning(tuning['cfl_blocks'])
     start = seconds_to_ticks(3600.)
     _set_forcing(eng, forcing, 3600.)
     eng.reset(start)
@@ -416,7 +441,7 @@ def main():
         nonlocal now
         done = 0
         while done < steps:
-            n = min(32, steps - done)
+            n = min(per_graph, steps - done)
             eng.begin(horizon, now)
             rows = eng.run_batch(n)
             ok = rows[:, 3] == 1
@@ -486,7 +511,8 @@ def main():
                                'seeded channel network + diamond-square DEM, 4 Horton land classes, '
                                'tide boundary, wet start, plain single-scenario path',
                    'cells': cells, 'sides': mesh.s_num - 1, 'side_list': r_len,
-                   'hanging_edges': layout.n_hang, 'batch_steps': 32, 'launch': 'hipGraph per 32-step batch'},
+                   'hanging_edges': layout.n_hang, 'batch_steps': per_graph,
+                   'launch': f'hipGraph per {per_graph}-step batch', 'tuning': tuning},
         'roofline': {'bound': 'hbm', 'kernel': f"{dominant} ({' + '.join('k_' + k for k in dp['kernels'])})",
                      'achieved': dp['achieved_gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': dp['achieved_gbs'] / HBM_PEAK_GBS, 'traffic': dp['hbm_bytes'],

@@ -176,6 +176,12 @@ int32_t nhf_sync(nhf_ctx *ctx);
 /* Launch mode: 1 replays each batch as one captured hipGraph, 0 launches kernels one by one. */
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable);
 
+/* Launch tuning (no reference counterpart; used by the autotuner): the number of CFL partial
+ * workgroups, a multiple of 8 up to the maximum the workspace holds. nhf_tuning_range reports
+ * the current grid and that maximum. Captured graphs are rebuilt after a change. */
+int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks);
+int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *cfl_blocks, int32_t *cfl_max);
+
 /* Per-kernel timing with HIP events on the context's stream (eager mode only). Kernel ids:
  * 0 prepare, 1 gradients, 2 face averages, 3 flux, 4 donor, 5 depth, 6 cfl, 7 finish. `ms` and `count`
  * receive totals since the last call with reset != 0. */
@@ -246,6 +252,11 @@ int32_t nhb_limit_capture(nhb_ctx *ctx, int32_t slot, float *values, int32_t *id
 int32_t nhb_cfl_state(nhb_ctx *ctx, int32_t slot, float *ndt, int32_t *side);
 int32_t nhb_sync(nhb_ctx *ctx);
 int32_t nhb_set_graph(nhb_ctx *ctx, int32_t enable);
+/* Launch tuning of a batch: grid of the grid-stride depth pass (cells x slots per thread follow
+ * from it) and of the CFL pass, multiples of 8 up to the allocated maxima. out4 of
+ * nhb_tuning_range = depth grid, depth maximum, CFL grid, CFL maximum. */
+int32_t nhb_set_tuning(nhb_ctx *ctx, int32_t depth_blocks, int32_t cfl_blocks);
+int32_t nhb_tuning_range(nhb_ctx *ctx, int32_t *out4);
 int32_t nhb_set_timing(nhb_ctx *ctx, int32_t enable);
 int32_t nhb_kernel_times(nhb_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);
 

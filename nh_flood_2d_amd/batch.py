@@ -195,6 +195,15 @@ class BatchEngine:
     def set_timing(self, enable: bool) -> None:
         self.lib.nhb_set_timing(self.ctx, int(enable))
 
+    def set_tuning(self, depth_blocks: int, cfl_blocks: int) -> None:
+        rc = self.lib.nhb_set_tuning(self.ctx, int(depth_blocks), int(cfl_blocks))
+        abi.check(self.lib, rc, 'nhb_set_tuning')
+
+    def tuning_range(self) -> dict:
+        out = np.zeros(4, dtype=np.int32)
+        self.lib.nhb_tuning_range(self.ctx, out.ctypes.data)
+        return dict(depth_blocks=int(out[0]), depth_max=int(out[1]), cfl_blocks=int(out[2]), cfl_max=int(out[3]))
+
     def kernel_times(self, reset: bool = True):
         ms = np.zeros(8, dtype=np.float64)
         count = np.zeros(8, dtype=np.int64)

@@ -27,6 +27,7 @@ struct BDev {
     nhf_mesh m;  // static mesh arrays (the per-scenario state pointers of nhf_mesh are unused)
     nhf_params p;
     int32_t shift, terms, cfl_blocks, depth_blocks, n_bcells, theta_one, b_num, n_pixels;
+    int32_t cfl_max, depth_max;  // allocated partial and accumulator blocks
     // per-scenario state, [entity][slot]
     float *depth, *rain_time, *capacity, *max_depth, *max_speed, *q;
     uint64_t *peak_tick;
@@ -124,7 +125,7 @@ __global__ void kb_reset(BDev d, int s, uint64_t now) {
         c->limit_value[0] = INFINITY;
         d.dt[s] = 0.0f;
     }
-    for (int i = tid; i < d.depth_blocks * 2; i += stride) d.acc[(size_t)i * S + s] = make_double2(0.0, 0.0);
+    for (int i = tid; i < d.depth_max * 2; i += stride) d.acc[(size_t)i * S + s] = make_double2(0.0, 0.0);
     for (int i = tid; i < d.terms; i += stride) d.bacc[(size_t)s * d.terms + i] = make_double2(0.0, 0.0);
 }
 
@@ -333,16 +334,39 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     float positive = (fmx(0.0f, 0.0f) + fmx(discharge, 0.0f)) * dt;
     float available = d.depth[IX(cell)] * area + positive + rainq * dt * area;
     float outgoing = (fmx(-0.0f, 0.0f) + fmx(-discharge, 0.0f)) * dt;
-    uint32_t cnt = m.csr_count[cell];
+    const uint32_t cnt = m.csr_count[cell];
     const int k0 = m.csr_start[cell];
-    int k = k0;
-    for (int dir = 0; dir < 4; ++dir) {
-        int n = face_n(cnt, dir);
-        for (int j = 0; j < n; ++j, ++k) {
-            int side = m.isl[k];
-            float sq = d.qflux[IX(side)] * m.length[side];
-            if (dir == 0 || dir == 2) sq = -sq;
-            outgoing += fmx(sq, 0.0f) * dt;
+    const int n0 = face_n(cnt, 0), n01 = n0 + face_n(cnt, 1), n012 = n01 + face_n(cnt, 2);
+    const int total = n012 + face_n(cnt, 3);
+    const bool short_list = total <= NHF_LIST_MAX;
+    int ids[NHF_LIST_MAX];
+    float qf[NHF_LIST_MAX];
+    if (short_list) {
+        float len[NHF_LIST_MAX];
+        load_list(m.isl, k0, total, ids);
+#pragma unroll
+        for (int j = 0; j < NHF_LIST_MAX; ++j) {
+            qf[j] = j < total ? d.qflux[IX(ids[j])] : 0.0f;
+            len[j] = j < total ? m.length[ids[j]] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < NHF_LIST_MAX; ++j) {
+            if (j < total) {
+                float sq = qf[j] * len[j];
+                if (!(entry_face(j, n0, n01, n012) & 1)) sq = -sq;
+                outgoing += fmx(sq, 0.0f) * dt;
+            }
+        }
+    } else {
+        int k = k0;
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                float sq = d.qflux[IX(side)] * m.length[side];
+                if (dir == 0 || dir == 2) sq = -sq;
+                outgoing += fmx(sq, 0.0f) * dt;
+            }
         }
     }
     float ds = 1.0f;
@@ -352,16 +376,26 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
         limited = true;
     }
     uint32_t faces = 0;
-    k = k0;
-    for (int dir = 0; dir < 4; ++dir) {
-        int n = face_n(cnt, dir);
-        const bool upper = dir & 1;
-        for (int j = 0; j < n; ++j, ++k) {
-            int side = m.isl[k];
-            float qf = d.qflux[IX(side)];
-            if ((qf >= 0.0f) == upper) {
-                d.q[IX(side)] = qf * ds;
-                if (qf != 0.0f && ds < 1.0f) faces += 1;
+    if (short_list) {
+#pragma unroll
+        for (int j = 0; j < NHF_LIST_MAX; ++j) {
+            if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
+                d.q[IX(ids[j])] = qf[j] * ds;
+                if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+            }
+        }
+    } else {
+        int k = k0;
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            const bool upper = dir & 1;
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                float q1 = d.qflux[IX(side)];
+                if ((q1 >= 0.0f) == upper) {
+                    d.q[IX(side)] = q1 * ds;
+                    if (q1 != 0.0f && ds < 1.0f) faces += 1;
+                }
             }
         }
     }
@@ -419,20 +453,52 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
             } else {
                 cap = d.capacity[IX(cell)];
             }
-            float fsum[4];
-            uint32_t cnt = m.csr_count[cell];
-            int k = m.csr_start[cell];
+            float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
+            const uint32_t cnt = m.csr_count[cell];
+            const int k0 = m.csr_start[cell];
+            const int n0 = face_n(cnt, 0), n01 = n0 + face_n(cnt, 1), n012 = n01 + face_n(cnt, 2);
+            const int total = n012 + face_n(cnt, 3);
+            if (total <= NHF_LIST_MAX) {
+                int ids[NHF_LIST_MAX];
+                float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
+                load_list(m.isl, k0, total, ids);
 #pragma unroll
-            for (int dir = 0; dir < 4; ++dir) {
-                int n = face_n(cnt, dir);
-                float acc = 0.0f;
-                for (int jj = 0; jj < n; ++jj, ++k) {
-                    int side = m.isl[k];
-                    acc += d.q[IX(side)] * m.length[side];
+                for (int jj = 0; jj < NHF_LIST_MAX; ++jj) {
+                    qv[jj] = jj < total ? d.q[IX(ids[jj])] : 0.0f;
+                    len[jj] = jj < total ? m.length[ids[jj]] : 0.0f;
                 }
-                fsum[dir] = acc;
+#pragma unroll
+                for (int jj = 0; jj < NHF_LIST_MAX; ++jj) {
+                    if (jj < total) {
+                        const float p = qv[jj] * len[jj];
+                        const int dir = entry_face(jj, n0, n01, n012);
+                        if (dir == 0)
+                            ql += p;
+                        else if (dir == 1)
+                            qr += p;
+                        else if (dir == 2)
+                            qb += p;
+                        else
+                            qt += p;
+                    }
+                }
+            } else {
+                float fsum[4];
+                int k = k0;
+                for (int dir = 0; dir < 4; ++dir) {
+                    int n = face_n(cnt, dir);
+                    float acc = 0.0f;
+                    for (int jj = 0; jj < n; ++jj, ++k) {
+                        int side = m.isl[k];
+                        acc += d.q[IX(side)] * m.length[side];
+                    }
+                    fsum[dir] = acc;
+                }
+                ql = fsum[0];
+                qr = fsum[1];
+                qb = fsum[2];
+                qt = fsum[3];
             }
-            float ql = fsum[0], qr = fsum[1], qb = fsum[2], qt = fsum[3];
             float tq = ql - qr + qb - qt + ssq;
             float area = m.area[cell];
             float rain_depth = rainq * dt;
@@ -643,7 +709,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flush(BDev d, int s) {
     __shared__ double s_hi[NHF_BLOCK], s_lo[NHF_BLOCK];
     for (int term = 0; term < 2; ++term) {
         double hi = 0.0, lo = 0.0;
-        for (int b = threadIdx.x; b < d.depth_blocks; b += NHF_BLOCK) {
+        for (int b = threadIdx.x; b < d.depth_max; b += NHF_BLOCK) {
             double2 *p = d.acc + ((size_t)b * 2 + term) * S + s;
             const double2 a = *p;
             *p = make_double2(0.0, 0.0);
@@ -708,9 +774,17 @@ static int b_blocks(size_t n) { return (int)((n + NHF_BLOCK - 1) / NHF_BLOCK); }
 
 static int b_blocks_xcd(size_t n) { return (b_blocks(n) + 7) & ~7; }
 
-// Grid of the grid-stride kernels: about 2048 workgroups (8 per CU), fewer on small meshes.
-static int b_stride_grid(size_t n) {
+// Grids of the grid-stride kernels: by default about 2048 workgroups (8 per CU), fewer on small
+// meshes; nhb_set_tuning picks others up to the allocated maximum (cells or sides per thread).
+#define NHB_MAX_STRIDE_BLOCKS 8192
+
+static int b_stride_max(size_t n) {
     int g = b_blocks_xcd(n);
+    return g < NHB_MAX_STRIDE_BLOCKS ? g : NHB_MAX_STRIDE_BLOCKS;
+}
+
+static int b_stride_grid(size_t n) {
+    int g = b_stride_max(n);
     return g < 2048 ? g : 2048;
 }
 
@@ -732,8 +806,8 @@ static BCarve bcarve(const nhf_mesh *m, const nhb_state *st, const nhf_params *p
     const int b = m->b_num > 0 ? m->b_num : 1;
     const int terms = 4 + 6 * m->b_num;
     const int nb = st->n_bcells > 0 ? st->n_bcells : 1;
-    const int cfl_blocks = b_stride_grid(((size_t)(m->s_num > 1 ? m->s_num - 1 : 1)) * S);
-    const int depth_blocks = b_stride_grid(((size_t)(m->e_num - 1)) * S);
+    const int cfl_blocks = b_stride_max(((size_t)(m->s_num > 1 ? m->s_num - 1 : 1)) * S);
+    const int depth_blocks = b_stride_max(((size_t)(m->e_num - 1)) * S);
     c.ctl = off; off = align_b(off + sizeof(Ctl) * S);
     c.dt = off; off = align_b(off + sizeof(float) * S);
     c.raining = off; off = align_b(off + sizeof(int32_t) * S);
@@ -863,6 +937,8 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     d.terms = 4 + 6 * mesh->b_num;
     d.cfl_blocks = b_stride_grid(((size_t)(mesh->s_num > 1 ? mesh->s_num - 1 : 1)) * slots);
     d.depth_blocks = b_stride_grid(((size_t)(mesh->e_num - 1)) * slots);
+    d.cfl_max = b_stride_max(((size_t)(mesh->s_num > 1 ? mesh->s_num - 1 : 1)) * slots);
+    d.depth_max = b_stride_max(((size_t)(mesh->e_num - 1)) * slots);
     d.n_bcells = state->n_bcells;
     d.theta_one = params->theta == 1.0f;
     d.b_num = mesh->b_num;
@@ -1146,6 +1222,33 @@ int32_t nhb_sync(nhb_ctx *ctx) {
 
 int32_t nhb_set_graph(nhb_ctx *ctx, int32_t enable) {
     ctx->graph_enabled = enable != 0;
+    return 0;
+}
+
+int32_t nhb_set_tuning(nhb_ctx *ctx, int32_t depth_blocks, int32_t cfl_blocks) {
+    BDev &d = ctx->d;
+    if (depth_blocks < 8 || depth_blocks > d.depth_max || (depth_blocks & 7) || cfl_blocks < 8 ||
+        cfl_blocks > d.cfl_max || (cfl_blocks & 7))
+        return set_error_text("nhb_set_tuning: grid out of range or not a multiple of 8");
+    if (ctx->pending_steps) return set_error_text("nhb_set_tuning: a batch is in flight");
+    if (depth_blocks != d.depth_blocks || cfl_blocks != d.cfl_blocks) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i <= NHF_MAX_BATCH; ++i)
+            if (ctx->exec[i]) {
+                hipGraphExecDestroy(ctx->exec[i]);
+                ctx->exec[i] = nullptr;
+            }
+        d.depth_blocks = depth_blocks;
+        d.cfl_blocks = cfl_blocks;
+    }
+    return 0;
+}
+
+int32_t nhb_tuning_range(nhb_ctx *ctx, int32_t *out4) {
+    out4[0] = ctx->d.depth_blocks;
+    out4[1] = ctx->d.depth_max;
+    out4[2] = ctx->d.cfl_blocks;
+    out4[3] = ctx->d.cfl_max;
     return 0;
 }
 

@@ -40,7 +40,7 @@ struct Dev {
     uint64_t *cfl_key;
     int32_t *cfl_idx;
     double2 *acc;  // per depth-block, per term: (sum, compensation)
-    int32_t terms, cfl_blocks, depth_blocks, theta_one;
+    int32_t terms, cfl_blocks, depth_blocks, theta_one, cfl_max;
 };
 
 // prepare_step (solver_compact.py:364-382), run by one thread.
@@ -673,9 +673,18 @@ static int blocks_for(int n) { return (n + NHF_BLOCK - 1) / NHF_BLOCK; }
 // Grids of the XCD-mapped kernels are rounded up to a multiple of the 8 XCDs.
 static int blocks_xcd(int n) { return (blocks_for(n) + 7) & ~7; }
 
-// CFL partial count: enough workgroups to fill 256 CUs, few enough for one finishing workgroup.
-static int cfl_grid(int s_num) {
+// CFL partials: the workspace holds up to NHF_CFL_MAX_BLOCKS (no more than the sides need); the
+// default grid of 2048 workgroups fills 256 CUs and stays small for the finishing workgroup.
+// nhf_set_tuning picks another grid within that range.
+#define NHF_CFL_MAX_BLOCKS 4096
+
+static int cfl_max_blocks(int s_num) {
     int n = blocks_xcd(s_num - 1 > 0 ? s_num - 1 : 1);
+    return n < NHF_CFL_MAX_BLOCKS ? n : NHF_CFL_MAX_BLOCKS;
+}
+
+static int cfl_grid(int s_num) {
+    int n = cfl_max_blocks(s_num);
     return n < 2048 ? n : 2048;
 }
 
@@ -684,7 +693,7 @@ static Carve carve(const nhf_mesh *m, const nhf_params *p) {
     size_t off = 0;
     int b = m->b_num > 0 ? m->b_num : 1;
     int terms = 4 + 6 * m->b_num;
-    int cfl_blocks = cfl_grid(m->s_num);
+    int cfl_blocks = cfl_max_blocks(m->s_num);
     int depth_blocks = blocks_xcd(m->e_num - 1 > 0 ? m->e_num - 1 : 1);
     bool theta_one = p->theta == 1.0f;
     c.ctl = off; off = align_up(off + sizeof(Ctl));
@@ -750,6 +759,7 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     d.acc = (double2 *)(ws + cv.acc);
     d.terms = 4 + 6 * mesh->b_num;
     d.cfl_blocks = cfl_grid(mesh->s_num);
+    d.cfl_max = cfl_max_blocks(mesh->s_num);
     d.depth_blocks = blocks_xcd(mesh->e_num - 1);
     d.theta_one = params->theta == 1.0f;
     ctx->stream = (hipStream_t)stream;
@@ -977,6 +987,28 @@ int32_t nhf_sync(nhf_ctx *ctx) {
 
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable) {
     ctx->graph_enabled = enable != 0;
+    return 0;
+}
+
+int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks) {
+    if (cfl_blocks < 8 || cfl_blocks > ctx->d.cfl_max || (cfl_blocks & 7))
+        return set_error_text("nhf_set_tuning: CFL grid out of range or not a multiple of 8");
+    if (ctx->pending_steps) return set_error_text("nhf_set_tuning: a batch is in flight");
+    if (cfl_blocks != ctx->d.cfl_blocks) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i <= NHF_MAX_BATCH; ++i)
+            if (ctx->exec[i]) {
+                hipGraphExecDestroy(ctx->exec[i]);
+                ctx->exec[i] = nullptr;
+            }
+        ctx->d.cfl_blocks = cfl_blocks;
+    }
+    return 0;
+}
+
+int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *cfl_blocks, int32_t *cfl_max) {
+    if (cfl_blocks) *cfl_blocks = ctx->d.cfl_blocks;
+    if (cfl_max) *cfl_max = ctx->d.cfl_max;
     return 0;
 }
 

@@ -206,6 +206,14 @@ class Engine:
     def set_timing(self, enable: bool) -> None:
         self.lib.nhf_set_timing(self.ctx, int(enable))
 
+    def set_tuning(self, cfl_blocks: int) -> None:
+        abi.check(self.lib, self.lib.nhf_set_tuning(self.ctx, int(cfl_blocks)), 'nhf_set_tuning')
+
+    def tuning_range(self) -> dict:
+        cur, top = ctypes.c_int32(), ctypes.c_int32()
+        self.lib.nhf_tuning_range(self.ctx, ctypes.byref(cur), ctypes.byref(top))
+        return dict(cfl_blocks=cur.value, cfl_max=top.value)
+
     def kernel_times(self, reset: bool = True):
         ms = np.zeros(8, dtype=np.float64)
         count = np.zeros(8, dtype=np.int64)

@@ -809,6 +809,15 @@ static void finish_step(nhf_ctx *c) {
     c->slot += 1;
 }
 
+/* Launch tuning has no meaning on the CPU: accepted and ignored. */
+int32_t nhf_set_tuning(nhf_ctx *c, int32_t cfl_blocks) { return 0; }
+
+int32_t nhf_tuning_range(nhf_ctx *c, int32_t *cfl_blocks, int32_t *cfl_max) {
+    if (cfl_blocks) *cfl_blocks = 8;
+    if (cfl_max) *cfl_max = 8;
+    return 0;
+}
+
 int32_t nhf_launch_batch(nhf_ctx *c, int32_t steps) {
     if (c->pending_steps) return fail("the previous batch was not waited for");
     int32_t rc = nhf_run_batch(c, steps, NULL);

@@ -217,7 +217,7 @@ def batched_bench(args, device) -> dict:
     cells = mesh.e_num - 1
     cf = cell_forcing_arrays(mesh, synth.scenario_forcing(variants[0]))
     layout0 = build_layout(mesh, variants[0].land_parameters, cf, base.sita)
-    params = make_params(base.afa, base.sita, base.min_h, 32)
+    params = make_params(base.afa, base.sita, base.min_h, 64)
     log(f'batch workload: {len(variants)} scenarios x {cells} cells, {mesh.s_num - 1} sides ({time.time() - t0:.1f} s)')
     t_start = 1800.
     start = seconds_to_ticks(t_start)

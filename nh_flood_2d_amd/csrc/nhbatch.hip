@@ -280,29 +280,32 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
     float hl = zl + dl, hr = zh + dh;
     float q = d.q[IX(si)];
     float wh = fmx(hl, hr) - fmx(fmx(zl, zh), -999.0f);
-    float difference = hr - hl;
-    if (m.edge_index) {
-        int idx = m.edge_index[si];
-        if (idx >= 0 && dl > d.p.min_h && dh > d.p.min_h) {
-            int axis = (meta & NHF_SIDE_ORIENT_MASK) == 2 ? 1 : 0;
-            float o0 = m.hang_offset[2 * idx], o1 = m.hang_offset[2 * idx + 1];
-            if (o0 != 0.0f) difference -= d.grad[IX(2 * m.hang_slot[2 * idx] + axis)] * o0;
-            if (o1 != 0.0f) difference += d.grad[IX(2 * m.hang_slot[2 * idx + 1] + axis)] * o1;
+    // a face below min_h carries nothing (solver_compact.py:462,484): its arithmetic is skipped
+    float nq = 0.0f;
+    if (!(wh < d.p.min_h)) {
+        float difference = hr - hl;
+        if (m.edge_index) {
+            int idx = m.edge_index[si];
+            if (idx >= 0 && dl > d.p.min_h && dh > d.p.min_h) {
+                int axis = (meta & NHF_SIDE_ORIENT_MASK) == 2 ? 1 : 0;
+                float o0 = m.hang_offset[2 * idx], o1 = m.hang_offset[2 * idx + 1];
+                if (o0 != 0.0f) difference -= d.grad[IX(2 * m.hang_slot[2 * idx] + axis)] * o0;
+                if (o1 != 0.0f) difference += d.grad[IX(2 * m.hang_slot[2 * idx + 1] + axis)] * o1;
+            }
         }
+        float dq = ((-g * difference) / m.sdc[si]) * fmx(wh, 0.0f) * dt;
+        float n = d.manning ? d.manning[IX(si)] : m.manning[si];
+        float f = 1.0f;
+        if (q != 0.0f) f = 1.0f + g * dt * (n * n) * fabsf(q / det_powf(fmx(wh, 0.00001f), 7.0f / 3.0f));
+        float neighbours = 0.0f;
+        if (!d.theta_one) {
+            int vertical = (meta & NHF_SIDE_ORIENT_MASK) == 2;
+            int fa = vertical ? 0 : 2, fb = vertical ? 1 : 3;
+            neighbours = neighbour_sum(q, d.face_qn[IX(4 * lh.x + fa)], d.face_qn[IX(4 * lh.y + fb)],
+                                       m.open_face[4 * lh.x + fa], m.open_face[4 * lh.y + fb]);
+        }
+        nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / f;
     }
-    float dq = ((-g * difference) / m.sdc[si]) * fmx(wh, 0.0f) * dt;
-    float n = d.manning ? d.manning[IX(si)] : m.manning[si];
-    float f = 1.0f;
-    if (q != 0.0f) f = 1.0f + g * dt * (n * n) * fabsf(q / det_powf(fmx(wh, 0.00001f), 7.0f / 3.0f));
-    float neighbours = 0.0f;
-    if (!d.theta_one) {
-        int vertical = (meta & NHF_SIDE_ORIENT_MASK) == 2;
-        int fa = vertical ? 0 : 2, fb = vertical ? 1 : 3;
-        neighbours = neighbour_sum(q, d.face_qn[IX(4 * lh.x + fa)], d.face_qn[IX(4 * lh.y + fb)],
-                                   m.open_face[4 * lh.x + fa], m.open_face[4 * lh.y + fb]);
-    }
-    float nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / f;
-    nq = wh < d.p.min_h ? 0.0f : nq;
     nq *= bf;
     d.qflux[IX(si)] = nq;
     if (meta & NHF_SIDE_EXTERIOR) d.q[IX(si)] = nq;  // final value, as in k_flux

@@ -95,7 +95,15 @@ def launch_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
 
 TRAFFIC_SUMMARY = ROOT / 'profiles' / 'hbm_traffic_multires_channel.txt'
 PMC_KERNELS = {'k_gradients': 'gradients', 'k_face_avg': 'face_avg', 'k_flux': 'flux', 'k_donor': 'donor',
-               'k_depth': 'depth', 'k_cfl': 'cfl', 'k_finish': 'finish'}
+               'k_depth': 'depth', 'k_cfl': 'cfl', 'k_finish': 'finish',
+               'kb_gradients': 'batch_gradients', 'kb_face_avg': 'batch_face_avg', 'kb_flux': 'batch_flux',
+               'kb_donor': 'batch_donor', 'kb_depth': 'batch_depth', 'kb_cfl': 'batch_cfl', 'kb_finish': 'batch_finish'}
+
+
+def kernel_base_name(name: str) -> str:
+    """'void kb_flux<true>(BDev)' -> 'kb_flux'."""
+    m = re.match(r'\s*(?:void\s+)?([A-Za-z_]\w*)', name)
+    return m.group(1) if m else name
 
 
 def pmc_summary(fetch_csv: str, write_csv: str) -> str:
@@ -109,7 +117,7 @@ def pmc_summary(fetch_csv: str, write_csv: str) -> str:
     for counter, path in (('FETCH_SIZE', fetch_csv), ('WRITE_SIZE', write_csv)):
         per_kernel, copies = defaultdict(list), []
         for row in csv.DictReader(open(path)):
-            name = row['Kernel_Name'].split('(')[0].strip()
+            name = kernel_base_name(row['Kernel_Name'])
             value = float(row['Counter_Value']) * 1024.0
             if 'copyBuffer' in name and int(row['Grid_Size']) >= 131072:
                 copies.append(value)
@@ -126,6 +134,54 @@ def pmc_summary(fetch_csv: str, write_csv: str) -> str:
         f, nf = scaled[k].get('FETCH_SIZE', (0.0, 0))
         w, _ = scaled[k].get('WRITE_SIZE', (0.0, 0))
         lines.append(f'kernel {k} dispatches {nf} fetch_bytes {f:.0f} write_bytes {w:.0f} hbm_bytes {f + w:.0f}')
+    return '\n'.join(lines) + '\n'
+
+
+def trace_summary(trace_csv: str, bench_json: str) -> str:
+    """Per-kernel durations from a rocprofv3 kernel trace of this script: every dispatch, and the
+    dispatches of the per-kernel HIP-event pass of the single-mesh section (the figures the JSON
+    line's roofline uses). That pass is the last --kernel-steps dispatches of each step kernel on
+    the single-mesh workload's grid; CFL and finish dispatches are taken from the same interval."""
+    import csv
+    from collections import defaultdict
+    with open(bench_json) as f:
+        res = json.loads([ln for ln in f if ln.startswith('{')][-1])
+    cfg = res['config']
+    cells, sides = cfg['cells'], cfg['sides']
+    steps = res['kernels']['flux']['launches']
+    grid = lambda n: ((n + 255) // 256 + 7) // 8 * 8 * 256
+    unique = {'k_flux': grid(sides), 'k_donor': grid(cells), 'k_depth': grid(cells)}
+    rows = list(csv.DictReader(open(trace_csv)))
+    every = defaultdict(list)
+    for r in rows:
+        every[kernel_base_name(r['Kernel_Name'])].append(int(r['End_Timestamp']) - int(r['Start_Timestamp']))
+    picked = {}
+    for name, g in unique.items():
+        sel = [r for r in rows if kernel_base_name(r['Kernel_Name']) == name and int(r['Grid_Size_X']) == g]
+        picked[name] = sel[-steps:]
+    # the timing pass launches each kernel `steps` times in a row: gradients, flux, donor, depth,
+    # then CFL and finish, so the CFL and finish repeats are the first ones after the depth repeats
+    t1 = max(int(r['End_Timestamp']) for r in picked['k_depth'])
+    for name in ('k_cfl', 'k_finish'):
+        picked[name] = [r for r in rows if kernel_base_name(r['Kernel_Name']) == name
+                        and int(r['Start_Timestamp']) > t1][:steps]
+    t0 = min(int(r['Start_Timestamp']) for r in picked['k_flux'])
+    grads = [r for r in rows if kernel_base_name(r['Kernel_Name']) == 'k_gradients' and int(r['End_Timestamp']) <= t0]
+    picked['k_gradients'] = grads[-steps:]
+    lines = ['| Kernel | Dispatches (all) | Average all (us) | Dispatches (timing pass) | Average timing pass (us) | HIP-event average (us) |',
+             '| --- | ---: | ---: | ---: | ---: | ---: |']
+    short = {'k_gradients': 'gradients', 'k_flux': 'flux', 'k_donor': 'donor', 'k_depth': 'depth', 'k_cfl': 'cfl',
+             'k_finish': 'finish'}
+    for name in ('k_gradients', 'k_flux', 'k_donor', 'k_depth', 'k_cfl', 'k_finish'):
+        d = [int(r['End_Timestamp']) - int(r['Start_Timestamp']) for r in picked[name]]
+        ev = res['kernels'].get(short[name], {}).get('avg_us')
+        lines.append(f'| {name} | {len(every[name])} | {np.mean(every[name]) / 1e3:.1f} | {len(d)} | '
+                     f'{np.mean(d) / 1e3:.1f} | {ev:.1f} |' if d else f'| {name} | {len(every[name])} | - | 0 | - | - |')
+    batch = sorted((k for k in every if k.startswith('kb_')), key=lambda k: -sum(every[k]))
+    lines += ['', '| Batch kernel | Dispatches | Average (us) | Min (us) | Max (us) |', '| --- | ---: | ---: | ---: | ---: |']
+    for name in batch:
+        d = np.array(every[name]) / 1e3
+        lines.append(f'| {name} | {len(d)} | {d.mean():.1f} | {d.min():.1f} | {d.max():.1f} |')
     return '\n'.join(lines) + '\n'
 
 
@@ -366,7 +422,8 @@ def main():
     ap.add_argument('--steps', type=int, default=640)
     ap.add_argument('--warmup', type=int, default=128)
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--kernel-steps', type=int, default=64, help='steps of the per-kernel timing pass')
+    ap.add_argument('--kernel-steps', type=int, default=64, help='steps of the batch engine\'s per-kernel timing pass')
+    ap.add_argument('--kernel-repeats', type=int, default=50, help='back-to-back launches per kernel when timing kernels')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--calibrate-copy', action='store_true',
@@ -381,9 +438,14 @@ def main():
     ap.add_argument('--serve-max-cells', type=int, default=1_000_000)
     ap.add_argument('--serve-min-h', type=float, default=1.0)
     ap.add_argument('--serve-max-h', type=float, default=6.0)
+    ap.add_argument('--trace-summary', nargs=2, metavar=('KERNEL_TRACE_CSV', 'BENCH_JSON'),
+                    help='summarise a rocprofv3 --kernel-trace run of this script (no GPU needed) and exit')
     ap.add_argument('--pmc-summary', nargs=3, metavar=('FETCH_CSV', 'WRITE_CSV', 'BENCH_JSON'),
                     help='summarise two rocprofv3 --pmc runs of this script (no GPU needed) and exit')
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(*args.trace_summary), end='')
+        return
     if args.pmc_summary:
         fetch_csv, write_csv, bench_json = args.pmc_summary
         with open(bench_json) as f:
@@ -471,13 +533,12 @@ def main():
     log(f'timed: {steps} steps in {wall:.4f} s wall ({event_ms:.2f} ms by events), '
         f'{value / 1e9:.3f} G cell-updates/s, simulated {sim_seconds:.2f} s')
 
-    # per-kernel timing pass (eager launches bracketed by HIP events on the same stream)
-    eng.set_graph(False)
-    eng.set_timing(True)
-    eng.kernel_times(reset=True)
-    run(args.kernel_steps)
-    ms, count = eng.kernel_times(reset=True)
-    eng.set_timing(False)
+    # per-kernel timing: each step kernel launched --kernel-repeats times back to back on the
+    # current inputs, bracketed by HIP events on the engine's stream (the state is discarded after)
+    eng.begin(horizon, now)
+    us = eng.time_kernels(args.kernel_repeats)
+    ms = us * 1e-3 * args.kernel_repeats
+    count = np.where(us > 0, args.kernel_repeats, 0)
     nbytes = launch_bytes(layout, mesh.e_num, mesh.s_num, r_len)
     alg = algorithmic_bytes(layout, mesh.e_num, mesh.s_num, r_len)
     traffic = read_traffic(cells)
@@ -518,6 +579,8 @@ def main():
                      'frac': dp['achieved_gbs'] / HBM_PEAK_GBS, 'traffic': dp['hbm_bytes'],
                      'algorithmic_bytes': dp['algorithmic_bytes'], 'avg_us': dp['avg_us']},
         'phases': phase,
+        'phase_note': 'phase (a) skips the arithmetic and most reads of faces below min_h, so its algorithmic '
+                      'figure (every face read) can exceed the bytes it moves; hbm_bytes is what it moved',
         'kernels': kernels,
         'step': {'algorithmic_bytes': alg['step'],
                  'achieved_gbs': alg['step'] / (event_ms / steps * 1e-3) / 1e9,

@@ -188,6 +188,13 @@ int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *cfl_blocks, int32_t *cfl_max);
 int32_t nhf_set_timing(nhf_ctx *ctx, int32_t enable);
 int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);
 
+/* Benchmark hook with no reference entry point: prepare one step towards the begun target, then
+ * time each step kernel over `repeats` back-to-back launches with HIP events on the context's
+ * stream; us[id] receives the mean per launch (kernel ids as for nhf_kernel_times, 0 for kernels
+ * that do not run on this mesh). The launches repeat each pass on the same inputs except the depth
+ * pass, which updates the state in place, so the context's state is not meaningful afterwards. */
+int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us);
+
 /* Test hook with no reference entry point: evaluate StepClock.choose (clock.py:94-167) for n
  * independent cases on the device. Inputs and outputs are host arrays. out_words[i*4 + 0..3]
  * = status, dt bits, CFL bits, event-short flag; out_ticks[i*2 + 0..1] = step end tick and the

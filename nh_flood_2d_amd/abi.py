@@ -76,6 +76,7 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_set_graph': (c_int32, [vp, c_int32]),
         'nhf_set_timing': (c_int32, [vp, c_int32]),
         'nhf_set_tuning': (c_int32, [vp, c_int32]),
+        'nhf_time_kernels': (c_int32, [vp, c_int32, vp]),
         'nhf_tuning_range': (c_int32, [vp, vp, vp]),
         'nhf_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
         'nhf_clock_probe': (c_int32, [vp, vp, vp, vp, c_float, c_int32, vp, vp, vp]),
@@ -117,7 +118,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhf_launch_batch', 'nhf_wait_batch',
             'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
             'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe',
-            'nhf_set_tuning', 'nhf_tuning_range',
+            'nhf_set_tuning', 'nhf_tuning_range', 'nhf_time_kernels',
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
             'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
             'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times', 'nhb_set_tuning',

@@ -925,6 +925,40 @@ int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
     return nhf_wait_batch(ctx, trace);
 }
 
+int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
+    if (repeats < 1) return set_error_text("nhf_time_kernels: repeats must be positive");
+    if (ctx->pending_steps) return set_error_text("nhf_time_kernels: a batch is in flight");
+    const Dev &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0 = timing_event(ctx, 0), e1 = timing_event(ctx, 1);
+    if (!e0 || !e1) return set_error_text("hipEventCreate failed");
+    for (int i = 0; i < K_COUNT; ++i) us[i] = 0.0;
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, s, d, 0);
+    const int sides = d.m.s_num - 1;
+    for (int k = K_GRAD; k < K_COUNT; ++k) {
+        if ((k == K_GRAD && d.m.n_grad == 0) || (k == K_FACE && d.theta_one) || (k == K_FLUX && sides <= 0)) continue;
+        HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < repeats; ++r) {
+            switch (k) {
+            case K_GRAD: hipLaunchKernelGGL(k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_FACE: hipLaunchKernelGGL(k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_FLUX: hipLaunchKernelGGL(k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_DONOR: hipLaunchKernelGGL(k_donor, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_DEPTH: hipLaunchKernelGGL(k_depth, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_CFL: hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1); break;
+            case K_FINISH: hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, d, 0, 0, 0); break;
+            }
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        us[k] = 1e3 * (double)ms / repeats;
+    }
+    return 0;
+}
+
 int32_t nhf_compute_cfl(nhf_ctx *ctx) {
     const Dev &d = ctx->d;
     hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);

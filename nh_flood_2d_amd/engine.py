@@ -206,6 +206,13 @@ class Engine:
     def set_timing(self, enable: bool) -> None:
         self.lib.nhf_set_timing(self.ctx, int(enable))
 
+    def time_kernels(self, repeats: int = 20) -> np.ndarray:
+        """Mean microseconds per launch of each step kernel (ids as kernel_times) over `repeats`
+        back-to-back launches; leaves the state meaningless (benchmark use)."""
+        us = np.zeros(8, dtype=np.float64)
+        abi.check(self.lib, self.lib.nhf_time_kernels(self.ctx, int(repeats), us.ctypes.data), 'nhf_time_kernels')
+        return us
+
     def set_tuning(self, cfl_blocks: int) -> None:
         abi.check(self.lib, self.lib.nhf_set_tuning(self.ctx, int(cfl_blocks)), 'nhf_set_tuning')
 

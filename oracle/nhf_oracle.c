@@ -196,6 +196,24 @@ static void load_state(nhf_ctx *c) {
     for (int i = 0; i < m->s_num; ++i) c->q[i] = m->q[i];
 }
 
+/* Take in what the host wrote since the last batch (extrema resets, restored state): an element
+ * whose stored F32 value no longer equals the F32 rounding of the working value was written by
+ * the host. Unchanged elements keep their working value, so the FP64 build stays FP64. */
+static void sync_host_writes(nhf_ctx *c) {
+    const nhf_mesh *m = &c->m;
+#define NHF_SYNC(arr, n)                                          \
+    for (int i = 0; i < (n); ++i)                                 \
+        if ((float)c->arr[i] != m->arr[i]) c->arr[i] = m->arr[i];
+    NHF_SYNC(depth, m->e_num)
+    NHF_SYNC(bed, m->e_num)
+    NHF_SYNC(rain_time, m->e_num)
+    NHF_SYNC(capacity, m->e_num)
+    NHF_SYNC(max_depth, m->e_num)
+    NHF_SYNC(max_speed, m->e_num)
+    NHF_SYNC(q, m->s_num)
+#undef NHF_SYNC
+}
+
 static void store_state(nhf_ctx *c) {
     const nhf_mesh *m = &c->m;
     for (int i = 0; i < m->e_num; ++i) {
@@ -840,6 +858,7 @@ int32_t nhf_wait_batch(nhf_ctx *c, uint32_t *trace) {
 
 int32_t nhf_run_batch(nhf_ctx *c, int32_t steps, uint32_t *trace) {
     if (steps < 1 || steps > c->p.batch_capacity) return fail("steps out of range");
+    sync_host_writes(c);
     c->slot = 0;
     memset(c->trace, 0, sizeof c->trace);
     for (int s = 0; s < steps; ++s) {
@@ -857,6 +876,7 @@ int32_t nhf_run_batch(nhf_ctx *c, int32_t steps, uint32_t *trace) {
 }
 
 int32_t nhf_compute_cfl(nhf_ctx *c) {
+    sync_host_writes(c);
     compute_cfl(c);
     return 0;
 }

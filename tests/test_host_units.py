@@ -358,3 +358,30 @@ def test_hip_library_loads_and_exports_every_declared_symbol():
     m.e_num, m.s_num, m.b_num = 10, 20, 1
     p = abi.Params(batch_capacity=32, theta=1.0)
     assert lib.nhf_workspace_bytes(ctypes.byref(m), ctypes.byref(p)) > 0
+
+
+def test_oracle_takes_host_writes_between_batches():
+    """A host write between batches (an extrema reset) is what the next batch starts from."""
+    import oracle
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state
+    sc = synth.small_multires(2, pond=0.3)
+    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, synth.scenario_forcing(sc)), 1.0)
+    state = initial_state(sc.mesh, layout)
+    state['depth'] = sc.initial_depth.copy()
+    for precision in ('f32', 'f64'):
+        eng = Engine(layout, make_params(sc.afa, sc.sita, sc.min_h, 32), state, device=oracle.HostDevice(precision))
+        eng.set_forcing(np.zeros(2 * layout.b_num, np.float32), None, False)
+        eng.reset(0)
+        eng.compute_cfl()
+        eng.begin(seconds_to_ticks(100.), 0)
+        eng.run_batch(8)
+        assert eng.read_state('max_depth').max() > 0
+        eng.write_state('max_depth', np.zeros(sc.mesh.e_num, np.float32))
+        eng.begin(seconds_to_ticks(100.), 0)
+        eng.run_batch(1)
+        np.testing.assert_array_equal(eng.read_state('max_depth'), eng.read_state('depth'))
+        eng.close()

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 2
+#define NHF_ABI_VERSION 3
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -106,6 +106,10 @@ typedef struct nhf_mesh {
     const int32_t *grad_axes;         /* [n_grad] bit0 x-gradient needed, bit1 y-gradient */
     /* per-boundary */
     const int32_t *kind;              /* [b_num] 0 level, 1 discharge */
+    /* pipe coupling (solver_compact.py:535-539), NULL unless pipe_q is set: the volume each cell
+       exchanged with the pipe network in the current coupling window, summed in F32 with a
+       compensation term; the host reads and clears both at each window end */
+    float *pipe_volume, *pipe_correction;
 } nhf_mesh;
 
 typedef struct nhf_ctx nhf_ctx;

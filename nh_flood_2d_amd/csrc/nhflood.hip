@@ -427,6 +427,13 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
             t_disc = discharge * dt;
         }
         float ssq = pipe + discharge;
+        if (m.pipe_volume) {  // compensated window volume (solver_compact.py:535-539)
+            const float increment = pipe * dt - m.pipe_correction[cell];
+            const float before = m.pipe_volume[cell];
+            const float total = before + increment;
+            m.pipe_correction[cell] = (total - before) - increment;
+            m.pipe_volume[cell] = total;
+        }
         float rainq = c->raining ? d.rain_rates[m.n_pixels > 1 ? m.rain_index[cell] : 0] : 0.0f;
         float cap;
         if (rainq > 0.0f) {
@@ -738,6 +745,8 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
         return set_error_text("nhf_create: open_face is required when sita != 1");
     if (mesh->n_hang > 0 && (!mesh->edge_index || !mesh->x || !mesh->y || !mesh->grad_cell))
         return set_error_text("nhf_create: hanging reconstruction arrays missing");
+    if ((mesh->pipe_volume != nullptr) != (mesh->pipe_correction != nullptr) || (mesh->pipe_volume && !mesh->pipe_q))
+        return set_error_text("nhf_create: pipe volume and correction go together, with pipe_q");
     if (const char *name = missing_mesh_array(mesh, true, true)) {
         snprintf(nhf_error_text, sizeof nhf_error_text, "nhf_create: mesh array %s is NULL", name);
         return -2;

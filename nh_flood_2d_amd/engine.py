@@ -59,6 +59,34 @@ class TorchDevice:
             buf.copy_(self.torch.from_numpy(host.copy()).to(self.device))
         self.stream.synchronize()
 
+    def _pinned(self, buf):
+        pin = self._pins.get(id(buf)) if hasattr(self, '_pins') else None
+        if pin is None or pin.numel() != buf.numel() or pin.dtype != buf.dtype:
+            self._pins = getattr(self, '_pins', {})
+            pin = self.torch.empty(buf.numel(), dtype=buf.dtype, pin_memory=True)
+            self._pins[id(buf)] = pin
+        return pin
+
+    def upload(self, buf, values: np.ndarray) -> None:
+        """Host -> device through a pinned staging buffer kept per device array."""
+        pin = self._pinned(buf)
+        pin.numpy()[:] = np.ascontiguousarray(values).view(pin.numpy().dtype)
+        with self.torch.cuda.stream(self.stream):
+            buf.copy_(pin, non_blocking=True)
+        self.stream.synchronize()   # the staging buffer is reused at the next upload
+
+    def download(self, buf, dtype) -> np.ndarray:
+        """Device -> host through the pinned staging buffer."""
+        pin = self._pinned(buf)
+        with self.torch.cuda.stream(self.stream):
+            pin.copy_(buf, non_blocking=True)
+        self.stream.synchronize()
+        return pin.numpy().copy().view(dtype)
+
+    def zero(self, buf) -> None:
+        with self.torch.cuda.stream(self.stream):
+            buf.zero_()
+
     @staticmethod
     def ptr(buf) -> int:
         return buf.data_ptr()
@@ -92,6 +120,8 @@ class Engine:
                 self.buffers[name] = dev.array(values)
         if pipe_q is not None:
             self.buffers['pipe_q'] = dev.array(layout.cells_to_device(pipe_q.astype(np.float32)))
+            self.buffers['pipe_volume'] = dev.array(np.zeros(e_num, np.float32))
+            self.buffers['pipe_correction'] = dev.array(np.zeros(e_num, np.float32))
         m = abi.MeshStruct()
         m.e_num, m.s_num = e_num, s_num
         m.r_len = int(arr['_r_len'][0])
@@ -142,6 +172,19 @@ class Engine:
         else:
             values = self.layout.cells_to_device(values)
         self.device.write(self.buffers[name], values)
+
+    # ----------------------------------------------------------------- pipe coupling
+    def set_pipe_q(self, rates: np.ndarray) -> None:
+        """Per-cell pipe source rates (m^3/s, reference numbering) for the next coupling window."""
+        self.device.upload(self.buffers['pipe_q'], self.layout.cells_to_device(np.asarray(rates, np.float32)))
+
+    def take_pipe_volume(self) -> np.ndarray:
+        """The window's applied pipe volume per cell (m^3, reference numbering); clears it and its
+        compensation term for the next window."""
+        volume = self.device.download(self.buffers['pipe_volume'], np.float32)
+        self.device.zero(self.buffers['pipe_volume'])
+        self.device.zero(self.buffers['pipe_correction'])
+        return self.layout.cells_from_device(volume)
 
     # ----------------------------------------------------------------- batch protocol
     def reset(self, now_ticks: int) -> None:

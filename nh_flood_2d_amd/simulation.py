@@ -160,7 +160,7 @@ class Simulation:
 
     def __init__(self, mesh: Mesh, land_parameters: dict, forcing: Forcing, settings: RunSettings,
                  checkpoint=None, engine_factory=None, outputs=None, record: dict | None = None,
-                 initial_depth: np.ndarray | None = None, base_layout=None):
+                 initial_depth: np.ndarray | None = None, base_layout=None, coupling=None):
         self.mesh = mesh
         self.forcing = forcing
         self.settings = s = settings
@@ -173,6 +173,9 @@ class Simulation:
         self.end_time = end_time
         meta = checkpoint.meta if checkpoint is not None else {}
         self.continuation = bool(meta.get('run_state', False))
+        self.coupling = coupling
+        if self.continuation and coupling is not None:
+            raise ValueError('A surface checkpoint cannot restore the pipe model state')
         self.origin_before_start = (meta['clock_origin_before_start_s'] if self.continuation
                                     else float(s.warmup_duration))
         initial_clock = (((checkpoint.clock_hi << 32) | checkpoint.clock_lo) / TICKS_PER_SECOND
@@ -252,7 +255,11 @@ class Simulation:
             state['max_speed'] = speed
             state['peak_tick'] = np.full(mesh.e_num, now, dtype=np.uint64)
         state['max_depth'][0] = state['max_speed'][0] = 0.
-        self.engine = (engine_factory or Engine)(self.layout, self.params, state)
+        if coupling is not None:
+            self.engine = (engine_factory or Engine)(self.layout, self.params, state,
+                                                     pipe_q=np.zeros(mesh.e_num, np.float32))
+        else:
+            self.engine = (engine_factory or Engine)(self.layout, self.params, state)
         self.engine.reset(self.host_now_ticks)
         self.engine.compute_cfl()
         self.steps = self.warmup_steps = 0
@@ -279,6 +286,7 @@ class Simulation:
         self.next_frame = min((math.floor(el / FRAME_INTERVAL_S) + 1) * FRAME_INTERVAL_S, 0. if el < 0. else end_time)
         self.batches = 0
         self.device_seconds = 0.
+        self.pipe_end, self.pipe_pending = self.elapsed, False
 
     # ------------------------------------------------------------------ helpers
     @property
@@ -402,12 +410,23 @@ class Simulation:
         if not elapsed < end_time:
             return False
         warming = elapsed < 0.
+        if self.coupling is not None and elapsed == self.pipe_end:
+            # coupling window boundary (solver_compact.py:921-929)
+            if self.pipe_pending:
+                self.coupling.end_window(self.engine.take_pipe_volume().astype(np.float64))
+            window_end = min(elapsed + self.coupling.interval, 0. if warming else end_time)
+            source = self.coupling.begin_window(elapsed, window_end - elapsed, self.engine.read_state('depth'),
+                                                self.engine.read_state('bed'), self.area, self.mesh.land)
+            self.engine.set_pipe_q(source)
+            self.pipe_pending, self.pipe_end = True, window_end
         target = min(self.next_output, self.next_frame, 0. if warming else end_time)
         if not warming:
             now_tick = seconds_to_ticks(self.host_now_seconds)
             i = np.searchsorted(self.event_ticks, np.uint64(now_tick), side='right')
             if i < len(self.event_ticks):
                 target = min(target, int(self.event_ticks[i]) / TICKS_PER_SECOND - ob)
+        if self.coupling is not None:
+            target = min(target, self.pipe_end)
         self.upload_forcing(elapsed, warming)
         target_ticks = seconds_to_ticks(target + ob)
         if target_ticks < self.host_now_ticks:
@@ -488,6 +507,11 @@ class Simulation:
 
     def finish(self):
         """Record the run's totals and close the outputs; returns the last mass-balance row."""
+        if self.coupling is not None:
+            if self.pipe_pending:   # solver_compact.py:986-988
+                self.coupling.end_window(self.engine.take_pipe_volume().astype(np.float64))
+                self.pipe_pending = False
+            self.record['coupling'] = self.coupling.metadata()
         self.record.update(
             steps=self.steps, warmup_steps=self.warmup_steps, mass_balance_final=self.budget.last_row,
             dt_seconds={'min': self.dt_stats[0] if self.steps else 0., 'max': self.dt_stats[1],

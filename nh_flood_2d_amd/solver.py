@@ -35,6 +35,8 @@ from .simulation import RunSettings, Simulation, velocities
 CHECKPOINT_INTERVAL_S = 30 * 60
 # Engine constructor used by _solve; None selects the HIP engine. Tests substitute the oracle.
 ENGINE_FACTORY = None
+# pipe model used by coupled runs: (pipe_cfg, topology, begin, end) -> actor; None = SWMM (pyswmm)
+PIPE_ACTOR_FACTORY = None
 STATS_FIELDS = ['time', 'elapsed_seconds', 'steps', 'standard_dt_min_s', 'standard_dt_max_s',
                 'standard_dt_mean_s', 'standard_steps', 'event_short_steps', 'event_dt_min_s',
                 'event_dt_sum_s', 'cfl_min_s', 'limited_cell_steps', 'limited_face_steps',
@@ -253,6 +255,9 @@ def run_simulation(domain_cfg: DomainConfig, force_cfg: ForceConfig, pipe_cfg=No
     with RunRecord(domain_cfg, force_cfg, pipe_cfg) as record:
         record.hash_inputs(domain_cfg, force_cfg)
         preprocess(domain_cfg, force_cfg)
+        if pipe_cfg is not None:
+            from .coupling import prepare_pipe
+            prepare_pipe(pipe_cfg, load_preprocessed(domain_cfg))
         return _solve(domain_cfg, force_cfg, record, pipe_cfg)
 
 
@@ -283,8 +288,6 @@ def _device_name(factory) -> str:
 
 def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, pipe_cfg=None,
            end_time=None):
-    if pipe_cfg is not None:
-        raise NotImplementedError('SWMM pipe coupling stays on the host and is not part of this build')
     if force_cfg.gate is not None and force_cfg.gate.get('enabled', False):
         raise NotImplementedError('gate control is not part of this build')
     end_time = float(domain_cfg.duration if end_time is None else end_time)
@@ -333,9 +336,20 @@ def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, 
     outputs = FileOutputs(domain_cfg, record, mesh, forcing, identity, boundary_identity, stations,
                           np.asarray(initial_bed, np.float32))
     started = perf_counter()
+    coupling = None
     try:
+        if pipe_cfg is not None:
+            if domain_cfg.start.microsecond:
+                raise ValueError('pipe coupling requires domain.start at a whole second')
+            from .coupling import PipeCoupling, PipeTopology
+            topology = PipeTopology.load(pipe_cfg.topology_file)
+            begin = domain_cfg.start - timedelta(seconds=domain_cfg.warmup_duration)
+            end = domain_cfg.start + timedelta(seconds=domain_cfg.duration)
+            actor = (PIPE_ACTOR_FACTORY or _swmm_actor)(pipe_cfg, topology, begin, end)
+            coupling = PipeCoupling(topology, pipe_cfg.coupling_interval, float(domain_cfg.warmup_duration),
+                                    float(domain_cfg.duration), actor, domain_cfg.start.isoformat())
         sim = Simulation(mesh, domain_cfg.land_parameters, forcing, settings, checkpoint=state,
-                         engine_factory=factory, outputs=outputs, record=record.data)
+                         engine_factory=factory, outputs=outputs, record=record.data, coupling=coupling)
         record.data['budget_accumulator']['interval_batches'] = settings.batch_size
         setup = perf_counter() - started
         result = sim.run()
@@ -349,5 +363,12 @@ def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, 
                                                 hanging_edges=sim.layout.n_hang, gradient_cells=sim.layout.n_grad)
         return result
     finally:
+        if coupling is not None:
+            coupling.close()
         outputs.close()
         record.write()
+
+
+def _swmm_actor(pipe_cfg, topology, begin, end):
+    from .coupling import SwmmActor
+    return SwmmActor(pipe_cfg, topology, begin, end)

@@ -518,7 +518,7 @@ def scenario_variants(base: Scenario, count: int, seed: int = 0, peak_range=(20.
 
 
 def simulation(sc: Scenario, engine_factory=None, end_time: float | None = None, yield_step: float = 300.,
-               batch_size: int = 32, base_layout=None, outputs=None):
+               batch_size: int = 32, base_layout=None, outputs=None, coupling=None):
     """A Simulation of a synthetic scenario (no files); `base_layout` reuses another scenario's
     layout of the same mesh."""
     from .simulation import RunSettings, Simulation
@@ -526,4 +526,5 @@ def simulation(sc: Scenario, engine_factory=None, end_time: float | None = None,
                            end_time=sc.duration if end_time is None else end_time,
                            yield_step=yield_step, batch_size=batch_size)
     return Simulation(sc.mesh, sc.land_parameters, scenario_forcing(sc), settings, engine_factory=engine_factory,
-                      initial_depth=sc.initial_depth, base_layout=base_layout, outputs=outputs)
+                      initial_depth=sc.initial_depth, base_layout=base_layout, outputs=outputs,
+                      coupling=coupling)

@@ -62,6 +62,16 @@ class HostDevice:
     def write(buf, values: np.ndarray) -> None:
         buf[...] = np.ascontiguousarray(values).view(buf.dtype)
 
+    def upload(self, buf, values: np.ndarray) -> None:
+        self.write(buf, values)
+
+    def download(self, buf, dtype) -> np.ndarray:
+        return self.read(buf, dtype)
+
+    @staticmethod
+    def zero(buf) -> None:
+        buf[...] = 0
+
     @staticmethod
     def ptr(buf) -> int:
         return buf.ctypes.data

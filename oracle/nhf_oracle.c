@@ -690,6 +690,13 @@ static void tick(nhf_ctx *c) {
             discharge = rmax(requested, 0) + rmin(requested, 0) * ds;
         }
         c->ssq[ei] = pipe + discharge;
+        if (m->pipe_volume) { /* compensated window volume (:535-539), F32 as the reference's fields */
+            float increment = (float)(pipe * dt) - m->pipe_correction[ei];
+            float before = m->pipe_volume[ei];
+            float total = before + increment;
+            m->pipe_correction[ei] = (total - before) - increment;
+            m->pipe_volume[ei] = total;
+        }
         real rainq = 0;
         if (c->raining) rainq = (real)c->rain_rates[cell_pixel(c, ei)];
         if (rainq > 0) {

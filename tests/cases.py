@@ -145,7 +145,7 @@ def grid(nx: int, ny: int, dx: float, bed=0., land=10) -> Mesh:
 
 
 def run(path: Path, mesh: Mesh, *, duration, rain, boundaries=(), warmup=0, land=None, yield_step=None,
-        sita=1., afa=.5, restart='', stations=None, warmup_only=False, preprocess=True):
+        sita=1., afa=.5, restart='', stations=None, warmup_only=False, preprocess=True, pipe=None):
     """Write the mesh and run the file-based solver; returns (row, manifest, domain config)."""
     from nh_flood_2d_amd.config import DomainConfig, ForceConfig
     from nh_flood_2d_amd import solver
@@ -158,8 +158,11 @@ def run(path: Path, mesh: Mesh, *, duration, rain, boundaries=(), warmup=0, land
     fc = ForceConfig(boundaries=list(boundaries), rainfall=rain)
     if preprocess:
         solver.preprocess(dc, fc)
+    if pipe is not None:
+        from nh_flood_2d_amd.coupling import prepare_pipe
+        prepare_pipe(pipe, solver.load_preprocessed(dc))
     fn = solver.warmup_solver if warmup_only else solver.solver
-    row = fn(dc, fc)
+    row = fn(dc, fc, pipe)
     manifest = json.loads((path / 'out' / 'manifest.json').read_text())
     return row, manifest, dc
 

@@ -486,11 +486,11 @@ def main():
                                                   forcing.rain.rates(forcing.rain.index_at(3600.)).astype(np.float32),
                                                   True), steps=96)
         per_graph = tuning['steps_per_graph']
-        log(f"tuned: CFL grid {tuning['cfl_blocks']}, {per_graph} steps per graph, "
+        log(f"tuned: cell workgroup {tuning['cell_block']}, CFL grid {tuning['cfl_blocks']}, {per_graph} steps per graph, "
             f"{tuning['us_per_step']:.1f} us/step (candidates {tuning['table']})")
     eng = Engine(layout, params, state, device=device)
     if tuning:
-        eng.set_tuning(tuning['cfl_blocks'])
+        eng.set_tuning(tuning['cfl_blocks'], tuning['cell_block'])
     start = seconds_to_ticks(3600.)
     _set_forcing(eng, forcing, 3600.)
     eng.reset(start)

@@ -41,6 +41,8 @@ struct Dev {
     int32_t *cfl_idx;
     double2 *acc;  // per depth-block, per term: (sum, compensation)
     int32_t terms, cfl_blocks, depth_blocks, theta_one, cfl_max;
+    int32_t cell_block;  // workgroup size of the two cell passes (128, 256 or 512)
+    int32_t acc_owners;  // budget accumulators allocated (wavefronts of the largest grid)
 };
 
 // prepare_step (solver_compact.py:364-382), run by one thread.
@@ -89,7 +91,7 @@ __global__ void k_reset(Dev d, uint64_t now) {
         c->min_scale_bits = __float_as_uint(1.0f);
         c->limit_value[0] = INFINITY;
     }
-    for (int i = tid; i < d.depth_blocks * NHF_WAVES * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i < d.acc_owners * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
     for (int i = tid; i < d.m.e_num; i += stride) d.donor[i] = 0.0f;
     for (int i = tid; i < d.m.s_num; i += stride) d.qflux[i] = 0.0f;
     if (d.face_qn)
@@ -119,7 +121,7 @@ __global__ void k_prepare(Dev d, int slot) {
 
 __global__ void k_clear_acc(Dev d) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int i = tid; i < d.depth_blocks * NHF_WAVES * d.terms; i += gridDim.x * blockDim.x)
+    for (int i = tid; i < d.acc_owners * d.terms; i += gridDim.x * blockDim.x)
         d.acc[i] = make_double2(0.0, 0.0);
 }
 
@@ -293,12 +295,13 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
 
 // --------------------------------------------------------------------------- donor scale
 
-__global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
     Ctl *c = d.ctl;
     if (!c->active) return;
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
-    int cell = xcd_block() * NHF_BLOCK + threadIdx.x + 1;
+    int cell = xcd_block() * BLOCK + threadIdx.x + 1;
     uint32_t limited = 0, faces = 0, scale_bits = __float_as_uint(1.0f);
     if (cell < m.e_num) {
         int bi = meta_boundary(m.cell_meta[cell]);
@@ -400,13 +403,14 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_donor(Dev d) {
 
 // --------------------------------------------------------------------------- (b) depth update
 
-__global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     Ctl *c = d.ctl;
     if (!c->active) return;
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
     const int b_num = m.b_num;
-    int cell = xcd_block() * NHF_BLOCK + threadIdx.x + 1;
+    int cell = xcd_block() * BLOCK + threadIdx.x + 1;
     bool live = cell < m.e_num;
     // per-cell budget contributions (solver_compact.py:533, 578-590)
     float t_infil = 0.0f, t_floor = 0.0f, t_level_floor = 0.0f, t_pipe = 0.0f, t_disc = 0.0f;
@@ -536,7 +540,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_depth(Dev d) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
     // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it.
-    double2 *acc = d.acc + ((size_t)blockIdx.x * NHF_WAVES + wave) * d.terms;
+    double2 *acc = d.acc + ((size_t)blockIdx.x * (BLOCK / 64) + wave) * d.terms;
     for (int t = 0; t < d.terms; ++t) {
         float v = 0.0f;
         if (t == 0)
@@ -698,6 +702,23 @@ static int cfl_grid(int s_num) {
     return n < 2048 ? n : 2048;
 }
 
+// Grid of a cell pass with workgroups of `block` threads, a multiple of the 8 XCDs.
+static int cell_grid(int e_num, int block) {
+    int n = e_num - 1 > 0 ? e_num - 1 : 1;
+    return ((n + block - 1) / block + 7) & ~7;
+}
+
+// Budget accumulators: one per wavefront of the cell pass, for the largest of the supported
+// workgroup sizes' grids, so a change of workgroup size keeps every accumulated term.
+static int acc_owner_count(int e_num) {
+    int most = 0;
+    for (int b = 128; b <= 512; b *= 2) {
+        int owners = cell_grid(e_num, b) * (b / 64);
+        most = owners > most ? owners : most;
+    }
+    return most;
+}
+
 static Carve carve(const nhf_mesh *m, const nhf_params *p) {
     Carve c;
     size_t off = 0;
@@ -717,7 +738,7 @@ static Carve carve(const nhf_mesh *m, const nhf_params *p) {
     c.grad = off; off = align_up(off + sizeof(float) * 2 * (m->n_grad > 0 ? m->n_grad : 1));
     c.key = off; off = align_up(off + sizeof(uint64_t) * cfl_blocks);
     c.idx = off; off = align_up(off + sizeof(int32_t) * cfl_blocks);
-    c.acc = off; off = align_up(off + sizeof(double2) * (size_t)depth_blocks * NHF_WAVES * terms);
+    c.acc = off; off = align_up(off + sizeof(double2) * (size_t)acc_owner_count(m->e_num) * terms);
     c.total = off;
     return c;
 }
@@ -772,7 +793,9 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     d.terms = 4 + 6 * mesh->b_num;
     d.cfl_blocks = cfl_grid(mesh->s_num);
     d.cfl_max = cfl_max_blocks(mesh->s_num);
-    d.depth_blocks = blocks_xcd(mesh->e_num - 1);
+    d.cell_block = NHF_BLOCK;
+    d.depth_blocks = cell_grid(mesh->e_num, d.cell_block);
+    d.acc_owners = acc_owner_count(mesh->e_num);
     d.theta_one = params->theta == 1.0f;
     ctx->stream = (hipStream_t)stream;
     ctx->graph_enabled = 0;
@@ -873,8 +896,16 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
         if (d.m.n_grad > 0) LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
         if (!d.theta_one) LAUNCH(K_FACE, k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
         if (sides > 0) LAUNCH(K_FLUX, k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d);
-        LAUNCH(K_DONOR, k_donor, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
-        LAUNCH(K_DEPTH, k_depth, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
+        if (d.cell_block == 128) {
+            LAUNCH(K_DONOR, k_donor<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
+            LAUNCH(K_DEPTH, k_depth<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
+        } else if (d.cell_block == 512) {
+            LAUNCH(K_DONOR, k_donor<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
+            LAUNCH(K_DEPTH, k_depth<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
+        } else {
+            LAUNCH(K_DONOR, k_donor<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
+            LAUNCH(K_DEPTH, k_depth<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
+        }
         LAUNCH(K_CFL, k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
         LAUNCH(K_FINISH, k_finish, dim3(1), dim3(1024), 0, s, d, 1, slot, steps);
     }
@@ -952,8 +983,22 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
             case K_GRAD: hipLaunchKernelGGL(k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_FACE: hipLaunchKernelGGL(k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_FLUX: hipLaunchKernelGGL(k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d); break;
-            case K_DONOR: hipLaunchKernelGGL(k_donor, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d); break;
-            case K_DEPTH: hipLaunchKernelGGL(k_depth, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_DONOR:
+                if (d.cell_block == 128)
+                    hipLaunchKernelGGL(k_donor<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
+                else if (d.cell_block == 512)
+                    hipLaunchKernelGGL(k_donor<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
+                else
+                    hipLaunchKernelGGL(k_donor<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
+                break;
+            case K_DEPTH:
+                if (d.cell_block == 128)
+                    hipLaunchKernelGGL(k_depth<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
+                else if (d.cell_block == 512)
+                    hipLaunchKernelGGL(k_depth<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
+                else
+                    hipLaunchKernelGGL(k_depth<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
+                break;
             case K_CFL: hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1); break;
             case K_FINISH: hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, d, 0, 0, 0); break;
             }
@@ -978,7 +1023,7 @@ int32_t nhf_compute_cfl(nhf_ctx *ctx) {
 
 int32_t nhf_flush_budget(nhf_ctx *ctx, double *out) {
     const Dev &d = ctx->d;
-    const int owners = d.depth_blocks * NHF_WAVES;
+    const int owners = d.acc_owners;
     size_t n = (size_t)owners * d.terms;
     if (!ctx->host_acc) HIP_TRY(hipHostMalloc((void **)&ctx->host_acc, sizeof(double2) * n, hipHostMallocDefault));
     HIP_TRY(hipMemcpyAsync(ctx->host_acc, d.acc, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -1036,11 +1081,13 @@ int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable) {
     return 0;
 }
 
-int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks) {
+int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block) {
     if (cfl_blocks < 8 || cfl_blocks > ctx->d.cfl_max || (cfl_blocks & 7))
         return set_error_text("nhf_set_tuning: CFL grid out of range or not a multiple of 8");
+    if (cell_block != 128 && cell_block != 256 && cell_block != 512)
+        return set_error_text("nhf_set_tuning: cell workgroup size must be 128, 256 or 512");
     if (ctx->pending_steps) return set_error_text("nhf_set_tuning: a batch is in flight");
-    if (cfl_blocks != ctx->d.cfl_blocks) {
+    if (cfl_blocks != ctx->d.cfl_blocks || cell_block != ctx->d.cell_block) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i <= NHF_MAX_BATCH; ++i)
             if (ctx->exec[i]) {
@@ -1048,13 +1095,17 @@ int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks) {
                 ctx->exec[i] = nullptr;
             }
         ctx->d.cfl_blocks = cfl_blocks;
+        ctx->d.cell_block = cell_block;
+        ctx->d.depth_blocks = cell_grid(ctx->d.m.e_num, cell_block);
     }
     return 0;
 }
 
-int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *cfl_blocks, int32_t *cfl_max) {
-    if (cfl_blocks) *cfl_blocks = ctx->d.cfl_blocks;
-    if (cfl_max) *cfl_max = ctx->d.cfl_max;
+int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *out4) {
+    out4[0] = ctx->d.cfl_blocks;
+    out4[1] = ctx->d.cfl_max;
+    out4[2] = ctx->d.cell_block;
+    out4[3] = 0;
     return 0;
 }
 

@@ -256,13 +256,14 @@ class Engine:
         abi.check(self.lib, self.lib.nhf_time_kernels(self.ctx, int(repeats), us.ctypes.data), 'nhf_time_kernels')
         return us
 
-    def set_tuning(self, cfl_blocks: int) -> None:
-        abi.check(self.lib, self.lib.nhf_set_tuning(self.ctx, int(cfl_blocks)), 'nhf_set_tuning')
+    def set_tuning(self, cfl_blocks: int, cell_block: int = 256) -> None:
+        rc = self.lib.nhf_set_tuning(self.ctx, int(cfl_blocks), int(cell_block))
+        abi.check(self.lib, rc, 'nhf_set_tuning')
 
     def tuning_range(self) -> dict:
-        cur, top = ctypes.c_int32(), ctypes.c_int32()
-        self.lib.nhf_tuning_range(self.ctx, ctypes.byref(cur), ctypes.byref(top))
-        return dict(cfl_blocks=cur.value, cfl_max=top.value)
+        out = np.zeros(4, dtype=np.int32)
+        self.lib.nhf_tuning_range(self.ctx, out.ctypes.data)
+        return dict(cfl_blocks=int(out[0]), cfl_max=int(out[1]), cell_block=int(out[2]))
 
     def kernel_times(self, reset: bool = True):
         ms = np.zeros(8, dtype=np.float64)

@@ -150,3 +150,26 @@ def test_graph_replays_match_eager(gpu_device):
         for name in ('depth', 'q', 'max_depth', 'capacity'):
             a, b = eager.gpu.read_state(name), graph.gpu.read_state(name)
             np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=f'{name} replay {k}')
+
+
+@pytest.mark.parametrize('cfl_blocks,cell_block', [(512, 128), (4096, 512)])
+def test_launch_tuning_keeps_the_bits(gpu_device, cfl_blocks, cell_block):
+    """Every workgroup size and CFL grid the autotuner may pick computes the same state, trace
+    and budget as the default launch shapes."""
+    from nh_flood_2d_amd.engine import Engine
+    sc = synth.small_multires(9, pond=0.3, discharge=1.0)
+    _say(f'[tuning] small_multires seed 9, CFL grid {cfl_blocks}, cell workgroup {cell_block}')
+    runs = []
+    for tuned in (False, True):
+        pair = Pair(sc, gpu_device)
+        if tuned:
+            top = pair.gpu.tuning_range()['cfl_max']
+            pair.gpu.set_tuning(min(cfl_blocks, top - top % 8), cell_block)
+        rows = [pair.fixed_step(0.05)[0] for _ in range(12)]
+        runs.append((np.array(rows), pair.gpu.read_state('depth'), pair.gpu.read_state('q'),
+                     pair.gpu.flush_budget()))
+    (ra, da, qa, ba), (rb, db, qb, bb) = runs
+    np.testing.assert_array_equal(ra, rb)
+    np.testing.assert_array_equal(da, db)
+    np.testing.assert_array_equal(qa, qb)
+    np.testing.assert_allclose(ba.sum(axis=1), bb.sum(axis=1), rtol=1e-12, atol=1e-12)

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 4
+#define NHF_ABI_VERSION 5
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -110,6 +110,9 @@ typedef struct nhf_mesh {
        exchanged with the pipe network in the current coupling window, summed in F32 with a
        compensation term; the host reads and clears both at each window end */
     float *pipe_volume, *pipe_correction;
+    /* [e_num*4] the cell passes' view of the side lists: a cell with one side per face holds its
+       left, right, bottom and top side ids; any other cell holds (-1, csr_start, csr_count, 0) */
+    const int32_t *cell_quad;
 } nhf_mesh;
 
 typedef struct nhf_ctx nhf_ctx;

@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -38,6 +38,7 @@ MESH_FIELDS = [
     ('grad_cell', 'p'), ('grad_axes', 'p'),
     ('kind', 'p'),
     ('pipe_volume', 'p'), ('pipe_correction', 'p'),
+    ('cell_quad', 'p'),
 ]
 
 

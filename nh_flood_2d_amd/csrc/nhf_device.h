@@ -48,7 +48,8 @@ static inline const char *missing_mesh_array(const nhf_mesh *m, bool state, bool
                   {m->class_horton, "class_horton"}, {m->bweight, "bweight"}, {m->csr_start, "csr_start"},
                   {m->csr_count, "csr_count"}, {m->isl, "isl"}, {m->isl_cell, "isl_cell"},
                   {m->side_cells, "side_cells"}, {m->side_meta, "side_meta"}, {m->length, "length"},
-                  {m->sdc, "sdc"}, {m->side_key, "side_key"}, {m->kind, "kind"}};
+                  {m->sdc, "sdc"}, {m->side_key, "side_key"}, {m->kind, "kind"},
+                  {m->cell_quad, "cell_quad"}};
     for (const Field &f : fields)
         if (!f.p) return f.name;
     if (m->n_pixels > 1 && !m->rain_index) return "rain_index";
@@ -70,9 +71,51 @@ static inline const char *missing_mesh_array(const nhf_mesh *m, bool state, bool
 // load chain per entry; longer lists take a plain loop.
 #define NHF_LIST_MAX 8
 
+// Side count of face d (0 left, 1 right, 2 bottom, 3 top) from a cell's packed face counts.
+static __device__ __forceinline__ int face_n(uint32_t counts, int d) { return (int)((counts >> (8 * d)) & 255u); }
+
 static __device__ __forceinline__ void load_list(const int32_t *isl, int k0, int total, int ids[NHF_LIST_MAX]) {
 #pragma unroll
     for (int j = 0; j < NHF_LIST_MAX; ++j) ids[j] = j < total ? isl[k0 + j] : 0;
+}
+
+// A cell's side list as the cell passes read it: one 16-byte record per cell gives the four side
+// ids of a cell with one side per face, which needs no further index load; any other cell's record
+// points into the packed lists. `total` entries (at most NHF_LIST_MAX on the short path) and the
+// running face counts come back; `short_list` is false for a list that needs the plain loop.
+struct CellList {
+    int ids[NHF_LIST_MAX];
+    int total, n0, n01, n012, k0;
+    uint32_t cnt;
+    bool short_list;
+};
+
+static __device__ __forceinline__ void cell_list(const nhf_mesh &m, int cell, CellList &l) {
+    const int4 rec = reinterpret_cast<const int4 *>(m.cell_quad)[cell];
+    if (rec.x >= 0) {
+        l.ids[0] = rec.x;
+        l.ids[1] = rec.y;
+        l.ids[2] = rec.z;
+        l.ids[3] = rec.w;
+#pragma unroll
+        for (int j = 4; j < NHF_LIST_MAX; ++j) l.ids[j] = 0;
+        l.total = 4;
+        l.n0 = 1;
+        l.n01 = 2;
+        l.n012 = 3;
+        l.k0 = -1;
+        l.cnt = 0x01010101u;
+        l.short_list = true;
+        return;
+    }
+    l.k0 = rec.y;
+    l.cnt = (uint32_t)rec.z;
+    l.n0 = face_n(l.cnt, 0);
+    l.n01 = l.n0 + face_n(l.cnt, 1);
+    l.n012 = l.n01 + face_n(l.cnt, 2);
+    l.total = l.n012 + face_n(l.cnt, 3);
+    l.short_list = l.total <= NHF_LIST_MAX;
+    if (l.short_list) load_list(m.isl, l.k0, l.total, l.ids);
 }
 
 // Face (0 left, 1 right, 2 bottom, 3 top) of list entry j, from the running face counts.
@@ -96,8 +139,6 @@ struct Ctl {
 // Comparisons written as selects so +0/-0 and NaN behave the same in every pass.
 static __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
 static __device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
-
-static __device__ __forceinline__ int face_n(uint32_t counts, int d) { return (int)((counts >> (8 * d)) & 255u); }
 
 // Workgroups are dealt to the 8 XCDs round-robin. For a grid that is a multiple of 8, this maps
 // the hardware order so XCD x runs the contiguous x-th eighth of the logical blocks: Morton-near

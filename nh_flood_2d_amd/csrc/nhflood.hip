@@ -318,16 +318,15 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
         float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
         float available = m.depth[cell] * area + positive + rainq * dt * area;
         float outgoing = (fmx(-pq, 0.0f) + fmx(-discharge, 0.0f)) * dt;
-        const uint32_t cnt = m.csr_count[cell];
-        const int k0 = m.csr_start[cell];
-        const int n0 = face_n(cnt, 0), n01 = n0 + face_n(cnt, 1), n012 = n01 + face_n(cnt, 2);
-        const int total = n012 + face_n(cnt, 3);
-        const bool short_list = total <= NHF_LIST_MAX;
-        int ids[NHF_LIST_MAX];
+        CellList l;
+        cell_list(m, cell, l);
+        const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
+        const uint32_t cnt = l.cnt;
+        const bool short_list = l.short_list;
+        const int *ids = l.ids;
         float qf[NHF_LIST_MAX];
         if (short_list) {
             float len[NHF_LIST_MAX];
-            load_list(m.isl, k0, total, ids);
 #pragma unroll
             for (int j = 0; j < NHF_LIST_MAX; ++j) {
                 qf[j] = j < total ? d.qflux[ids[j]] : 0.0f;
@@ -452,14 +451,13 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
         }
         // face totals of the donor-scaled discharge (stored in q by k_donor and k_flux)
         float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
-        const uint32_t cnt = m.csr_count[cell];
-        const int k0 = m.csr_start[cell];
-        const int n0 = face_n(cnt, 0), n01 = n0 + face_n(cnt, 1), n012 = n01 + face_n(cnt, 2);
-        const int total = n012 + face_n(cnt, 3);
-        if (total <= NHF_LIST_MAX) {
-            int ids[NHF_LIST_MAX];
+        CellList l;
+        cell_list(m, cell, l);
+        const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
+        const uint32_t cnt = l.cnt;
+        if (l.short_list) {
+            const int *ids = l.ids;
             float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
-            load_list(m.isl, k0, total, ids);
 #pragma unroll
             for (int j = 0; j < NHF_LIST_MAX; ++j) {
                 qv[j] = j < total ? m.q[ids[j]] : 0.0f;

@@ -201,6 +201,19 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
         manning=side_n[sperm], side_key=sperm.astype(np.uint32),
         kind=(forcing.kinds.astype(np.int32) if len(forcing.kinds) else np.zeros(1, np.int32)),
     )
+    # one 16-byte record per cell for the cell passes: a cell with exactly one side per face
+    # ("regular", nearly every cell of a 2:1 quadtree) holds its left, right, bottom and top side
+    # ids; any other holds (-1, first list entry, packed face counts, 0)
+    regular = np.all(counts == 1, axis=1)
+    regular[0] = False
+    quad = np.zeros((e_num, 4), dtype=np.int32)
+    quad[:, 0] = -1
+    quad[:, 1] = start
+    quad[:, 2] = packed.view(np.int32)
+    if r_len:
+        first = np.flatnonzero(regular)
+        quad[first] = isl[start[first][:, None] + np.arange(4)[None, :]]
+    arrays['cell_quad'] = quad.ravel()
     layout = Layout(e_num, s_num, cperm, cinv, sperm, sinv, arrays, forcing.n_pixels,
                     len(forcing.kinds), 0, 0, float(theta))
     layout.cell_f0 = f0.astype(np.float32)

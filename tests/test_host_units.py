@@ -341,6 +341,37 @@ def test_station_resolution_and_csv_order(tmp_path):
     assert rows[1].split(',')[2] == 'joint' and rows[2].split(',')[2] == 'right'
 
 
+# ------------------------------------------------------------------------------ device layout
+
+def test_cell_record_names_the_four_sides_of_regular_cells_and_points_into_lists_otherwise():
+    """Checked against the mesh's own face lists through the cell and side permutations."""
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout
+    sc = synth.small_multires(3)
+    mesh = sc.mesh
+    layout = build_layout(mesh, sc.land_parameters, cell_forcing_arrays(mesh, synth.scenario_forcing(sc)), 1.0)
+    quad = layout.arrays['cell_quad'].reshape(-1, 4)
+    isl = layout.arrays['isl']
+    assert quad.shape == (layout.e_num, 4) and quad.dtype == np.int32
+    regular = irregular = 0
+    for new in range(1, layout.e_num):
+        old = layout.cell_perm[new]
+        faces = [mesh.face_sides(old, d) for d in range(4)]
+        if all(len(f) == 1 for f in faces):
+            regular += 1
+            assert list(quad[new]) == [layout.side_inv[f[0]] for f in faces], new
+        else:
+            irregular += 1
+            assert quad[new, 0] == -1 and quad[new, 3] == 0
+            counts = [(int(quad[new, 2]) >> (8 * d)) & 255 for d in range(4)]
+            assert counts == [len(f) for f in faces]
+            listed = isl[quad[new, 1]:quad[new, 1] + sum(counts)]
+            assert list(listed) == [layout.side_inv[s] for f in faces for s in f]
+    assert quad[0, 0] == -1
+    assert regular > 0 and irregular > 0
+
+
 # ------------------------------------------------------------------------------ C ABI
 
 def test_hip_library_loads_and_exports_every_declared_symbol():

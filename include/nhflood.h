@@ -110,8 +110,9 @@ typedef struct nhf_mesh {
        exchanged with the pipe network in the current coupling window, summed in F32 with a
        compensation term; the host reads and clears both at each window end */
     float *pipe_volume, *pipe_correction;
-    /* [e_num*4] the cell passes' view of the side lists: a cell with one side per face holds its
-       left, right, bottom and top side ids; any other cell holds (-1, csr_start, csr_count, 0) */
+    /* [e_num*4] the cell passes' view of the side lists: a cell with one side per face, each of
+       length size[cell], and area size*size (in F32) holds its left, right, bottom and top side
+       ids; any other cell holds (-1, csr_start, csr_count, list length) */
     const int32_t *cell_quad;
 } nhf_mesh;
 

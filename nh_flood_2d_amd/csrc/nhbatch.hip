@@ -333,12 +333,13 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
         level = kind == 0;
     }
     float rainq = d.raining[s] ? d.rain_rates[(size_t)s * d.n_pixels + (d.n_pixels > 1 ? m.rain_index[cell] : 0)] : 0.0f;
-    float area = m.area[cell];
+    CellList l;
+    cell_list(m, cell, l);
+    const float size = l.regular ? m.size[cell] : 0.0f;
+    const float area = l.regular ? size * size : m.area[cell];
     float positive = (fmx(0.0f, 0.0f) + fmx(discharge, 0.0f)) * dt;
     float available = d.depth[IX(cell)] * area + positive + rainq * dt * area;
     float outgoing = (fmx(-0.0f, 0.0f) + fmx(-discharge, 0.0f)) * dt;
-    CellList l;
-    cell_list(m, cell, l);
     const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
     const uint32_t cnt = l.cnt;
     const bool short_list = l.short_list;
@@ -346,11 +347,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     float qf[NHF_LIST_MAX];
     if (short_list) {
         float len[NHF_LIST_MAX];
-#pragma unroll
-        for (int j = 0; j < NHF_LIST_MAX; ++j) {
-            qf[j] = j < total ? d.qflux[IX(ids[j])] : 0.0f;
-            len[j] = j < total ? m.length[ids[j]] : 0.0f;
-        }
+        gather_list(l, [&](int side) { return d.qflux[IX(side)]; }, qf);
+        list_lengths(m, l, size, len);
 #pragma unroll
         for (int j = 0; j < NHF_LIST_MAX; ++j) {
             if (j < total) {
@@ -458,16 +456,14 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
             float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
             CellList l;
             cell_list(m, cell, l);
+            const float reg_size = l.regular ? m.size[cell] : 0.0f;
             const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
             const uint32_t cnt = l.cnt;
             if (l.short_list) {
                 const int *ids = l.ids;
                 float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
-#pragma unroll
-                for (int jj = 0; jj < NHF_LIST_MAX; ++jj) {
-                    qv[jj] = jj < total ? d.q[IX(ids[jj])] : 0.0f;
-                    len[jj] = jj < total ? m.length[ids[jj]] : 0.0f;
-                }
+                gather_list(l, [&](int side) { return d.q[IX(side)]; }, qv);
+                list_lengths(m, l, reg_size, len);
 #pragma unroll
                 for (int jj = 0; jj < NHF_LIST_MAX; ++jj) {
                     if (jj < total) {
@@ -501,7 +497,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
                 qt = fsum[3];
             }
             float tq = ql - qr + qb - qt + ssq;
-            float area = m.area[cell];
+            const float area = l.regular ? reg_size * reg_size : m.area[cell];
             float rain_depth = rainq * dt;
             float increment = tq * dt / area + rain_depth;
             float previous = d.depth[IX(cell)];
@@ -530,7 +526,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
             float hv = fmx(h_new, 0.01f);
             float u = 0.0f, v = 0.0f;
             if (h_new >= d.p.min_h) {
-                const float size = m.size[cell];
+                const float size = l.regular ? reg_size : m.size[cell];
                 u = (ql + qr) / size / hv / 2.0f;
                 v = (qb + qt) / size / hv / 2.0f;
             }

@@ -74,20 +74,25 @@ static inline const char *missing_mesh_array(const nhf_mesh *m, bool state, bool
 // Side count of face d (0 left, 1 right, 2 bottom, 3 top) from a cell's packed face counts.
 static __device__ __forceinline__ int face_n(uint32_t counts, int d) { return (int)((counts >> (8 * d)) & 255u); }
 
+// Every entry is loaded, past the end of the list from its first entry (a real cell lists at least
+// one side), so the loads carry no branch and are issued together.
 static __device__ __forceinline__ void load_list(const int32_t *isl, int k0, int total, int ids[NHF_LIST_MAX]) {
 #pragma unroll
-    for (int j = 0; j < NHF_LIST_MAX; ++j) ids[j] = j < total ? isl[k0 + j] : 0;
+    for (int j = 0; j < NHF_LIST_MAX; ++j) ids[j] = isl[k0 + (j < total ? j : 0)];
 }
 
 // A cell's side list as the cell passes read it: one 16-byte record per cell gives the four side
-// ids of a cell with one side per face, which needs no further index load; any other cell's record
-// points into the packed lists. `total` entries (at most NHF_LIST_MAX on the short path) and the
-// running face counts come back; `short_list` is false for a list that needs the plain loop.
+// ids of a regular cell (one side per face, every side as long as the cell is wide, area equal to
+// size*size in F32, all checked when the record is built), which needs no further index, length
+// or area load; any other cell's record points into the packed lists and holds the list length.
+// Both forms use all four words, so the record is read with one 16-byte load. `total` entries (at most
+// NHF_LIST_MAX on the short path) and the running face counts come back; `short_list` is false for
+// a list that needs the plain loop.
 struct CellList {
     int ids[NHF_LIST_MAX];
     int total, n0, n01, n012, k0;
     uint32_t cnt;
-    bool short_list;
+    bool short_list, regular;
 };
 
 static __device__ __forceinline__ void cell_list(const nhf_mesh &m, int cell, CellList &l) {
@@ -98,7 +103,7 @@ static __device__ __forceinline__ void cell_list(const nhf_mesh &m, int cell, Ce
         l.ids[2] = rec.z;
         l.ids[3] = rec.w;
 #pragma unroll
-        for (int j = 4; j < NHF_LIST_MAX; ++j) l.ids[j] = 0;
+        for (int j = 4; j < NHF_LIST_MAX; ++j) l.ids[j] = rec.x;
         l.total = 4;
         l.n0 = 1;
         l.n01 = 2;
@@ -106,6 +111,7 @@ static __device__ __forceinline__ void cell_list(const nhf_mesh &m, int cell, Ce
         l.k0 = -1;
         l.cnt = 0x01010101u;
         l.short_list = true;
+        l.regular = true;
         return;
     }
     l.k0 = rec.y;
@@ -113,9 +119,37 @@ static __device__ __forceinline__ void cell_list(const nhf_mesh &m, int cell, Ce
     l.n0 = face_n(l.cnt, 0);
     l.n01 = l.n0 + face_n(l.cnt, 1);
     l.n012 = l.n01 + face_n(l.cnt, 2);
-    l.total = l.n012 + face_n(l.cnt, 3);
+    l.total = rec.w;
     l.short_list = l.total <= NHF_LIST_MAX;
+    l.regular = false;
     if (l.short_list) load_list(m.isl, l.k0, l.total, l.ids);
+}
+
+// One value per entry of a short list, load(side) for each. The first four loads carry no
+// condition (entries past the end name the list's first side), the other four are made only when
+// the list is longer than four; entries past the end are not to be used.
+template <class Load>
+static __device__ __forceinline__ void gather_list(const CellList &l, Load load, float out[NHF_LIST_MAX]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[j] = load(l.ids[j]);
+    if (l.total > 4) {
+#pragma unroll
+        for (int j = 4; j < NHF_LIST_MAX; ++j) out[j] = load(l.ids[j]);
+    } else {
+#pragma unroll
+        for (int j = 4; j < NHF_LIST_MAX; ++j) out[j] = 0.0f;
+    }
+}
+
+// Side lengths of a short list: the cell's size for a regular cell, else the stored lengths.
+static __device__ __forceinline__ void list_lengths(const nhf_mesh &m, const CellList &l, float size,
+                                                    float len[NHF_LIST_MAX]) {
+    if (l.regular) {
+#pragma unroll
+        for (int j = 0; j < NHF_LIST_MAX; ++j) len[j] = j < 4 ? size : 0.0f;
+    } else {
+        gather_list(l, [&](int side) { return m.length[side]; }, len);
+    }
 }
 
 // Face (0 left, 1 right, 2 bottom, 3 top) of list entry j, from the running face counts.

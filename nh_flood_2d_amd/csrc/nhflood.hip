@@ -295,43 +295,56 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
 
 // --------------------------------------------------------------------------- donor scale
 
+// The two cell passes are bound by memory latency rather than bandwidth, so their loads come in
+// rounds: first every load that needs the cell index only (issued together, before any of their
+// values is used), then the side gathers and the boundary and rain lookups, which need the first
+// round, and every store last. A store ahead of a load would make the load wait for it, since the
+// arrays may alias as far as the compiler can tell.
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
     Ctl *c = d.ctl;
     if (!c->active) return;
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
+    const bool raining = c->raining;
     int cell = xcd_block() * BLOCK + threadIdx.x + 1;
     uint32_t limited = 0, faces = 0, scale_bits = __float_as_uint(1.0f);
     if (cell < m.e_num) {
-        int bi = meta_boundary(m.cell_meta[cell]);
-        float discharge = 0.0f;
-        bool level = false;
-        if (bi >= 0) {
-            int kind = m.kind[bi];
-            if (kind == 1) discharge = d.values[bi] * m.bweight[cell];
-            level = kind == 0;
-        }
-        float rainq = c->raining ? d.rain_rates[m.n_pixels > 1 ? m.rain_index[cell] : 0] : 0.0f;
-        float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
-        float area = m.area[cell];
-        float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
-        float available = m.depth[cell] * area + positive + rainq * dt * area;
-        float outgoing = (fmx(-pq, 0.0f) + fmx(-discharge, 0.0f)) * dt;
+        // first round: the cell's own values and its side record
+        const uint32_t meta = m.cell_meta[cell];
+        const float h0 = m.depth[cell];
+        const float size = m.size[cell];
+        const uint32_t pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
+        const float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
         CellList l;
         cell_list(m, cell, l);
+        // second round: the sides' discharges, the boundary and the rain rate
         const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
         const uint32_t cnt = l.cnt;
         const bool short_list = l.short_list;
         const int *ids = l.ids;
-        float qf[NHF_LIST_MAX];
+        float qf[NHF_LIST_MAX], len[NHF_LIST_MAX];
         if (short_list) {
-            float len[NHF_LIST_MAX];
-#pragma unroll
-            for (int j = 0; j < NHF_LIST_MAX; ++j) {
-                qf[j] = j < total ? d.qflux[ids[j]] : 0.0f;
-                len[j] = j < total ? m.length[ids[j]] : 0.0f;
-            }
+            gather_list(l, [&](int side) { return d.qflux[side]; }, qf);
+            list_lengths(m, l, size, len);
+        }
+        const int bi = meta_boundary(meta);
+        int kind = -1;
+        float bvalue = 0.0f, bw = 0.0f;
+        if (bi >= 0) {
+            kind = m.kind[bi];
+            bvalue = d.values[bi];
+            bw = m.bweight[cell];
+        }
+        const float rainq = raining ? d.rain_rates[pix] : 0.0f;
+        const float area = l.regular ? size * size : m.area[cell];
+        const float discharge = kind == 1 ? bvalue * bw : 0.0f;
+        const bool level = kind == 0;
+        float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
+        float available = h0 * area + positive + rainq * dt * area;
+        float outgoing = (fmx(-pq, 0.0f) + fmx(-discharge, 0.0f)) * dt;
+        if (short_list) {
 #pragma unroll
             for (int j = 0; j < NHF_LIST_MAX; ++j) {
                 if (j < total) {
@@ -408,6 +421,8 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     if (!c->active) return;
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
+    const bool raining = c->raining;
+    const uint64_t step_end = c->step_end;
     const int b_num = m.b_num;
     int cell = xcd_block() * BLOCK + threadIdx.x + 1;
     bool live = cell < m.e_num;
@@ -416,53 +431,66 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     float t_lv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int bi = -1, kind = -1;
     if (live) {
+        // first round: the cell's own values and its side record
         const uint32_t meta = m.cell_meta[cell];
+        const float previous = m.depth[cell];
+        const float size = m.size[cell];
+        const float cap_stored = m.capacity[cell];
+        const float max_d = m.max_depth[cell], max_s = m.max_speed[cell];
+        const float rt_stored = raining ? m.rain_time[cell] : 0.0f;
+        const uint32_t pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
+        const float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+        float pv_before = 0.0f, pv_correction = 0.0f;
+        if (m.pipe_volume) {
+            pv_correction = m.pipe_correction[cell];
+            pv_before = m.pipe_volume[cell];
+        }
+        CellList l;
+        cell_list(m, cell, l);
+        // second round: the donor-scaled side discharges (stored in q by k_donor and k_flux),
+        // the boundary, the cell's own scale, the rain rate and the Horton parameters
+        const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
+        const uint32_t cnt = l.cnt;
+        float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
+        if (l.short_list) {
+            gather_list(l, [&](int side) { return m.q[side]; }, qv);
+            list_lengths(m, l, size, len);
+        }
         bi = meta_boundary(meta);
-        if (bi >= 0) kind = m.kind[bi];
+        float bvalue = 0.0f, bw = 0.0f, lvalue = 0.0f, bed = 0.0f;
+        if (bi >= 0) {
+            kind = m.kind[bi];
+            bvalue = d.values[bi];
+            bw = m.bweight[cell];
+            lvalue = d.level_values[bi];
+            bed = m.bed[cell];
+        }
         // the cell's own scale, stored by k_donor for boundary and pipe cells only
-        float ds = (bi >= 0 || m.pipe_q) ? d.donor[cell] : 1.0f;
-        float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+        const float ds = (bi >= 0 || m.pipe_q) ? d.donor[cell] : 1.0f;
+        const float rainq = raining ? d.rain_rates[pix] : 0.0f;
+        float f0 = 0.0f, fc = 0.0f, hk = 0.0f;
+        if (raining) {
+            const float *hp = m.class_horton + 3 * meta_class(meta);
+            f0 = hp[0];
+            fc = hp[1];
+            hk = hp[2];
+        }
+        const float area = l.regular ? size * size : m.area[cell];
+
         float pipe = fmx(pq, 0.0f) + fmn(pq, 0.0f) * ds;
         float discharge = 0.0f;
         if (kind == 1) {
-            float requested = d.values[bi] * m.bweight[cell];
+            float requested = bvalue * bw;
             discharge = fmx(requested, 0.0f) + fmn(requested, 0.0f) * ds;
             t_disc = discharge * dt;
         }
         float ssq = pipe + discharge;
-        if (m.pipe_volume) {  // compensated window volume (solver_compact.py:535-539)
-            const float increment = pipe * dt - m.pipe_correction[cell];
-            const float before = m.pipe_volume[cell];
-            const float total = before + increment;
-            m.pipe_correction[cell] = (total - before) - increment;
-            m.pipe_volume[cell] = total;
-        }
-        float rainq = c->raining ? d.rain_rates[m.n_pixels > 1 ? m.rain_index[cell] : 0] : 0.0f;
-        float cap;
-        if (rainq > 0.0f) {
-            float rt = m.rain_time[cell] + dt;
-            m.rain_time[cell] = rt;
-            const float *hp = m.class_horton + 3 * meta_class(meta);
-            float f0 = hp[0], fc = hp[1], hk = hp[2];
-            cap = fc + (f0 - fc) * det_expf(-hk * rt / 3600.0f);
-            m.capacity[cell] = cap;
-        } else {
-            cap = m.capacity[cell];
-        }
-        // face totals of the donor-scaled discharge (stored in q by k_donor and k_flux)
+        // compensated window volume (solver_compact.py:535-539)
+        const float pv_increment = pipe * dt - pv_correction;
+        const float pv_total = pv_before + pv_increment;
+        const float cap = rainq > 0.0f ? fc + (f0 - fc) * det_expf(-hk * (rt_stored + dt) / 3600.0f) : cap_stored;
         float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
-        CellList l;
-        cell_list(m, cell, l);
-        const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
-        const uint32_t cnt = l.cnt;
         if (l.short_list) {
-            const int *ids = l.ids;
-            float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
-#pragma unroll
-            for (int j = 0; j < NHF_LIST_MAX; ++j) {
-                qv[j] = j < total ? m.q[ids[j]] : 0.0f;
-                len[j] = j < total ? m.length[ids[j]] : 0.0f;
-            }
 #pragma unroll
             for (int j = 0; j < NHF_LIST_MAX; ++j) {
                 if (j < total) {
@@ -496,10 +524,8 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
             qt = fsum[3];
         }
         float tq = ql - qr + qb - qt + ssq;
-        float area = m.area[cell];
         float rain_depth = rainq * dt;
         float increment = tq * dt / area + rain_depth;
-        float previous = m.depth[cell];
         float raw = previous + increment;
         float removed = fmn(cap * dt, fmx(raw, 0.0f));
         raw -= removed;
@@ -509,7 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
         float h_new;
         if (kind == 0) {
             t_level_floor = floor_v * area;
-            float target = fmx(d.level_values[bi] - m.bed[cell], 0.0f);
+            float target = fmx(lvalue - bed, 0.0f);
             t_lv[0] = (target - previous) * area;
             t_lv[1] = -(ql - qr + qb - qt) * dt;
             t_lv[2] = -pipe * dt;
@@ -520,19 +546,27 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
             t_floor = floor_v * area;
             h_new = fmx(raw, 0.0f);
         }
-        m.depth[cell] = h_new;
         float hv = fmx(h_new, 0.01f);
         float u = 0.0f, v = 0.0f;
         if (h_new >= d.p.min_h) {
-            const float size = m.size[cell];
             u = (ql + qr) / size / hv / 2.0f;
             v = (qb + qt) / size / hv / 2.0f;
         }
         float speed = sqrtf(u * u + v * v);
-        if (speed > m.max_speed[cell]) m.max_speed[cell] = speed;
-        if (h_new > m.max_depth[cell]) {
+        // stores last
+        if (m.pipe_volume) {
+            m.pipe_correction[cell] = (pv_total - pv_before) - pv_increment;
+            m.pipe_volume[cell] = pv_total;
+        }
+        if (rainq > 0.0f) {
+            m.rain_time[cell] = rt_stored + dt;
+            m.capacity[cell] = cap;
+        }
+        m.depth[cell] = h_new;
+        if (speed > max_s) m.max_speed[cell] = speed;
+        if (h_new > max_d) {
             m.max_depth[cell] = h_new;
-            m.peak_tick[cell] = c->step_end;
+            m.peak_tick[cell] = step_end;
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -131,6 +131,37 @@ def relayout(base: Layout, mesh: Mesh, land_parameters: dict, forcing: Forcing) 
     return layout
 
 
+def cell_records(counts, start, packed, isl, length, size, area) -> np.ndarray:
+    """One 16-byte record per cell for the cell passes (int32 [e_num*4]).
+
+    A regular cell has exactly one side per face (nearly every cell of a 2:1 quadtree: a face
+    with a finer neighbour has two sides) and each of its side lengths equals its size, and its
+    area equals size*size, all in F32: its record holds its left, right, bottom and top side ids,
+    and the kernels take the lengths and area from the size. Any other cell holds (-1, first list
+    entry, packed face counts, list length) and reads its list, lengths and area as they are
+    stored. Every real cell must list a side (the kernels read a list's first entry in place of
+    the entries past its end).
+    """
+    e_num = len(counts)
+    totals = counts.sum(axis=1)
+    if e_num > 1 and np.any(totals[1:] == 0):
+        raise ValueError('a cell lists no side')
+    regular = np.all(counts == 1, axis=1)
+    regular[0] = False
+    quad = np.zeros((e_num, 4), dtype=np.int32)
+    quad[:, 0] = -1
+    quad[:, 1] = start
+    quad[:, 2] = packed.view(np.int32)
+    quad[:, 3] = totals
+    cand = np.flatnonzero(regular)
+    if len(cand):
+        ids = isl[start[cand][:, None] + np.arange(4)[None, :]]
+        s = size[cand].astype(np.float32)
+        same = np.all(length[ids] == s[:, None], axis=1) & (s * s == area[cand])
+        quad[cand[same]] = ids[same]
+    return quad.ravel()
+
+
 def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: float,
                  order: str = 'morton') -> Layout:
     """Derive every static kernel array from the mesh, land table and forcing geometry."""
@@ -201,19 +232,8 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
         manning=side_n[sperm], side_key=sperm.astype(np.uint32),
         kind=(forcing.kinds.astype(np.int32) if len(forcing.kinds) else np.zeros(1, np.int32)),
     )
-    # one 16-byte record per cell for the cell passes: a cell with exactly one side per face
-    # ("regular", nearly every cell of a 2:1 quadtree) holds its left, right, bottom and top side
-    # ids; any other holds (-1, first list entry, packed face counts, 0)
-    regular = np.all(counts == 1, axis=1)
-    regular[0] = False
-    quad = np.zeros((e_num, 4), dtype=np.int32)
-    quad[:, 0] = -1
-    quad[:, 1] = start
-    quad[:, 2] = packed.view(np.int32)
-    if r_len:
-        first = np.flatnonzero(regular)
-        quad[first] = isl[start[first][:, None] + np.arange(4)[None, :]]
-    arrays['cell_quad'] = quad.ravel()
+    arrays['cell_quad'] = cell_records(counts, start, packed, isl, arrays['length'], arrays['size'],
+                                       arrays['area'])
     layout = Layout(e_num, s_num, cperm, cinv, sperm, sinv, arrays, forcing.n_pixels,
                     len(forcing.kinds), 0, 0, float(theta))
     layout.cell_f0 = f0.astype(np.float32)

@@ -354,18 +354,21 @@ def test_cell_record_names_the_four_sides_of_regular_cells_and_points_into_lists
     quad = layout.arrays['cell_quad'].reshape(-1, 4)
     isl = layout.arrays['isl']
     assert quad.shape == (layout.e_num, 4) and quad.dtype == np.int32
+    _, widths, _, area = mesh.geometry()
     regular = irregular = 0
     for new in range(1, layout.e_num):
         old = layout.cell_perm[new]
         faces = [mesh.face_sides(old, d) for d in range(4)]
-        if all(len(f) == 1 for f in faces):
+        size = np.float32(widths[old])
+        if (all(len(f) == 1 for f in faces) and all(np.float32(mesh.length[f[0]]) == size for f in faces)
+                and size * size == np.float32(area[old])):
             regular += 1
             assert list(quad[new]) == [layout.side_inv[f[0]] for f in faces], new
         else:
             irregular += 1
-            assert quad[new, 0] == -1 and quad[new, 3] == 0
             counts = [(int(quad[new, 2]) >> (8 * d)) & 255 for d in range(4)]
             assert counts == [len(f) for f in faces]
+            assert quad[new, 0] == -1 and quad[new, 3] == sum(counts) > 0
             listed = isl[quad[new, 1]:quad[new, 1] + sum(counts)]
             assert list(listed) == [layout.side_inv[s] for f in faces for s in f]
     assert quad[0, 0] == -1

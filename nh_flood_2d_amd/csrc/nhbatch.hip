@@ -334,7 +334,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     }
     float rainq = d.raining[s] ? d.rain_rates[(size_t)s * d.n_pixels + (d.n_pixels > 1 ? m.rain_index[cell] : 0)] : 0.0f;
     CellList l;
-    cell_list(m, cell, l);
+    cell_list(m, cell_record(m, cell), l);
     const float size = l.regular ? m.size[cell] : 0.0f;
     const float area = l.regular ? size * size : m.area[cell];
     float positive = (fmx(0.0f, 0.0f) + fmx(discharge, 0.0f)) * dt;
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
             }
             float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
             CellList l;
-            cell_list(m, cell, l);
+            cell_list(m, cell_record(m, cell), l);
             const float reg_size = l.regular ? m.size[cell] : 0.0f;
             const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
             const uint32_t cnt = l.cnt;
@@ -985,9 +985,9 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
         (err = hipHostMalloc((void **)&ctx->host_ctl, sizeof(Ctl) * slots, hipHostMallocDefault)) != hipSuccess ||
         (err = hipHostMalloc((void **)&ctx->host_flush, sizeof(double2) * d.terms, hipHostMallocDefault)) !=
             hipSuccess) {
-        if (ctx->stage_host) hipHostFree(ctx->stage_host);
-        if (ctx->stage_dev) hipFree(ctx->stage_dev);
-        if (ctx->host_ctl) hipHostFree(ctx->host_ctl);
+        if (ctx->stage_host) (void)hipHostFree(ctx->stage_host);
+        if (ctx->stage_dev) (void)hipFree(ctx->stage_dev);
+        if (ctx->host_ctl) (void)hipHostFree(ctx->host_ctl);
         delete ctx;
         return set_error("nhb_create: staging allocation", err);
     }
@@ -995,10 +995,10 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     ctx->pending_steps = ctx->stage_inflight = 0;
     if ((err = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming)) != hipSuccess ||
         (err = hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming)) != hipSuccess) {
-        hipHostFree(ctx->stage_host);
-        hipFree(ctx->stage_dev);
-        hipHostFree(ctx->host_ctl);
-        hipHostFree(ctx->host_flush);
+        (void)hipHostFree(ctx->stage_host);
+        (void)hipFree(ctx->stage_dev);
+        (void)hipHostFree(ctx->host_ctl);
+        (void)hipHostFree(ctx->host_flush);
         delete ctx;
         return set_error("nhb_create: events", err);
     }
@@ -1008,10 +1008,10 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     for (int s = 0; s < slots; ++s)
         hipLaunchKernelGGL(kb_reset, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, s, (uint64_t)0);
     if ((err = hipGetLastError()) != hipSuccess) {
-        hipHostFree(ctx->stage_host);
-        hipFree(ctx->stage_dev);
-        hipHostFree(ctx->host_ctl);
-        hipHostFree(ctx->host_flush);
+        (void)hipHostFree(ctx->stage_host);
+        (void)hipFree(ctx->stage_dev);
+        (void)hipHostFree(ctx->host_ctl);
+        (void)hipHostFree(ctx->host_flush);
         delete ctx;
         return set_error("nhb_create: reset", err);
     }
@@ -1021,16 +1021,16 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
 
 int32_t nhb_destroy(nhb_ctx *ctx) {
     if (!ctx) return 0;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i <= NHF_MAX_BATCH; ++i)
-        if (ctx->exec[i]) hipGraphExecDestroy(ctx->exec[i]);
-    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
-    hipEventDestroy(ctx->done);
-    hipEventDestroy(ctx->stage_done);
-    hipHostFree(ctx->stage_host);
-    hipFree(ctx->stage_dev);
-    hipHostFree(ctx->host_ctl);
-    hipHostFree(ctx->host_flush);
+        if (ctx->exec[i]) (void)hipGraphExecDestroy(ctx->exec[i]);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ctx->done);
+    (void)hipEventDestroy(ctx->stage_done);
+    (void)hipHostFree(ctx->stage_host);
+    (void)hipFree(ctx->stage_dev);
+    (void)hipHostFree(ctx->host_ctl);
+    (void)hipHostFree(ctx->host_flush);
     delete ctx;
     return 0;
 }
@@ -1118,7 +1118,7 @@ int32_t nhb_launch_batch(nhb_ctx *ctx, int32_t steps) {
             if (rc) return rc;
             if (end != hipSuccess) return set_error("hipStreamEndCapture", end);
             hipError_t inst = hipGraphInstantiate(&ctx->exec[steps], graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             if (inst != hipSuccess) return set_error("hipGraphInstantiate", inst);
         }
         HIP_TRY(hipGraphLaunch(ctx->exec[steps], s));
@@ -1232,7 +1232,7 @@ int32_t nhb_set_tuning(nhb_ctx *ctx, int32_t depth_blocks, int32_t cfl_blocks) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i <= NHF_MAX_BATCH; ++i)
             if (ctx->exec[i]) {
-                hipGraphExecDestroy(ctx->exec[i]);
+                (void)hipGraphExecDestroy(ctx->exec[i]);
                 ctx->exec[i] = nullptr;
             }
         d.depth_blocks = depth_blocks;

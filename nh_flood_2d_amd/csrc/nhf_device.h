@@ -95,9 +95,14 @@ struct CellList {
     bool short_list, regular;
 };
 
-static __device__ __forceinline__ void cell_list(const nhf_mesh &m, int cell, CellList &l) {
-    const int4 rec = reinterpret_cast<const int4 *>(m.cell_quad)[cell];
-    if (rec.x >= 0) {
+static __device__ __forceinline__ int4 cell_record(const nhf_mesh &m, int cell) {
+    return reinterpret_cast<const int4 *>(m.cell_quad)[cell];
+}
+
+// REGULAR: the caller knows the record is a regular cell's, so no list code is generated.
+template <bool REGULAR = false>
+static __device__ __forceinline__ void cell_list(const nhf_mesh &m, const int4 rec, CellList &l) {
+    if (REGULAR || rec.x >= 0) {
         l.ids[0] = rec.x;
         l.ids[1] = rec.y;
         l.ids[2] = rec.z;

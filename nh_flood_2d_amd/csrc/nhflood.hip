@@ -296,10 +296,109 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
 // --------------------------------------------------------------------------- donor scale
 
 // The two cell passes are bound by memory latency rather than bandwidth, so their loads come in
-// rounds: first every load that needs the cell index only (issued together, before any of their
-// values is used), then the side gathers and the boundary and rain lookups, which need the first
-// round, and every store last. A store ahead of a load would make the load wait for it, since the
-// arrays may alias as far as the compiler can tell.
+// rounds: first every load that needs the cell index only, the side record included (issued
+// together, before any of their values is used), then the side gathers and the boundary and rain
+// lookups, which need the first round, and every store last (a store ahead of a load would make
+// the load wait for it, since the arrays may alias as far as the compiler can tell). A wavefront
+// whose cells are all regular (nearly all of them) runs a body with no list code, so nothing in
+// it waits on a load it does not need.
+
+struct DonorCell {
+    uint32_t meta, pix;
+    float h, size, pq;
+    int4 rec;
+};
+
+template <bool REGULAR>
+static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float dt, bool raining, const DonorCell &v,
+                                                  uint32_t &limited, uint32_t &faces, uint32_t &scale_bits) {
+    const nhf_mesh &m = d.m;
+    CellList l;
+    cell_list<REGULAR>(m, v.rec, l);
+    // second round: the sides' discharges, the boundary and the rain rate
+    const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
+    const uint32_t cnt = l.cnt;
+    const bool short_list = l.short_list;
+    const int *ids = l.ids;
+    float qf[NHF_LIST_MAX], len[NHF_LIST_MAX];
+    if (short_list) {
+        gather_list(l, [&](int side) { return d.qflux[side]; }, qf);
+        list_lengths(m, l, v.size, len);
+    }
+    const int bi = meta_boundary(v.meta);
+    int kind = -1;
+    float bvalue = 0.0f, bw = 0.0f;
+    if (bi >= 0) {
+        kind = m.kind[bi];
+        bvalue = d.values[bi];
+        bw = m.bweight[cell];
+    }
+    const float rainq = raining ? d.rain_rates[v.pix] : 0.0f;
+    const float area = l.regular ? v.size * v.size : m.area[cell];
+    const float discharge = kind == 1 ? bvalue * bw : 0.0f;
+    const bool level = kind == 0;
+    const float pq = v.pq;
+    float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
+    float available = v.h * area + positive + rainq * dt * area;
+    float outgoing = (fmx(-pq, 0.0f) + fmx(-discharge, 0.0f)) * dt;
+    if (short_list) {
+#pragma unroll
+        for (int j = 0; j < NHF_LIST_MAX; ++j) {
+            if (j < total) {
+                float sq = qf[j] * len[j];
+                if (!(entry_face(j, n0, n01, n012) & 1)) sq = -sq;  // left and bottom faces
+                outgoing += fmx(sq, 0.0f) * dt;
+            }
+        }
+    } else {
+        int k = k0;
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                float sq = d.qflux[side] * m.length[side];
+                if (dir == 0 || dir == 2) sq = -sq;
+                outgoing += fmx(sq, 0.0f) * dt;
+            }
+        }
+    }
+    float ds = 1.0f;
+    if (outgoing > available && !level) {
+        ds = available / outgoing;
+        limited = 1;
+        scale_bits = __float_as_uint(ds);
+    }
+    // Shared-subedge scaling (solver_compact.py:516-524): every side takes the scale of its
+    // donor, the lower cell when q >= 0 and the higher cell otherwise. This cell is the lower
+    // cell of its right/top sides, so it is the donor of a side exactly when
+    // (q >= 0) == (face is right/top), and it stores that side's scaled discharge: one writer
+    // per side.
+    if (short_list) {
+#pragma unroll
+        for (int j = 0; j < NHF_LIST_MAX; ++j) {
+            if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
+                m.q[ids[j]] = qf[j] * ds;
+                if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+            }
+        }
+    } else {
+        int k = k0;
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            const bool upper = dir & 1;
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                float q1 = d.qflux[side];
+                if ((q1 >= 0.0f) == upper) {
+                    m.q[side] = q1 * ds;
+                    if (q1 != 0.0f && ds < 1.0f) faces += 1;
+                }
+            }
+        }
+    }
+    // the depth pass needs the scale itself only for boundary and pipe exchange
+    if (bi >= 0 || m.pipe_q) d.donor[cell] = ds;
+}
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
@@ -308,99 +407,23 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
     const bool raining = c->raining;
-    int cell = xcd_block() * BLOCK + threadIdx.x + 1;
+    const int cell = xcd_block() * BLOCK + threadIdx.x + 1;
+    const bool live = cell < m.e_num;
     uint32_t limited = 0, faces = 0, scale_bits = __float_as_uint(1.0f);
-    if (cell < m.e_num) {
-        // first round: the cell's own values and its side record
-        const uint32_t meta = m.cell_meta[cell];
-        const float h0 = m.depth[cell];
-        const float size = m.size[cell];
-        const uint32_t pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
-        const float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
-        CellList l;
-        cell_list(m, cell, l);
-        // second round: the sides' discharges, the boundary and the rain rate
-        const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
-        const uint32_t cnt = l.cnt;
-        const bool short_list = l.short_list;
-        const int *ids = l.ids;
-        float qf[NHF_LIST_MAX], len[NHF_LIST_MAX];
-        if (short_list) {
-            gather_list(l, [&](int side) { return d.qflux[side]; }, qf);
-            list_lengths(m, l, size, len);
-        }
-        const int bi = meta_boundary(meta);
-        int kind = -1;
-        float bvalue = 0.0f, bw = 0.0f;
-        if (bi >= 0) {
-            kind = m.kind[bi];
-            bvalue = d.values[bi];
-            bw = m.bweight[cell];
-        }
-        const float rainq = raining ? d.rain_rates[pix] : 0.0f;
-        const float area = l.regular ? size * size : m.area[cell];
-        const float discharge = kind == 1 ? bvalue * bw : 0.0f;
-        const bool level = kind == 0;
-        float positive = (fmx(pq, 0.0f) + fmx(discharge, 0.0f)) * dt;
-        float available = h0 * area + positive + rainq * dt * area;
-        float outgoing = (fmx(-pq, 0.0f) + fmx(-discharge, 0.0f)) * dt;
-        if (short_list) {
-#pragma unroll
-            for (int j = 0; j < NHF_LIST_MAX; ++j) {
-                if (j < total) {
-                    float sq = qf[j] * len[j];
-                    if (!(entry_face(j, n0, n01, n012) & 1)) sq = -sq;  // left and bottom faces
-                    outgoing += fmx(sq, 0.0f) * dt;
-                }
-            }
-        } else {
-            int k = k0;
-            for (int dir = 0; dir < 4; ++dir) {
-                int n = face_n(cnt, dir);
-                for (int j = 0; j < n; ++j, ++k) {
-                    int side = m.isl[k];
-                    float sq = d.qflux[side] * m.length[side];
-                    if (dir == 0 || dir == 2) sq = -sq;
-                    outgoing += fmx(sq, 0.0f) * dt;
-                }
-            }
-        }
-        float ds = 1.0f;
-        if (outgoing > available && !level) {
-            ds = available / outgoing;
-            limited = 1;
-            scale_bits = __float_as_uint(ds);
-        }
-        // Shared-subedge scaling (solver_compact.py:516-524): every side takes the scale of its
-        // donor, the lower cell when q >= 0 and the higher cell otherwise. This cell is the
-        // lower cell of its right/top sides, so it is the donor of a side exactly when
-        // (q >= 0) == (face is right/top), and it stores that side's scaled discharge: one
-        // writer per side.
-        if (short_list) {
-#pragma unroll
-            for (int j = 0; j < NHF_LIST_MAX; ++j) {
-                if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
-                    m.q[ids[j]] = qf[j] * ds;
-                    if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
-                }
-            }
-        } else {
-            int k = k0;
-            for (int dir = 0; dir < 4; ++dir) {
-                int n = face_n(cnt, dir);
-                const bool upper = dir & 1;
-                for (int j = 0; j < n; ++j, ++k) {
-                    int side = m.isl[k];
-                    float q1 = d.qflux[side];
-                    if ((q1 >= 0.0f) == upper) {
-                        m.q[side] = q1 * ds;
-                        if (q1 != 0.0f && ds < 1.0f) faces += 1;
-                    }
-                }
-            }
-        }
-        // the depth pass needs the scale itself only for boundary and pipe exchange
-        if (bi >= 0 || m.pipe_q) d.donor[cell] = ds;
+    // first round: the cell's own values and its side record
+    DonorCell v = {};
+    if (live) {
+        v.meta = m.cell_meta[cell];
+        v.h = m.depth[cell];
+        v.size = m.size[cell];
+        v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
+        v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+        v.rec = cell_record(m, cell);
+    }
+    if (__all(!live || v.rec.x >= 0)) {
+        if (live) donor_cell<true>(d, cell, dt, raining, v, limited, faces, scale_bits);
+    } else if (live) {
+        donor_cell<false>(d, cell, dt, raining, v, limited, faces, scale_bits);
     }
     // limiter counters: one integer atomic per wavefront that limited a cell or face (rare)
     if (__any(limited | faces)) {
@@ -415,6 +438,153 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
 
 // --------------------------------------------------------------------------- (b) depth update
 
+struct DepthCell {
+    uint32_t meta, pix;
+    float h, size, cap, max_d, max_s, rain_time, pq, pv_before, pv_correction;
+    int4 rec;
+};
+
+// Per-cell budget contributions (solver_compact.py:533, 578-590).
+struct DepthTerms {
+    float infil, floor, level_floor, pipe, disc;
+    float lv[5];
+    int bi, kind;
+};
+
+template <bool REGULAR>
+static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float dt, bool raining, uint64_t step_end,
+                                                  const DepthCell &v, DepthTerms &t) {
+    const nhf_mesh &m = d.m;
+    CellList l;
+    cell_list<REGULAR>(m, v.rec, l);
+    // second round: the donor-scaled side discharges (stored in q by k_donor and k_flux), the
+    // boundary, the cell's own scale, the rain rate and the Horton parameters
+    const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
+    const uint32_t cnt = l.cnt;
+    float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
+    if (l.short_list) {
+        gather_list(l, [&](int side) { return m.q[side]; }, qv);
+        list_lengths(m, l, v.size, len);
+    }
+    const int bi = meta_boundary(v.meta);
+    int kind = -1;
+    float bvalue = 0.0f, bw = 0.0f, lvalue = 0.0f, bed = 0.0f;
+    if (bi >= 0) {
+        kind = m.kind[bi];
+        bvalue = d.values[bi];
+        bw = m.bweight[cell];
+        lvalue = d.level_values[bi];
+        bed = m.bed[cell];
+    }
+    // the cell's own scale, stored by k_donor for boundary and pipe cells only
+    const float ds = (bi >= 0 || m.pipe_q) ? d.donor[cell] : 1.0f;
+    const float rainq = raining ? d.rain_rates[v.pix] : 0.0f;
+    float f0 = 0.0f, fc = 0.0f, hk = 0.0f;
+    if (raining) {
+        const float *hp = m.class_horton + 3 * meta_class(v.meta);
+        f0 = hp[0];
+        fc = hp[1];
+        hk = hp[2];
+    }
+    const float area = l.regular ? v.size * v.size : m.area[cell];
+    t.bi = bi;
+    t.kind = kind;
+
+    const float pq = v.pq;
+    float pipe = fmx(pq, 0.0f) + fmn(pq, 0.0f) * ds;
+    float discharge = 0.0f;
+    if (kind == 1) {
+        float requested = bvalue * bw;
+        discharge = fmx(requested, 0.0f) + fmn(requested, 0.0f) * ds;
+        t.disc = discharge * dt;
+    }
+    float ssq = pipe + discharge;
+    // compensated window volume (solver_compact.py:535-539)
+    const float pv_increment = pipe * dt - v.pv_correction;
+    const float pv_total = v.pv_before + pv_increment;
+    const float cap = rainq > 0.0f ? fc + (f0 - fc) * det_expf(-hk * (v.rain_time + dt) / 3600.0f) : v.cap;
+    float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
+    if (l.short_list) {
+#pragma unroll
+        for (int j = 0; j < NHF_LIST_MAX; ++j) {
+            if (j < total) {
+                const float p = qv[j] * len[j];
+                const int dir = entry_face(j, n0, n01, n012);
+                if (dir == 0)
+                    ql += p;
+                else if (dir == 1)
+                    qr += p;
+                else if (dir == 2)
+                    qb += p;
+                else
+                    qt += p;
+            }
+        }
+    } else {
+        float fsum[4];
+        int k = k0;
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            float acc = 0.0f;
+            for (int j = 0; j < n; ++j, ++k) {
+                int side = m.isl[k];
+                acc += m.q[side] * m.length[side];
+            }
+            fsum[dir] = acc;
+        }
+        ql = fsum[0];
+        qr = fsum[1];
+        qb = fsum[2];
+        qt = fsum[3];
+    }
+    const float previous = v.h;
+    float tq = ql - qr + qb - qt + ssq;
+    float rain_depth = rainq * dt;
+    float increment = tq * dt / area + rain_depth;
+    float raw = previous + increment;
+    float removed = fmn(cap * dt, fmx(raw, 0.0f));
+    raw -= removed;
+    float floor_v = fmx(-raw, 0.0f);
+    t.infil = removed * area;
+    t.pipe = pipe * dt;
+    float h_new;
+    if (kind == 0) {
+        t.level_floor = floor_v * area;
+        float target = fmx(lvalue - bed, 0.0f);
+        t.lv[0] = (target - previous) * area;
+        t.lv[1] = -(ql - qr + qb - qt) * dt;
+        t.lv[2] = -pipe * dt;
+        t.lv[3] = -rain_depth * area;
+        t.lv[4] = removed * area;
+        h_new = target;
+    } else {
+        t.floor = floor_v * area;
+        h_new = fmx(raw, 0.0f);
+    }
+    float hv = fmx(h_new, 0.01f);
+    float u = 0.0f, w = 0.0f;
+    if (h_new >= d.p.min_h) {
+        u = (ql + qr) / v.size / hv / 2.0f;
+        w = (qb + qt) / v.size / hv / 2.0f;
+    }
+    float speed = sqrtf(u * u + w * w);
+    // stores last
+    if (m.pipe_volume) {
+        m.pipe_correction[cell] = (pv_total - v.pv_before) - pv_increment;
+        m.pipe_volume[cell] = pv_total;
+    }
+    if (rainq > 0.0f) {
+        m.rain_time[cell] = v.rain_time + dt;
+        m.capacity[cell] = cap;
+    }
+    m.depth[cell] = h_new;
+    if (speed > v.max_s) m.max_speed[cell] = speed;
+    if (h_new > v.max_d) {
+        m.max_depth[cell] = h_new;
+        m.peak_tick[cell] = step_end;
+    }
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     Ctl *c = d.ctl;
@@ -424,179 +594,75 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     const bool raining = c->raining;
     const uint64_t step_end = c->step_end;
     const int b_num = m.b_num;
-    int cell = xcd_block() * BLOCK + threadIdx.x + 1;
-    bool live = cell < m.e_num;
-    // per-cell budget contributions (solver_compact.py:533, 578-590)
-    float t_infil = 0.0f, t_floor = 0.0f, t_level_floor = 0.0f, t_pipe = 0.0f, t_disc = 0.0f;
-    float t_lv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    int bi = -1, kind = -1;
-    if (live) {
-        // first round: the cell's own values and its side record
-        const uint32_t meta = m.cell_meta[cell];
-        const float previous = m.depth[cell];
-        const float size = m.size[cell];
-        const float cap_stored = m.capacity[cell];
-        const float max_d = m.max_depth[cell], max_s = m.max_speed[cell];
-        const float rt_stored = raining ? m.rain_time[cell] : 0.0f;
-        const uint32_t pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
-        const float pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
-        float pv_before = 0.0f, pv_correction = 0.0f;
-        if (m.pipe_volume) {
-            pv_correction = m.pipe_correction[cell];
-            pv_before = m.pipe_volume[cell];
-        }
-        CellList l;
-        cell_list(m, cell, l);
-        // second round: the donor-scaled side discharges (stored in q by k_donor and k_flux),
-        // the boundary, the cell's own scale, the rain rate and the Horton parameters
-        const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
-        const uint32_t cnt = l.cnt;
-        float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
-        if (l.short_list) {
-            gather_list(l, [&](int side) { return m.q[side]; }, qv);
-            list_lengths(m, l, size, len);
-        }
-        bi = meta_boundary(meta);
-        float bvalue = 0.0f, bw = 0.0f, lvalue = 0.0f, bed = 0.0f;
-        if (bi >= 0) {
-            kind = m.kind[bi];
-            bvalue = d.values[bi];
-            bw = m.bweight[cell];
-            lvalue = d.level_values[bi];
-            bed = m.bed[cell];
-        }
-        // the cell's own scale, stored by k_donor for boundary and pipe cells only
-        const float ds = (bi >= 0 || m.pipe_q) ? d.donor[cell] : 1.0f;
-        const float rainq = raining ? d.rain_rates[pix] : 0.0f;
-        float f0 = 0.0f, fc = 0.0f, hk = 0.0f;
-        if (raining) {
-            const float *hp = m.class_horton + 3 * meta_class(meta);
-            f0 = hp[0];
-            fc = hp[1];
-            hk = hp[2];
-        }
-        const float area = l.regular ? size * size : m.area[cell];
-
-        float pipe = fmx(pq, 0.0f) + fmn(pq, 0.0f) * ds;
-        float discharge = 0.0f;
-        if (kind == 1) {
-            float requested = bvalue * bw;
-            discharge = fmx(requested, 0.0f) + fmn(requested, 0.0f) * ds;
-            t_disc = discharge * dt;
-        }
-        float ssq = pipe + discharge;
-        // compensated window volume (solver_compact.py:535-539)
-        const float pv_increment = pipe * dt - pv_correction;
-        const float pv_total = pv_before + pv_increment;
-        const float cap = rainq > 0.0f ? fc + (f0 - fc) * det_expf(-hk * (rt_stored + dt) / 3600.0f) : cap_stored;
-        float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
-        if (l.short_list) {
-#pragma unroll
-            for (int j = 0; j < NHF_LIST_MAX; ++j) {
-                if (j < total) {
-                    const float p = qv[j] * len[j];
-                    const int dir = entry_face(j, n0, n01, n012);
-                    if (dir == 0)
-                        ql += p;
-                    else if (dir == 1)
-                        qr += p;
-                    else if (dir == 2)
-                        qb += p;
-                    else
-                        qt += p;
-                }
-            }
-        } else {
-            float fsum[4];
-            int k = k0;
-            for (int dir = 0; dir < 4; ++dir) {
-                int n = face_n(cnt, dir);
-                float acc = 0.0f;
-                for (int j = 0; j < n; ++j, ++k) {
-                    int side = m.isl[k];
-                    acc += m.q[side] * m.length[side];
-                }
-                fsum[dir] = acc;
-            }
-            ql = fsum[0];
-            qr = fsum[1];
-            qb = fsum[2];
-            qt = fsum[3];
-        }
-        float tq = ql - qr + qb - qt + ssq;
-        float rain_depth = rainq * dt;
-        float increment = tq * dt / area + rain_depth;
-        float raw = previous + increment;
-        float removed = fmn(cap * dt, fmx(raw, 0.0f));
-        raw -= removed;
-        float floor_v = fmx(-raw, 0.0f);
-        t_infil = removed * area;
-        t_pipe = pipe * dt;
-        float h_new;
-        if (kind == 0) {
-            t_level_floor = floor_v * area;
-            float target = fmx(lvalue - bed, 0.0f);
-            t_lv[0] = (target - previous) * area;
-            t_lv[1] = -(ql - qr + qb - qt) * dt;
-            t_lv[2] = -pipe * dt;
-            t_lv[3] = -rain_depth * area;
-            t_lv[4] = removed * area;
-            h_new = target;
-        } else {
-            t_floor = floor_v * area;
-            h_new = fmx(raw, 0.0f);
-        }
-        float hv = fmx(h_new, 0.01f);
-        float u = 0.0f, v = 0.0f;
-        if (h_new >= d.p.min_h) {
-            u = (ql + qr) / size / hv / 2.0f;
-            v = (qb + qt) / size / hv / 2.0f;
-        }
-        float speed = sqrtf(u * u + v * v);
-        // stores last
-        if (m.pipe_volume) {
-            m.pipe_correction[cell] = (pv_total - pv_before) - pv_increment;
-            m.pipe_volume[cell] = pv_total;
-        }
-        if (rainq > 0.0f) {
-            m.rain_time[cell] = rt_stored + dt;
-            m.capacity[cell] = cap;
-        }
-        m.depth[cell] = h_new;
-        if (speed > max_s) m.max_speed[cell] = speed;
-        if (h_new > max_d) {
-            m.max_depth[cell] = h_new;
-            m.peak_tick[cell] = step_end;
-        }
-    }
+    const int cell = xcd_block() * BLOCK + threadIdx.x + 1;
+    const bool live = cell < m.e_num;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
-    // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it.
     double2 *acc = d.acc + ((size_t)blockIdx.x * (BLOCK / 64) + wave) * d.terms;
-    for (int t = 0; t < d.terms; ++t) {
-        float v = 0.0f;
-        if (t == 0)
-            v = t_infil;
-        else if (t == 1)
-            v = t_floor;
-        else if (t == 2)
-            v = t_level_floor;
-        else if (t == 3)
-            v = t_pipe;
-        else if (t < 4 + 5 * b_num) {
-            int tb = (t - 4) / 5, comp = (t - 4) % 5;
-            if (kind == 0 && bi == tb) v = t_lv[comp];
-        } else if (kind == 1 && bi == t - 4 - 5 * b_num) {
-            v = t_disc;
+    // first round: the cell's own values and its side record
+    DepthCell v = {};
+    if (live) {
+        v.meta = m.cell_meta[cell];
+        v.h = m.depth[cell];
+        v.size = m.size[cell];
+        v.cap = m.capacity[cell];
+        v.max_d = m.max_depth[cell];
+        v.max_s = m.max_speed[cell];
+        v.rain_time = raining ? m.rain_time[cell] : 0.0f;
+        v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
+        v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+        if (m.pipe_volume) {
+            v.pv_correction = m.pipe_correction[cell];
+            v.pv_before = m.pipe_volume[cell];
         }
-        if (!__any(v != 0.0f)) continue;
-        double w = wave_sum((double)v);
-        if (lane == 0) {
-            double2 a = acc[t];
+        v.rec = cell_record(m, cell);
+    }
+    DepthTerms t = {};
+    t.bi = -1;
+    t.kind = -1;
+    if (__all(!live || v.rec.x >= 0)) {
+        if (live) depth_cell<true>(d, cell, dt, raining, step_end, v, t);
+    } else if (live) {
+        depth_cell<false>(d, cell, dt, raining, step_end, v, t);
+    }
+    // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
+    // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it. With at most
+    // 64 terms, lane t holds the accumulator of term t, loaded once while the sums are formed.
+    const bool lane_terms = d.terms <= 64;
+    double2 held = make_double2(0.0, 0.0);
+    if (lane_terms && lane < d.terms) held = acc[lane];
+    bool touched = false;
+    for (int k = 0; k < d.terms; ++k) {
+        float x = 0.0f;
+        if (k == 0)
+            x = t.infil;
+        else if (k == 1)
+            x = t.floor;
+        else if (k == 2)
+            x = t.level_floor;
+        else if (k == 3)
+            x = t.pipe;
+        else if (k < 4 + 5 * b_num) {
+            int tb = (k - 4) / 5, comp = (k - 4) % 5;
+            // a select chain, not an indexed load: the terms stay in registers
+            const float lv = comp == 0 ? t.lv[0] : comp == 1 ? t.lv[1] : comp == 2 ? t.lv[2] : comp == 3 ? t.lv[3] : t.lv[4];
+            if (t.kind == 0 && t.bi == tb) x = lv;
+        } else if (t.kind == 1 && t.bi == k - 4 - 5 * b_num) {
+            x = t.disc;
+        }
+        if (!__any(x != 0.0f)) continue;
+        double w = wave_sum((double)x);
+        if (lane_terms) {
+            if (lane == k) {
+                acc_add(held, w);
+                touched = true;
+            }
+        } else if (lane == 0) {
+            double2 a = acc[k];
             acc_add(a, w);
-            acc[t] = a;
+            acc[k] = a;
         }
     }
+    if (touched) acc[lane] = held;
 }
 
 // --------------------------------------------------------------------------- (c) CFL dt
@@ -844,7 +910,7 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     ctx->pending_steps = 0;
     err = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming);
     if (err != hipSuccess) {
-        hipHostFree(ctx->host_trace);
+        (void)hipHostFree(ctx->host_trace);
         delete ctx;
         return set_error("hipEventCreate", err);
     }
@@ -856,13 +922,13 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
 
 int32_t nhf_destroy(nhf_ctx *ctx) {
     if (!ctx) return 0;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i <= NHF_MAX_BATCH; ++i)
-        if (ctx->exec[i]) hipGraphExecDestroy(ctx->exec[i]);
-    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
-    hipEventDestroy(ctx->done);
-    if (ctx->host_trace) hipHostFree(ctx->host_trace);
-    if (ctx->host_acc) hipHostFree(ctx->host_acc);
+        if (ctx->exec[i]) (void)hipGraphExecDestroy(ctx->exec[i]);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ctx->done);
+    if (ctx->host_trace) (void)hipHostFree(ctx->host_trace);
+    if (ctx->host_acc) (void)hipHostFree(ctx->host_acc);
     delete ctx;
     return 0;
 }
@@ -959,7 +1025,7 @@ int32_t nhf_launch_batch(nhf_ctx *ctx, int32_t steps) {
             if (rc) return rc;
             if (end != hipSuccess) return set_error("hipStreamEndCapture", end);
             hipError_t inst = hipGraphInstantiate(&ctx->exec[steps], graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             if (inst != hipSuccess) return set_error("hipGraphInstantiate", inst);
         }
         HIP_TRY(hipGraphLaunch(ctx->exec[steps], s));
@@ -1123,7 +1189,7 @@ int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i <= NHF_MAX_BATCH; ++i)
             if (ctx->exec[i]) {
-                hipGraphExecDestroy(ctx->exec[i]);
+                (void)hipGraphExecDestroy(ctx->exec[i]);
                 ctx->exec[i] = nullptr;
             }
         ctx->d.cfl_blocks = cfl_blocks;
@@ -1188,7 +1254,7 @@ int32_t nhf_clock_probe(const uint64_t *now, const uint64_t *target, const uint6
             (e = hipMemcpy(out_times, d_times, 2 * n * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)
             rc = set_error("nhf_clock_probe", e);
     }
-    hipFree(buf);
+    (void)hipFree(buf);
     return rc;
 }
 

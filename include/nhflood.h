@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 5
+#define NHF_ABI_VERSION 6
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -81,7 +81,8 @@ typedef struct nhf_mesh {
     /* per-cell static */
     const float *area;                /* F32 of the F64 width x height (MeshGeometry.area) */
     const float *size;                /* edge length; cells are square (force.py:52-53) */
-    const uint32_t *cell_meta;        /* land class | (boundary index + 1) << 8, 0 = none */
+    const uint32_t *cell_meta;        /* land class | (boundary index + 1) << 8 (0 = none)
+                                         | size level << 16 when size_base > 0 */
     const float *class_horton;        /* [256*3] Horton f0, fc (m/s), k (1/h) per land class */
     const int32_t *rain_index;        /* cell -> rain pixel, read only when n_pixels > 1 */
     const float *bweight;             /* discharge share of the boundary */
@@ -114,6 +115,10 @@ typedef struct nhf_mesh {
        length size[cell], and area size*size (in F32) holds its left, right, bottom and top side
        ids; any other cell holds (-1, csr_start, csr_count, list length) */
     const int32_t *cell_quad;
+    /* > 0: every cell's size is size_base * 2^level exactly (level = cell_meta bits 16..19), and
+       the kernels compute it rather than read `size`; 0: they read `size` */
+    float size_base;
+    int32_t reserved_tail;
 } nhf_mesh;
 
 typedef struct nhf_ctx nhf_ctx;

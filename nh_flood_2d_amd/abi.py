@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -39,11 +39,12 @@ MESH_FIELDS = [
     ('kind', 'p'),
     ('pipe_volume', 'p'), ('pipe_correction', 'p'),
     ('cell_quad', 'p'),
+    ('size_base', 'f'), ('reserved_tail', 'i'),
 ]
 
 
 class MeshStruct(ctypes.Structure):
-    _fields_ = [(name, c_int32 if kind == 'i' else c_void_p) for name, kind in MESH_FIELDS]
+    _fields_ = [(name, {'i': c_int32, 'f': ctypes.c_float, 'p': c_void_p}[kind]) for name, kind in MESH_FIELDS]
 
 
 # nhb_state: per-scenario state of a batch ([entity][slot] device arrays) and the boundary cells

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import abi
 from .engine import STATE_CELLS, STATE_SIDES, TorchDevice
-from .layout import Layout
+from .layout import Layout, meta_boundary
 
 # per-slot cell state of a batch; the bed is shared
 SLOT_CELLS = tuple(n for n in STATE_CELLS if n != 'bed')
@@ -40,8 +40,7 @@ def lanes_for(count: int) -> int:
 
 def boundary_cells(layout: Layout):
     """Boundary-cell list of a layout (device order): per-cell index or -1, and per-boundary ranges."""
-    meta = layout.arrays['cell_meta'].astype(np.int64)
-    bid = (meta >> 8) - 1
+    bid = meta_boundary(layout.arrays['cell_meta'])
     e_num = layout.e_num
     cells = np.nonzero(bid[1:] >= 0)[0] + 1
     order = np.lexsort((cells, bid[cells]))
@@ -97,6 +96,7 @@ class BatchEngine:
         m.r_len = int(layout.arrays['_r_len'][0])
         m.b_num, m.n_pixels = layout.b_num, layout.n_pixels
         m.n_grad, m.n_hang, m.flags = layout.n_grad, layout.n_hang, 0
+        m.size_base = layout.size_base
         for name, kind in abi.MESH_FIELDS:
             if kind == 'p':
                 buf = self.static.get(name)

@@ -324,7 +324,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     if (dt == 0.0f) return;
     const nhf_mesh &m = d.m;
     Ctl *c = &d.ctl[s];
-    int bi = meta_boundary(m.cell_meta[cell]);
+    const uint32_t meta = m.cell_meta[cell];
+    int bi = meta_boundary(meta);
     float discharge = 0.0f;
     bool level = false;
     if (bi >= 0) {
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     float rainq = d.raining[s] ? d.rain_rates[(size_t)s * d.n_pixels + (d.n_pixels > 1 ? m.rain_index[cell] : 0)] : 0.0f;
     CellList l;
     cell_list(m, cell_record(m, cell), l);
-    const float size = l.regular ? m.size[cell] : 0.0f;
+    const float size = l.regular ? cell_size(m, meta, cell) : 0.0f;
     const float area = l.regular ? size * size : m.area[cell];
     float positive = (fmx(0.0f, 0.0f) + fmx(discharge, 0.0f)) * dt;
     float available = d.depth[IX(cell)] * area + positive + rainq * dt * area;
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
             float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
             CellList l;
             cell_list(m, cell_record(m, cell), l);
-            const float reg_size = l.regular ? m.size[cell] : 0.0f;
+            const float reg_size = l.regular ? cell_size(m, meta, cell) : 0.0f;
             const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
             const uint32_t cnt = l.cnt;
             if (l.short_list) {
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
             float hv = fmx(h_new, 0.01f);
             float u = 0.0f, v = 0.0f;
             if (h_new >= d.p.min_h) {
-                const float size = l.regular ? reg_size : m.size[cell];
+                const float size = l.regular ? reg_size : cell_size(m, meta, cell);
                 u = (ql + qr) / size / hv / 2.0f;
                 v = (qb + qt) / size / hv / 2.0f;
             }

@@ -190,6 +190,12 @@ static __device__ __forceinline__ int xcd_block() {
 static __device__ __forceinline__ int meta_boundary(uint32_t meta) { return (int)((meta >> 8) & 255u) - 1; }
 static __device__ __forceinline__ int meta_class(uint32_t meta) { return (int)(meta & 255u); }
 
+// A cell's size: size_base * 2^level from its meta word when the mesh is of that form (exact: a
+// power-of-two scaling), else the stored value.
+static __device__ __forceinline__ float cell_size(const nhf_mesh &m, uint32_t meta, int cell) {
+    return m.size_base > 0.0f ? ldexpf(m.size_base, (int)((meta >> 16) & 15u)) : m.size[cell];
+}
+
 // --------------------------------------------------------------------------- x^y and e^x
 // Evaluated in FP64 and rounded once to F32, so the result is the same on any IEEE machine:
 // log2 by range reduction to m in [sqrt(1/2), sqrt(2)) and the odd series of

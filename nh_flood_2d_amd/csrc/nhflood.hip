@@ -313,6 +313,7 @@ template <bool REGULAR>
 static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float dt, bool raining, const DonorCell &v,
                                                   uint32_t &limited, uint32_t &faces, uint32_t &scale_bits) {
     const nhf_mesh &m = d.m;
+    const float size = m.size_base > 0.0f ? cell_size(m, v.meta, cell) : v.size;
     CellList l;
     cell_list<REGULAR>(m, v.rec, l);
     // second round: the sides' discharges, the boundary and the rain rate
@@ -323,7 +324,7 @@ static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float 
     float qf[NHF_LIST_MAX], len[NHF_LIST_MAX];
     if (short_list) {
         gather_list(l, [&](int side) { return d.qflux[side]; }, qf);
-        list_lengths(m, l, v.size, len);
+        list_lengths(m, l, size, len);
     }
     const int bi = meta_boundary(v.meta);
     int kind = -1;
@@ -334,7 +335,7 @@ static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float 
         bw = m.bweight[cell];
     }
     const float rainq = raining ? d.rain_rates[v.pix] : 0.0f;
-    const float area = l.regular ? v.size * v.size : m.area[cell];
+    const float area = l.regular ? size * size : m.area[cell];
     const float discharge = kind == 1 ? bvalue * bw : 0.0f;
     const bool level = kind == 0;
     const float pq = v.pq;
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
     if (live) {
         v.meta = m.cell_meta[cell];
         v.h = m.depth[cell];
-        v.size = m.size[cell];
+        v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
         v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
         v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
         v.rec = cell_record(m, cell);
@@ -455,6 +456,7 @@ template <bool REGULAR>
 static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float dt, bool raining, uint64_t step_end,
                                                   const DepthCell &v, DepthTerms &t) {
     const nhf_mesh &m = d.m;
+    const float size = m.size_base > 0.0f ? cell_size(m, v.meta, cell) : v.size;
     CellList l;
     cell_list<REGULAR>(m, v.rec, l);
     // second round: the donor-scaled side discharges (stored in q by k_donor and k_flux), the
@@ -464,7 +466,7 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
     float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
     if (l.short_list) {
         gather_list(l, [&](int side) { return m.q[side]; }, qv);
-        list_lengths(m, l, v.size, len);
+        list_lengths(m, l, size, len);
     }
     const int bi = meta_boundary(v.meta);
     int kind = -1;
@@ -486,7 +488,7 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
         fc = hp[1];
         hk = hp[2];
     }
-    const float area = l.regular ? v.size * v.size : m.area[cell];
+    const float area = l.regular ? size * size : m.area[cell];
     t.bi = bi;
     t.kind = kind;
 
@@ -564,8 +566,8 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
     float hv = fmx(h_new, 0.01f);
     float u = 0.0f, w = 0.0f;
     if (h_new >= d.p.min_h) {
-        u = (ql + qr) / v.size / hv / 2.0f;
-        w = (qb + qt) / v.size / hv / 2.0f;
+        u = (ql + qr) / size / hv / 2.0f;
+        w = (qb + qt) / size / hv / 2.0f;
     }
     float speed = sqrtf(u * u + w * w);
     // stores last
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     if (live) {
         v.meta = m.cell_meta[cell];
         v.h = m.depth[cell];
-        v.size = m.size[cell];
+        v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
         v.cap = m.capacity[cell];
         v.max_d = m.max_depth[cell];
         v.max_s = m.max_speed[cell];
@@ -676,9 +678,11 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode) {
     __shared__ uint64_t s_key[NHF_WAVES];
     __shared__ int32_t s_idx[NHF_WAVES];
     const nhf_mesh &m = d.m;
-    uint64_t key = ~0ull;
-    int32_t idx = 0;
     const float afa = d.p.afa, g = d.p.g;
+    // The thread's minimum of (bound bits, reference id): the reference id of a side is read only
+    // where the bounds tie, and for the thread's winner at the end.
+    uint32_t best_bits = ~0u;
+    int32_t idx = -1;
     for (int si = xcd_block() * NHF_BLOCK + threadIdx.x + 1; si < m.s_num; si += gridDim.x * NHF_BLOCK) {
         int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
         if (lh.x != 0 && lh.y != 0) {
@@ -686,13 +690,15 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode) {
             float head = fmx(zl + m.depth[lh.x], zh + m.depth[lh.y]);
             float h = fmx(head - fmx(fmx(zl, zh), -999.0f), 0.01f);
             float cfl = afa * m.length[si] / (sqrtf(g * h) + fabsf(m.q[si]) / h);
-            uint64_t k2 = ((uint64_t)__float_as_uint(cfl) << 32) | m.side_key[si];
-            if (k2 < key) {
-                key = k2;
+            const uint32_t bits = __float_as_uint(cfl);
+            if (bits < best_bits || (bits == best_bits && idx >= 0 && m.side_key[si] < m.side_key[idx])) {
+                best_bits = bits;
                 idx = si;
             }
         }
     }
+    uint64_t key = idx >= 0 ? ((uint64_t)best_bits << 32) | m.side_key[idx] : ~0ull;
+    if (idx < 0) idx = 0;
     wave_min_key(key, idx);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) {

@@ -129,6 +129,7 @@ class Engine:
         m.n_pixels = layout.n_pixels
         m.n_grad, m.n_hang = layout.n_grad, layout.n_hang
         m.flags = 1 if pipe_q is not None else 0
+        m.size_base = layout.size_base
         for name, kind in abi.MESH_FIELDS:
             if kind == 'p':
                 buf = self.buffers.get(name)

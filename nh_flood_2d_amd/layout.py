@@ -82,6 +82,7 @@ class Layout:
     n_hang: int = 0
     theta: float = 1.0
     cell_f0: np.ndarray | None = None
+    size_base: float = 0.0     # > 0: every cell's size is size_base * 2**level (level in cell_meta)
 
     # --- permutations between the reference numbering and the device order
     def cells_to_device(self, a: np.ndarray) -> np.ndarray:
@@ -115,7 +116,7 @@ def relayout(base: Layout, mesh: Mesh, land_parameters: dict, forcing: Forcing) 
     if (mesh.e_num, mesh.s_num) != (base.e_num, base.s_num):
         raise ValueError('relayout needs the mesh of the base layout')
     bid = forcing.boundary_id.astype(np.int64)
-    same = (np.array_equal(((base.arrays['cell_meta'].astype(np.int64) >> 8) - 1)[1:], bid[base.cell_perm][1:])
+    same = (np.array_equal(meta_boundary(base.arrays['cell_meta'])[1:], bid[base.cell_perm][1:])
             and np.array_equal(base.arrays['bweight'], forcing.bweight.astype(np.float32)[base.cell_perm])
             and np.array_equal(base.arrays['rain_index'], forcing.rain_pixel.astype(np.int32)[base.cell_perm])
             and base.n_pixels == forcing.n_pixels and base.b_num == len(forcing.kinds))
@@ -129,6 +130,27 @@ def relayout(base: Layout, mesh: Mesh, land_parameters: dict, forcing: Forcing) 
     layout.arrays = arrays
     layout.cell_f0 = f0.astype(np.float32)
     return layout
+
+
+META_LEVEL_SHIFT = 16   # cell_meta bits 16..19: size level (size = size_base * 2**level)
+
+
+def meta_boundary(meta: np.ndarray) -> np.ndarray:
+    """Boundary index (-1 for none) from cell_meta."""
+    return ((meta.astype(np.int64) >> 8) & 255) - 1
+
+
+def size_levels(size: np.ndarray):
+    """(base, levels) with size == base * 2**levels exactly in F32 for every real cell (index 0 is
+    the sentinel), levels in 0..15; (0.0, None) when the sizes are not of that form."""
+    s = size[1:].astype(np.float32)
+    if len(s) == 0 or not np.all(np.isfinite(s)) or s.min() <= 0:
+        return 0.0, None
+    base = np.float32(s.min())
+    levels = np.rint(np.log2(s.astype(np.float64) / float(base))).astype(np.int64)
+    if levels.max() > 15 or not np.array_equal(np.ldexp(base, levels).astype(np.float32), s):
+        return 0.0, None
+    return float(base), np.concatenate([[0], levels])
 
 
 def cell_records(counts, start, packed, isl, length, size, area) -> np.ndarray:
@@ -234,8 +256,12 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
     )
     arrays['cell_quad'] = cell_records(counts, start, packed, isl, arrays['length'], arrays['size'],
                                        arrays['area'])
+    size_base, levels = size_levels(arrays['size'])
+    if levels is not None:
+        arrays['cell_meta'] = arrays['cell_meta'] | (levels.astype(np.uint32) << META_LEVEL_SHIFT)
+        arrays['cell_meta'][0] = 0
     layout = Layout(e_num, s_num, cperm, cinv, sperm, sinv, arrays, forcing.n_pixels,
-                    len(forcing.kinds), 0, 0, float(theta))
+                    len(forcing.kinds), 0, 0, float(theta), size_base=size_base)
     layout.cell_f0 = f0.astype(np.float32)
     if theta != 1.:
         fr = open_face_fractions(forcing.selected_sides, mesh.length, topo, mesh.face_ptr,

@@ -27,12 +27,14 @@ def coefficients_at(forcing, t: float) -> np.ndarray:
 class Pair:
     """The same scenario bound to a HIP engine and an oracle engine."""
 
-    def __init__(self, sc, gpu_device, order='morton', theta=None, precision='f32'):
+    def __init__(self, sc, gpu_device, order='morton', theta=None, precision='f32', stored_sizes=False):
         self.sc = sc
         self.forcing = scenario_forcing(sc)
         theta = sc.sita if theta is None else theta
         self.layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, self.forcing),
                                    theta, order)
+        if stored_sizes:        # the kernels read the size array instead of computing sizes
+            self.layout.size_base = 0.0
         self.params = make_params(sc.afa, theta, sc.min_h, 32)
         state = initial_state(sc.mesh, self.layout)
         if sc.initial_depth is not None:

@@ -11,7 +11,7 @@ from batch_oracle import OracleBatch
 from nh_flood_2d_amd import synth
 from nh_flood_2d_amd.batch import BatchRunner, boundary_cells, lanes_for
 from nh_flood_2d_amd.forcing import cell_forcing_arrays
-from nh_flood_2d_amd.layout import build_layout, relayout
+from nh_flood_2d_amd.layout import build_layout, meta_boundary, relayout
 
 
 @pytest.fixture(scope='module')
@@ -65,7 +65,7 @@ def test_boundary_cell_list_is_sorted_by_boundary():
     sc = synth.small_multires(2, discharge=1.0)
     layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, synth.scenario_forcing(sc)), 1.0)
     bslot, start, n = boundary_cells(layout)
-    bid = (layout.arrays['cell_meta'].astype(np.int64) >> 8) - 1
+    bid = meta_boundary(layout.arrays['cell_meta'])
     cells = np.argsort(np.where(bslot >= 0, bslot, 1 << 30))[:n]
     assert n == int(np.sum(bid[1:] >= 0)) and n > 0
     assert np.all(np.diff(bid[cells]) >= 0)

@@ -173,3 +173,24 @@ def test_launch_tuning_keeps_the_bits(gpu_device, cfl_blocks, cell_block):
     np.testing.assert_array_equal(da, db)
     np.testing.assert_array_equal(qa, qb)
     np.testing.assert_allclose(ba.sum(axis=1), bb.sum(axis=1), rtol=1e-12, atol=1e-12)
+
+
+def test_stored_and_computed_cell_sizes_give_the_same_bits(gpu_device):
+    """A mesh whose sizes are not base * 2^level makes the kernels read the size array; on a mesh
+    where both work, both give the same state, trace and budget."""
+    sc = synth.small_multires(5, pond=0.3, discharge=1.0)
+    _say('[sizes] small_multires seed 5: computed vs stored cell sizes')
+    runs = []
+    for stored in (False, True):
+        pair = Pair(sc, gpu_device, stored_sizes=stored)
+        assert (pair.layout.size_base == 0.0) == stored
+        rows = [pair.fixed_step(0.05)[0] for _ in range(12)]
+        assert pair.diff('depth') <= H_TOL and pair.diff('q') <= Q_TOL
+        runs.append((np.array(rows), pair.gpu.read_state('depth'), pair.gpu.read_state('q'),
+                     pair.gpu.read_state('max_speed'), pair.gpu.flush_budget()))
+    (ra, da, qa, sa, ba), (rb, db, qb, sb, bb) = runs
+    np.testing.assert_array_equal(ra, rb)
+    np.testing.assert_array_equal(da.view(np.uint32), db.view(np.uint32))
+    np.testing.assert_array_equal(qa.view(np.uint32), qb.view(np.uint32))
+    np.testing.assert_array_equal(sa.view(np.uint32), sb.view(np.uint32))
+    np.testing.assert_allclose(ba.sum(axis=1), bb.sum(axis=1), rtol=1e-12, atol=1e-12)

@@ -375,6 +375,33 @@ def test_cell_record_names_the_four_sides_of_regular_cells_and_points_into_lists
     assert regular > 0 and irregular > 0
 
 
+def test_size_levels_only_for_exact_power_of_two_multiples():
+    from nh_flood_2d_amd.layout import size_levels
+    base, levels = size_levels(np.array([0, 8, 2, 4, 2], np.float32))
+    assert base == 2.0 and list(levels) == [0, 2, 0, 1, 0]
+    base, levels = size_levels(np.array([0, 0.1, 0.2, 0.4], np.float32))   # 2 * F32(0.1) == F32(0.2)
+    assert base == np.float32(0.1) and list(levels) == [0, 0, 1, 2]
+    assert size_levels(np.array([0, 3, 4], np.float32)) == (0.0, None)
+    assert size_levels(np.array([0, 1, 2.0 ** 16], np.float32)) == (0.0, None)   # level above 15
+    assert size_levels(np.array([0], np.float32)) == (0.0, None)
+
+
+def test_size_level_bits_leave_class_and_boundary_fields():
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, meta_boundary
+    sc = synth.small_multires(3, discharge=1.0)
+    forcing = cell_forcing_arrays(sc.mesh, synth.scenario_forcing(sc))
+    layout = build_layout(sc.mesh, sc.land_parameters, forcing, 1.0)
+    meta = layout.arrays['cell_meta']
+    size = layout.arrays['size']
+    assert layout.size_base > 0
+    np.testing.assert_array_equal(np.ldexp(np.float32(layout.size_base), (meta[1:] >> 16) & 15).astype(np.float32),
+                                  size[1:])
+    np.testing.assert_array_equal(meta & 255, sc.mesh.land[layout.cell_perm].astype(np.uint32) * (np.arange(len(meta)) > 0))
+    np.testing.assert_array_equal(meta_boundary(meta)[1:], forcing.boundary_id[layout.cell_perm][1:])
+
+
 # ------------------------------------------------------------------------------ C ABI
 
 def test_hip_library_loads_and_exports_every_declared_symbol():

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 6
+#define NHF_ABI_VERSION 7
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -185,6 +185,14 @@ int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side);
 
 /* Wait for all work of the context's stream. */
 int32_t nhf_sync(nhf_ctx *ctx);
+
+/* Make max_depth and peak_tick final (the extrema of solver_compact.py:600-606 as
+ * reset_extrema/write_extrema see them, :674-679, output.py:130-150). The step leaves a cell
+ * whose depth rose in its last step marked by a negative max_depth (its maximum is its depth,
+ * reached at the end of that step) and writes the pair out when the rise stops; this call writes
+ * out every marked cell. Call it before reading either array, and before writing depth alone.
+ * No reference counterpart (the reference writes both arrays on every rise). */
+int32_t nhf_settle_extrema(nhf_ctx *ctx);
 
 /* Launch mode: 1 replays each batch as one captured hipGraph, 0 launches kernels one by one. */
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable);

@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -80,6 +80,7 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_set_timing': (c_int32, [vp, c_int32]),
         'nhf_set_tuning': (c_int32, [vp, c_int32, c_int32]),
         'nhf_time_kernels': (c_int32, [vp, c_int32, vp]),
+        'nhf_settle_extrema': (c_int32, [vp]),
         'nhf_tuning_range': (c_int32, [vp, vp]),
         'nhf_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
         'nhf_clock_probe': (c_int32, [vp, vp, vp, vp, c_float, c_int32, vp, vp, vp]),
@@ -121,7 +122,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhf_launch_batch', 'nhf_wait_batch',
             'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
             'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe',
-            'nhf_set_tuning', 'nhf_tuning_range', 'nhf_time_kernels',
+            'nhf_set_tuning', 'nhf_tuning_range', 'nhf_time_kernels', 'nhf_settle_extrema',
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
             'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
             'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times', 'nhb_set_tuning',

@@ -125,6 +125,19 @@ __global__ void k_clear_acc(Dev d) {
         d.acc[i] = make_double2(0.0, 0.0);
 }
 
+// Writes out the deferred extrema: a cell marked as rising (max_depth < 0) gets its depth as
+// maximum and the clock's current tick (the end of the last step) as peak tick.
+__global__ __launch_bounds__(NHF_BLOCK) void k_settle_extrema(Dev d) {
+    const nhf_mesh &m = d.m;
+    const uint64_t now = d.ctl->now;
+    for (int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1; cell < m.e_num; cell += gridDim.x * NHF_BLOCK) {
+        if (m.max_depth[cell] < 0.0f) {
+            m.max_depth[cell] = m.depth[cell];
+            m.peak_tick[cell] = now;
+        }
+    }
+}
+
 __global__ void k_reset_limit(Dev d) {
     if (threadIdx.x == 0) d.ctl->limit_value[0] = INFINITY;
 }
@@ -453,7 +466,7 @@ struct DepthTerms {
 };
 
 template <bool REGULAR>
-static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float dt, bool raining, uint64_t step_end,
+static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float dt, bool raining, uint64_t step_start,
                                                   const DepthCell &v, DepthTerms &t) {
     const nhf_mesh &m = d.m;
     const float size = m.size_base > 0.0f ? cell_size(m, v.meta, cell) : v.size;
@@ -581,9 +594,17 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
     }
     m.depth[cell] = h_new;
     if (speed > v.max_s) m.max_speed[cell] = speed;
-    if (h_new > v.max_d) {
-        m.max_depth[cell] = h_new;
-        m.peak_tick[cell] = step_end;
+    // Extrema (solver_compact.py:600-606): the maximum depth and the end tick of the step that
+    // first reached it. A negative max_depth marks a cell whose depth rose in its last step: its
+    // maximum is its depth and its tick the end of that step, written out only when the rise
+    // stops (or by k_settle_extrema), so a filling cell writes nothing step after step.
+    if (v.max_d < 0.0f) {
+        if (!(h_new > previous)) {
+            m.max_depth[cell] = previous;
+            m.peak_tick[cell] = step_start;
+        }
+    } else if (h_new > v.max_d) {
+        m.max_depth[cell] = -1.0f;
     }
 }
 
@@ -594,7 +615,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
     const bool raining = c->raining;
-    const uint64_t step_end = c->step_end;
+    const uint64_t step_start = c->now;  // the clock commits this step's end in k_finish
     const int b_num = m.b_num;
     const int cell = xcd_block() * BLOCK + threadIdx.x + 1;
     const bool live = cell < m.e_num;
@@ -622,9 +643,9 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     t.bi = -1;
     t.kind = -1;
     if (__all(!live || v.rec.x >= 0)) {
-        if (live) depth_cell<true>(d, cell, dt, raining, step_end, v, t);
+        if (live) depth_cell<true>(d, cell, dt, raining, step_start, v, t);
     } else if (live) {
-        depth_cell<false>(d, cell, dt, raining, step_end, v, t);
+        depth_cell<false>(d, cell, dt, raining, step_start, v, t);
     }
     // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
     // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it. With at most
@@ -1177,6 +1198,15 @@ int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side) {
 
 int32_t nhf_sync(nhf_ctx *ctx) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int32_t nhf_settle_extrema(nhf_ctx *ctx) {
+    if (ctx->pending_steps) return set_error_text("nhf_settle_extrema: a batch is in flight");
+    const Dev &d = ctx->d;
+    hipLaunchKernelGGL(k_settle_extrema, dim3(blocks_for(d.m.e_num) < 2048 ? blocks_for(d.m.e_num) : 2048),
+                       dim3(NHF_BLOCK), 0, ctx->stream, d);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

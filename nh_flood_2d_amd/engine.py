@@ -158,7 +158,13 @@ class Engine:
             pass
 
     # ----------------------------------------------------------------- state in original order
+    def settle_extrema(self) -> None:
+        """Write out the extrema the step defers for rising cells (nhf_settle_extrema)."""
+        abi.check(self.lib, self.lib.nhf_settle_extrema(self.ctx), 'nhf_settle_extrema')
+
     def read_state(self, name: str) -> np.ndarray:
+        if name in ('max_depth', 'peak_tick'):
+            self.settle_extrema()
         dtype = np.uint64 if name == 'peak_tick' else np.float32
         values = self.device.read(self.buffers[name], dtype)
         if name in STATE_SIDES:
@@ -166,6 +172,8 @@ class Engine:
         return self.layout.cells_from_device(values)
 
     def write_state(self, name: str, values: np.ndarray) -> None:
+        if name in ('depth', 'max_depth', 'peak_tick'):
+            self.settle_extrema()     # a marked cell's maximum is its depth: keep the pair final
         dtype = np.uint64 if name == 'peak_tick' else np.float32
         values = np.asarray(values, dtype=dtype)
         if name in STATE_SIDES:

@@ -918,6 +918,12 @@ int32_t nhf_sync(nhf_ctx *c) {
     (void)c;
     return 0;
 }
+
+/* The oracle writes max_depth and peak_tick on every rise, so there is nothing to settle. */
+int32_t nhf_settle_extrema(nhf_ctx *c) {
+    (void)c;
+    return 0;
+}
 int32_t nhf_set_graph(nhf_ctx *c, int32_t enable) {
     (void)c;
     (void)enable;

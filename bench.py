@@ -75,18 +75,18 @@ def launch_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
     e_hang, g = layout.n_hang, layout.n_grad
     pixel = 4 if layout.n_pixels > 1 else 0
     return {
-        # per side: side_cells 8, meta 1, q 4, sdc 4, manning 4, qflux out 4 (+ edge_index 4);
-        # per cell: bed + depth 8; per hanging edge: offsets + slots 16; per gradient cell 8
+        # per side: side_cells 8, meta 1, q r+w 8, sdc 4, manning 4 (+ edge_index 4); per cell:
+        # bed + depth 8; per hanging edge: offsets + slots 16; per gradient cell 8
         'flux': ns * (25 + (4 if e_hang else 0)) + 8 * nc + 16 * e_hang + 8 * g,
-        # per cell: meta 4, area 4, depth 4, side-list header 8, donor out 4 (+ rain pixel);
-        # per side-list entry: side id 4; per side: qflux 4, length 4
-        'donor': (24 + pixel) * nc + 4 * r + 8 * ns,
-        # per cell: meta 4, donor 4, rain time r+w 8, capacity w 4, area 4, size 4, depth r+w 8,
-        # max speed 4, max depth 4, side-list header 8 (+ rain pixel); per side-list entry: side
-        # id 4, neighbour id 4; per side: qflux 4, length 4, scaled q out 4
-        'depth': (52 + pixel) * nc + 8 * r + 12 * ns,
-        # per side: side_cells 8, length 4, q 4, reference id 4; per cell: bed + depth 8
-        'cfl': 20 * ns + 8 * nc,
+        # per cell: meta 4, depth 4, side record 16 (+ rain pixel); per side: q 4 (the scaled
+        # rewrite of a limited cell's sides and the boundary cells' scale are excluded)
+        'donor': (24 + pixel) * nc + 4 * ns,
+        # per cell: meta 4, side record 16, depth r+w 8, capacity r+w 8, rain time r+w 8, max
+        # speed 4, max depth 4 (+ rain pixel); per side: q 4 (cell sizes come from the meta word)
+        'depth': (52 + pixel) * nc + 4 * ns,
+        # per side: side_cells 8, length 4, q 4 (reference ids only on ties); per cell: bed +
+        # depth 8
+        'cfl': 16 * ns + 8 * nc,
         # per gradient cell: its face neighbours' bed, depth, centre (about 5 x 12 B)
         'gradients': g * 60,
         'face_avg': 0, 'prepare': 0, 'finish': 0,

@@ -42,7 +42,6 @@ struct BDev {
     float *values, *level_values; // [slot][b]
     float *rain_rates;            // [slot][n_pixels]
     float *bdonor;                // [boundary cell][slot]
-    float *qflux;                 // [s_num][slot]
     float *face_qn;               // [e_num][4][slot]
     float *grad;                  // [n_grad][2][slot]
     uint64_t *cfl_key;            // [cfl_blocks][slot]
@@ -307,8 +306,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
         nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / f;
     }
     nq *= bf;
-    d.qflux[IX(si)] = nq;
-    if (meta & NHF_SIDE_EXTERIOR) d.q[IX(si)] = nq;  // final value, as in k_flux
+    d.q[IX(si)] = nq;  // in place, as in k_flux; kb_donor scales it where its donor is limited
 }
 
 // --------------------------------------------------------------------------- donor scale
@@ -348,7 +346,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     float qf[NHF_LIST_MAX];
     if (short_list) {
         float len[NHF_LIST_MAX];
-        gather_list(l, [&](int side) { return d.qflux[IX(side)]; }, qf);
+        gather_list(l, [&](int side) { return d.q[IX(side)]; }, qf);
         list_lengths(m, l, size, len);
 #pragma unroll
         for (int j = 0; j < NHF_LIST_MAX; ++j) {
@@ -364,7 +362,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
             int n = face_n(cnt, dir);
             for (int j = 0; j < n; ++j, ++k) {
                 int side = m.isl[k];
-                float sq = d.qflux[IX(side)] * m.length[side];
+                float sq = d.q[IX(side)] * m.length[side];
                 if (dir == 0 || dir == 2) sq = -sq;
                 outgoing += fmx(sq, 0.0f) * dt;
             }
@@ -376,26 +374,30 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
         ds = available / outgoing;
         limited = true;
     }
+    // in-place scaling of the sides this cell is the donor of, only where the scale is not 1
+    // (the argument of k_donor covers the neighbour that reads a side while it is scaled)
     uint32_t faces = 0;
-    if (short_list) {
+    if (ds != 1.0f) {
+        if (short_list) {
 #pragma unroll
-        for (int j = 0; j < NHF_LIST_MAX; ++j) {
-            if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
-                d.q[IX(ids[j])] = qf[j] * ds;
-                if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+            for (int j = 0; j < NHF_LIST_MAX; ++j) {
+                if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
+                    d.q[IX(ids[j])] = qf[j] * ds;
+                    if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+                }
             }
-        }
-    } else {
-        int k = k0;
-        for (int dir = 0; dir < 4; ++dir) {
-            int n = face_n(cnt, dir);
-            const bool upper = dir & 1;
-            for (int j = 0; j < n; ++j, ++k) {
-                int side = m.isl[k];
-                float q1 = d.qflux[IX(side)];
-                if ((q1 >= 0.0f) == upper) {
-                    d.q[IX(side)] = q1 * ds;
-                    if (q1 != 0.0f && ds < 1.0f) faces += 1;
+        } else {
+            int k = k0;
+            for (int dir = 0; dir < 4; ++dir) {
+                int n = face_n(cnt, dir);
+                const bool upper = dir & 1;
+                for (int j = 0; j < n; ++j, ++k) {
+                    int side = m.isl[k];
+                    float q1 = d.q[IX(side)];
+                    if ((q1 >= 0.0f) == upper) {
+                        d.q[IX(side)] = q1 * ds;
+                        if (q1 != 0.0f && ds < 1.0f) faces += 1;
+                    }
                 }
             }
         }
@@ -793,7 +795,7 @@ static int slot_shift(int slots) {
 }
 
 struct BCarve {
-    size_t ctl, dt, raining, coeff, values, level, rain, bdonor, qflux, face_qn, grad, key, idx, acc, bvals, bacc,
+    size_t ctl, dt, raining, coeff, values, level, rain, bdonor, face_qn, grad, key, idx, acc, bvals, bacc,
         flush, total;
 };
 
@@ -814,7 +816,6 @@ static BCarve bcarve(const nhf_mesh *m, const nhb_state *st, const nhf_params *p
     c.level = off; off = align_b(off + sizeof(float) * b * S);
     c.rain = off; off = align_b(off + sizeof(float) * (m->n_pixels > 0 ? m->n_pixels : 1) * S);
     c.bdonor = off; off = align_b(off + sizeof(float) * nb * S);
-    c.qflux = off; off = align_b(off + sizeof(float) * (size_t)m->s_num * S);
     c.face_qn = off; off = align_b(off + (p->theta == 1.0f ? 0 : sizeof(float) * 4 * (size_t)m->e_num * S));
     c.grad = off; off = align_b(off + sizeof(float) * 2 * (size_t)(m->n_grad > 0 ? m->n_grad : 1) * S);
     c.key = off; off = align_b(off + sizeof(uint64_t) * cfl_blocks * S);
@@ -959,7 +960,6 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     d.level_values = (float *)(ws + cv.level);
     d.rain_rates = (float *)(ws + cv.rain);
     d.bdonor = (float *)(ws + cv.bdonor);
-    d.qflux = (float *)(ws + cv.qflux);
     d.face_qn = d.theta_one ? nullptr : (float *)(ws + cv.face_qn);
     d.grad = (float *)(ws + cv.grad);
     d.cfl_key = (uint64_t *)(ws + cv.key);

@@ -36,7 +36,7 @@ struct Dev {
     nhf_params p;
     Ctl *ctl;
     float *coefficients, *values, *level_values, *rain_rates;
-    float *donor, *qflux, *face_qn, *grad;
+    float *donor, *face_qn, *grad;
     uint64_t *cfl_key;
     int32_t *cfl_idx;
     double2 *acc;  // per depth-block, per term: (sum, compensation)
@@ -93,7 +93,6 @@ __global__ void k_reset(Dev d, uint64_t now) {
     }
     for (int i = tid; i < d.acc_owners * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
     for (int i = tid; i < d.m.e_num; i += stride) d.donor[i] = 0.0f;
-    for (int i = tid; i < d.m.s_num; i += stride) d.qflux[i] = 0.0f;
     if (d.face_qn)
         for (int i = tid; i < d.m.e_num * 4; i += stride) d.face_qn[i] = 0.0f;
 }
@@ -300,10 +299,9 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
         nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / f;
     }
     nq *= bf;
-    d.qflux[si] = nq;
-    // An exterior side's discharge is +-0 (or NaN), which any donor scale in [0, 1] leaves as it
-    // is, so its final value is stored here; k_donor stores every side that has a real donor.
-    if (meta & NHF_SIDE_EXTERIOR) m.q[si] = nq;
+    // In place, as the reference's sq_t (solver_compact.py:486): no other thread of this pass
+    // reads this side's discharge. k_donor scales it afterwards where its donor is limited.
+    m.q[si] = nq;
 }
 
 // --------------------------------------------------------------------------- donor scale
@@ -336,7 +334,7 @@ static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float 
     const int *ids = l.ids;
     float qf[NHF_LIST_MAX], len[NHF_LIST_MAX];
     if (short_list) {
-        gather_list(l, [&](int side) { return d.qflux[side]; }, qf);
+        gather_list(l, [&](int side) { return m.q[side]; }, qf);
         list_lengths(m, l, size, len);
     }
     const int bi = meta_boundary(v.meta);
@@ -370,7 +368,7 @@ static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float 
             int n = face_n(cnt, dir);
             for (int j = 0; j < n; ++j, ++k) {
                 int side = m.isl[k];
-                float sq = d.qflux[side] * m.length[side];
+                float sq = m.q[side] * m.length[side];
                 if (dir == 0 || dir == 2) sq = -sq;
                 outgoing += fmx(sq, 0.0f) * dt;
             }
@@ -385,27 +383,32 @@ static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float 
     // Shared-subedge scaling (solver_compact.py:516-524): every side takes the scale of its
     // donor, the lower cell when q >= 0 and the higher cell otherwise. This cell is the lower
     // cell of its right/top sides, so it is the donor of a side exactly when
-    // (q >= 0) == (face is right/top), and it stores that side's scaled discharge: one writer
-    // per side.
-    if (short_list) {
+    // (q >= 0) == (face is right/top), and it scales that side's discharge in place: one writer
+    // per side, and only where the scale is not 1 (q * 1 is q). A neighbour may read a side
+    // while its donor scales it: the scale is in [0, 1), so the sign stays, or a zero of the
+    // same sign results. For the neighbour the side is inflow either way (no outgoing volume),
+    // and if it reads that zero it takes itself for the donor and stores the same zero.
+    if (ds != 1.0f) {
+        if (short_list) {
 #pragma unroll
-        for (int j = 0; j < NHF_LIST_MAX; ++j) {
-            if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
-                m.q[ids[j]] = qf[j] * ds;
-                if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+            for (int j = 0; j < NHF_LIST_MAX; ++j) {
+                if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
+                    m.q[ids[j]] = qf[j] * ds;
+                    if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+                }
             }
-        }
-    } else {
-        int k = k0;
-        for (int dir = 0; dir < 4; ++dir) {
-            int n = face_n(cnt, dir);
-            const bool upper = dir & 1;
-            for (int j = 0; j < n; ++j, ++k) {
-                int side = m.isl[k];
-                float q1 = d.qflux[side];
-                if ((q1 >= 0.0f) == upper) {
-                    m.q[side] = q1 * ds;
-                    if (q1 != 0.0f && ds < 1.0f) faces += 1;
+        } else {
+            int k = k0;
+            for (int dir = 0; dir < 4; ++dir) {
+                int n = face_n(cnt, dir);
+                const bool upper = dir & 1;
+                for (int j = 0; j < n; ++j, ++k) {
+                    int side = m.isl[k];
+                    float q1 = m.q[side];
+                    if ((q1 >= 0.0f) == upper) {
+                        m.q[side] = q1 * ds;
+                        if (q1 != 0.0f && ds < 1.0f) faces += 1;
+                    }
                 }
             }
         }
@@ -804,7 +807,7 @@ struct nhf_ctx {
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Carve {
-    size_t ctl, coeff, values, level, rain, donor, qflux, face_qn, grad, key, idx, acc, total;
+    size_t ctl, coeff, values, level, rain, donor, face_qn, grad, key, idx, acc, total;
 };
 
 static int blocks_for(int n) { return (n + NHF_BLOCK - 1) / NHF_BLOCK; }
@@ -858,7 +861,6 @@ static Carve carve(const nhf_mesh *m, const nhf_params *p) {
     c.level = off; off = align_up(off + sizeof(float) * b);
     c.rain = off; off = align_up(off + sizeof(float) * (m->n_pixels > 0 ? m->n_pixels : 1));
     c.donor = off; off = align_up(off + sizeof(float) * m->e_num);
-    c.qflux = off; off = align_up(off + sizeof(float) * m->s_num);
     c.face_qn = off; off = align_up(off + (theta_one ? 0 : sizeof(float) * 4 * (size_t)m->e_num));
     c.grad = off; off = align_up(off + sizeof(float) * 2 * (m->n_grad > 0 ? m->n_grad : 1));
     c.key = off; off = align_up(off + sizeof(uint64_t) * cfl_blocks);
@@ -909,7 +911,6 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     d.level_values = (float *)(ws + cv.level);
     d.rain_rates = (float *)(ws + cv.rain);
     d.donor = (float *)(ws + cv.donor);
-    d.qflux = (float *)(ws + cv.qflux);
     d.face_qn = params->theta == 1.0f ? nullptr : (float *)(ws + cv.face_qn);
     d.grad = (float *)(ws + cv.grad);
     d.cfl_key = (uint64_t *)(ws + cv.key);

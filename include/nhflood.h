@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 7
+#define NHF_ABI_VERSION 8
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -121,6 +121,32 @@ typedef struct nhf_mesh {
     int32_t reserved_tail;
 } nhf_mesh;
 
+/* Gate control (core/gate.py:35-268; layout preprocess/gate.py:8-96). Arrays are in the library's
+ * numbering and owned by the caller, like nhf_mesh's: device arrays for the HIP library, host
+ * arrays for the CPU oracle. flags, ticks and acc are state the library updates in place. */
+typedef struct nhf_gates {
+    int32_t count;            /* gates */
+    int32_t n_cells;          /* gate cells listed in `cells` */
+    int32_t n_items;          /* accounting items */
+    int32_t n_chunks;         /* accounting groups of at most 256 items */
+    float trigger_level, opening_seconds, open_elevation, min_h; /* F32, as the reference's kernels */
+    uint64_t formal_tick;     /* a gate can be commanded from this clock tick on (the formal start) */
+    const int32_t *control;   /* [count] control cell of each gate */
+    const int32_t *cells;     /* [n_cells] every gate cell */
+    const int32_t *cell_gate; /* [e_num] gate of each cell, -1 elsewhere */
+    const uint8_t *pond;      /* [e_num] 1 for a pond cell */
+    const float *closed;      /* [e_num] the closed bed (the mesh bed) */
+    /* accounting: side ids (bit 31 set: the side's volume counts negated) and cell ids; each
+       chunk [4] = first item, items, kind (0 sides, 1 cells) | (first-flow gate + 1) << 8, first
+       term. Sides add max(v, 0), max(-v, 0) of v = +-q*length*dt to two terms, cells their six
+       source terms (record_flux, record_cell, collect: core/gate.py:119-188). */
+    const int32_t *items;
+    const int32_t *chunks;
+    int32_t *flags;           /* [count*3] commanded, first flow seen, fully open seen */
+    uint64_t *ticks;          /* [count*3] their clock ticks */
+    double *acc;              /* [n_chunks*6*2] per chunk and term: sum, compensation (zeroed) */
+} nhf_gates;
+
 typedef struct nhf_ctx nhf_ctx;
 
 /* Version of this ABI; the loader refuses a library that reports another. */
@@ -186,6 +212,20 @@ int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side);
 /* Wait for all work of the context's stream. */
 int32_t nhf_sync(nhf_ctx *ctx);
 
+/* Bind gate control to the step (NULL: none). Each step then runs, as the reference's graph
+ * (solver_compact.py:894-909): the trigger test and gate-bed motion at its start (check_gates),
+ * the exchange and source accounting after the limiter (record_flux, record_cell, collect), and
+ * the gate-bed motion to its end (finish_gate_bed). The arrays must outlive the binding. */
+int32_t nhf_set_gates(nhf_ctx *ctx, const nhf_gates *gates);
+
+/* The trigger test and bed motion at the current clock, once, outside a batch (the check_gates
+ * call before the first CFL, solver_compact.py:885-889). */
+int32_t nhf_gate_check(nhf_ctx *ctx);
+
+/* Gate budget since the last flush (GateController.budget.flush, core/gate.py:222) as
+ * double-double pairs: out[2*t] + out[2*t+1] for each of the 10*count + 8 terms; cleared. */
+int32_t nhf_flush_gate_budget(nhf_ctx *ctx, double *out);
+
 /* Make max_depth and peak_tick final (the extrema of solver_compact.py:600-606 as
  * reset_extrema/write_extrema see them, :674-679, output.py:130-150). The step leaves a cell
  * whose depth rose in its last step marked by a negative max_depth (its maximum is its depth,
@@ -206,16 +246,17 @@ int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block);
 int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *out4);
 
 /* Per-kernel timing with HIP events on the context's stream (eager mode only). Kernel ids:
- * 0 prepare, 1 gradients, 2 face averages, 3 flux, 4 donor, 5 depth, 6 cfl, 7 finish. `ms` and `count`
- * receive totals since the last call with reset != 0. */
+ * 0 prepare, 1 gradients, 2 face averages, 3 flux, 4 donor, 5 depth, 6 cfl, 7 finish, 8 gate
+ * control. `ms` and `count` receive totals since the last call with reset != 0. */
 int32_t nhf_set_timing(nhf_ctx *ctx, int32_t enable);
 int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);
 
 /* Benchmark hook with no reference entry point: prepare one step towards the begun target, then
  * time each step kernel over `repeats` back-to-back launches with HIP events on the context's
- * stream; us[id] receives the mean per launch (kernel ids as for nhf_kernel_times, 0 for kernels
- * that do not run on this mesh). The launches repeat each pass on the same inputs except the depth
- * pass, which updates the state in place, so the context's state is not meaningful afterwards. */
+ * stream; us[id] (9 entries) receives the mean per launch (kernel ids as for nhf_kernel_times, 0
+ * for kernels that do not run on this mesh, and for gate control). The launches repeat each pass
+ * on the same inputs except the depth pass, which updates the state in place, so the context's
+ * state is not meaningful afterwards. */
 int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us);
 
 /* Test hook with no reference entry point: evaluate StepClock.choose (clock.py:94-167) for n

@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -56,6 +56,15 @@ class BatchState(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in BATCH_STATE_FIELDS] + [('n_bcells', c_int32), ('reserved', c_int32)]
 
 
+class GatesStruct(ctypes.Structure):
+    """nhf_gates: gate control lists and state (device arrays for the HIP library)."""
+    _fields_ = [('count', c_int32), ('n_cells', c_int32), ('n_items', c_int32), ('n_chunks', c_int32),
+                ('trigger_level', c_float), ('opening_seconds', c_float), ('open_elevation', c_float),
+                ('min_h', c_float), ('formal_tick', c_uint64)] + [
+        (name, c_void_p) for name in ('control', 'cells', 'cell_gate', 'pond', 'closed', 'items', 'chunks',
+                                      'flags', 'ticks', 'acc')]
+
+
 def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
     """Declare every entry point of include/nhflood.h on a loaded library."""
     vp = c_void_p
@@ -81,6 +90,9 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_set_tuning': (c_int32, [vp, c_int32, c_int32]),
         'nhf_time_kernels': (c_int32, [vp, c_int32, vp]),
         'nhf_settle_extrema': (c_int32, [vp]),
+        'nhf_set_gates': (c_int32, [vp, POINTER(GatesStruct)]),
+        'nhf_gate_check': (c_int32, [vp]),
+        'nhf_flush_gate_budget': (c_int32, [vp, vp]),
         'nhf_tuning_range': (c_int32, [vp, vp]),
         'nhf_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
         'nhf_clock_probe': (c_int32, [vp, vp, vp, vp, c_float, c_int32, vp, vp, vp]),
@@ -123,6 +135,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
             'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe',
             'nhf_set_tuning', 'nhf_tuning_range', 'nhf_time_kernels', 'nhf_settle_extrema',
+            'nhf_set_gates', 'nhf_gate_check', 'nhf_flush_gate_budget',
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
             'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
             'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times', 'nhb_set_tuning',
@@ -146,6 +159,6 @@ def check(lib, rc: int, what: str) -> None:
         raise RuntimeError(f'{what} failed ({rc}): {lib.nhf_last_error().decode(errors="replace")}')
 
 
-__all__ = ['Params', 'MeshStruct', 'MESH_FIELDS', 'BatchState', 'BATCH_STATE_FIELDS', 'MAX_SLOTS', 'bind', 'load_hip_library', 'check',
+__all__ = ['Params', 'MeshStruct', 'MESH_FIELDS', 'GatesStruct', 'BatchState', 'BATCH_STATE_FIELDS', 'MAX_SLOTS', 'bind', 'load_hip_library', 'check',
            'TRACE_COLS', 'MAX_BATCH', 'DONE', 'COMPLETED', 'BAD_CFL', 'BAD_DT', 'OVERSHOOT',
            'c_double', 'c_int64', 'c_uint32']

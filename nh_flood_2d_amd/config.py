@@ -128,11 +128,37 @@ class RainfallConfig(InputModel):
     intervals: str
 
 
+class GatePointConfig(InputModel):
+    """One gate (input/force.py:32-35): a point inside its gate cells and its control point."""
+    name: str = Field(min_length=1)
+    inside_point: tuple[float, float]
+    control_point: tuple[float, float]
+
+
+class GateConfig(InputModel):
+    """Gate control (input/force.py:38-54), same fields and checks."""
+    land_type: int = Field(ge=0, le=255)
+    pond_land_type: int = Field(ge=0, le=255)
+    enabled: bool
+    trigger_level: float
+    opening_seconds: float = Field(gt=0)
+    open_elevation: float
+    gates: list[GatePointConfig] = Field(min_length=1)
+
+    @model_validator(mode='after')
+    def _distinct(self):
+        if self.pond_land_type == self.land_type:
+            raise ValueError('pond_land_type must differ from gate land_type')
+        names = [g.name for g in self.gates]
+        if len(names) != len(set(names)):
+            raise ValueError('Gate names must be unique')
+        return self
+
+
 class ForceConfig(InputModel):
     boundaries: list[BoundaryConfig]
     rainfall: RainfallConfig
-    # Gate control stays a host concern; a configured gate is rejected by the solver.
-    gate: dict | None = None
+    gate: GateConfig | None = None
 
     @model_validator(mode='after')
     def _unique_names(self):

@@ -43,6 +43,8 @@ struct Dev {
     int32_t terms, cfl_blocks, depth_blocks, theta_one, cfl_max;
     int32_t cell_block;  // workgroup size of the two cell passes (128, 256 or 512)
     int32_t acc_owners;  // budget accumulators allocated (wavefronts of the largest grid)
+    int32_t gates_on;    // gate control bound (nhf_set_gates)
+    nhf_gates g;
 };
 
 // prepare_step (solver_compact.py:364-382), run by one thread.
@@ -468,9 +470,17 @@ struct DepthTerms {
     int bi, kind;
 };
 
-template <bool REGULAR>
+// A gate-accounted cell's six source terms (record_cell and collect, core/gate.py:149-188): rain,
+// infiltration, positivity floor, pipe, boundary exchange, inventory rounding (all m^3, F32).
+struct GateSources {
+    float v[6];
+};
+
+// RECORD: compute the cell's update without storing anything and return its gate source terms
+// (the gate accounting runs this between the donor and depth passes, on the same inputs).
+template <bool REGULAR, bool RECORD = false>
 static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float dt, bool raining, uint64_t step_start,
-                                                  const DepthCell &v, DepthTerms &t) {
+                                                  const DepthCell &v, DepthTerms &t, GateSources *gs = nullptr) {
     const nhf_mesh &m = d.m;
     const float size = m.size_base > 0.0f ? cell_size(m, v.meta, cell) : v.size;
     CellList l;
@@ -586,6 +596,22 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
         w = (qb + qt) / size / hv / 2.0f;
     }
     float speed = sqrtf(u * u + w * w);
+    if constexpr (RECORD) {
+        // record_cell / collect (core/gate.py:149-188; solver_compact.py:595-603), in F32
+        const float applied_floor = kind == 0 ? 0.0f : floor_v;
+        float level_exchange = 0.0f;
+        if (kind == 0)
+            level_exchange = (h_new - previous) * area - (ql - qr + qb - qt) * dt - pipe * dt - rain_depth * area +
+                             removed * area;
+        const float rounding = (h_new - previous) - increment + removed - applied_floor;
+        gs->v[0] = rain_depth * area;
+        gs->v[1] = removed * area;
+        gs->v[2] = applied_floor * area;
+        gs->v[3] = pipe * dt;
+        gs->v[4] = (ssq - pipe) * dt + level_exchange;
+        gs->v[5] = rounding * area - level_exchange;
+        return;
+    }
     // stores last
     if (m.pipe_volume) {
         m.pipe_correction[cell] = (pv_total - v.pv_before) - pv_increment;
@@ -611,6 +637,24 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
     }
 }
 
+// First round of the depth pass: every load that needs only the cell index.
+static __device__ __forceinline__ void load_depth_cell(const nhf_mesh &m, int cell, bool raining, DepthCell &v) {
+    v.meta = m.cell_meta[cell];
+    v.h = m.depth[cell];
+    v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
+    v.cap = m.capacity[cell];
+    v.max_d = m.max_depth[cell];
+    v.max_s = m.max_speed[cell];
+    v.rain_time = raining ? m.rain_time[cell] : 0.0f;
+    v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
+    v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+    if (m.pipe_volume) {
+        v.pv_correction = m.pipe_correction[cell];
+        v.pv_before = m.pipe_volume[cell];
+    }
+    v.rec = cell_record(m, cell);
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     Ctl *c = d.ctl;
@@ -626,22 +670,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     double2 *acc = d.acc + ((size_t)blockIdx.x * (BLOCK / 64) + wave) * d.terms;
     // first round: the cell's own values and its side record
     DepthCell v = {};
-    if (live) {
-        v.meta = m.cell_meta[cell];
-        v.h = m.depth[cell];
-        v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
-        v.cap = m.capacity[cell];
-        v.max_d = m.max_depth[cell];
-        v.max_s = m.max_speed[cell];
-        v.rain_time = raining ? m.rain_time[cell] : 0.0f;
-        v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
-        v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
-        if (m.pipe_volume) {
-            v.pv_correction = m.pipe_correction[cell];
-            v.pv_before = m.pipe_volume[cell];
-        }
-        v.rec = cell_record(m, cell);
-    }
+    if (live) load_depth_cell(m, cell, raining, v);
     DepthTerms t = {};
     t.bi = -1;
     t.kind = -1;
@@ -689,6 +718,107 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
         }
     }
     if (touched) acc[lane] = held;
+}
+
+// --------------------------------------------------------------------------- gate control
+
+// check_gates / finish_gate_bed (core/gate.py:77-101), one workgroup. At a step start (at_end 0)
+// a gate not yet commanded is commanded when the formal start is reached and its control cell is
+// wet (depth >= min_h) with a level above the trigger; then every commanded gate's cells take the
+// bed of the opening fraction at the step's start or end tick. `standalone` runs it outside a
+// batch (the clock need not be active).
+__global__ __launch_bounds__(1024) void k_gate_move(Dev d, int at_end, int standalone) {
+    const Ctl *c = d.ctl;
+    if (!standalone && !c->active) return;
+    const nhf_gates &g = d.g;
+    const uint64_t tick = at_end ? c->step_end : c->now;
+    if (!at_end) {
+        for (int k = threadIdx.x; k < g.count; k += blockDim.x) {
+            const int cell = g.control[k];
+            const float h = d.m.depth[cell];
+            if (tick >= g.formal_tick && g.flags[3 * k] == 0 && h >= g.min_h && d.m.bed[cell] + h > g.trigger_level) {
+                g.flags[3 * k] = 1;
+                g.ticks[3 * k] = tick;
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t hi = (uint32_t)(tick >> 32), lo = (uint32_t)tick;
+    for (int i = threadIdx.x; i < g.n_cells; i += blockDim.x) {
+        const int cell = g.cells[i];
+        const int k = g.cell_gate[cell];
+        if (g.flags[3 * k] != 1) continue;
+        // since_command (core/gate.py:66-70) on the tick's 32-bit words
+        const uint64_t command = g.ticks[3 * k];
+        const uint32_t chi = (uint32_t)(command >> 32), clo = (uint32_t)command;
+        const uint32_t borrow = lo < clo ? 1u : 0u;
+        const uint32_t high = hi - chi - borrow, low = lo - clo;
+        const float since = (float)high / 256.0f + (float)low * 0x1p-40f;
+        const float fraction = fmn(since / g.opening_seconds, 1.0f);
+        if (fraction >= 1.0f) {
+            d.m.bed[cell] = g.open_elevation;
+            if (atomicOr(&g.flags[3 * k + 2], 1) == 0) g.ticks[3 * k + 2] = tick;
+        } else {
+            const float closed = g.closed[cell];
+            d.m.bed[cell] = closed + (g.open_elevation - closed) * fraction;
+        }
+    }
+}
+
+// Gate accounting of one step, between the donor and depth passes: one workgroup per chunk of
+// one term group. A side item adds its post-limiter volume into/out of its two terms (and marks
+// the gate's first pond flow); a cell item computes the cell's update as k_depth will and adds
+// its six source terms. Sums are FP64 in a fixed order per chunk, then one compensated
+// accumulator per chunk and term.
+__global__ __launch_bounds__(NHF_BLOCK) void k_gate_account(Dev d) {
+    const Ctl *c = d.ctl;
+    if (!c->active) return;
+    const nhf_gates &g = d.g;
+    const nhf_mesh &m = d.m;
+    const int4 ch = reinterpret_cast<const int4 *>(g.chunks)[blockIdx.x];
+    const int kind = ch.z & 255, flow_gate = (ch.z >> 8) - 1, width = kind == 0 ? 2 : 6;
+    const float dt = c->dt;
+    float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if ((int)threadIdx.x < ch.y) {
+        const uint32_t code = (uint32_t)g.items[ch.x + threadIdx.x];
+        if (kind == 0) {
+            const int side = (int)(code & 0x7fffffffu);
+            const float volume = m.q[side] * m.length[side] * dt;
+            const float into = (code >> 31) ? -volume : volume;
+            v[0] = fmx(into, 0.0f);
+            v[1] = fmx(-into, 0.0f);
+            if (flow_gate >= 0 && volume != 0.0f && atomicOr(&g.flags[3 * flow_gate + 1], 1) == 0)
+                g.ticks[3 * flow_gate + 1] = c->step_end;
+        } else {
+            const int cell = (int)code;
+            const bool raining = c->raining;
+            DepthCell cv = {};
+            load_depth_cell(m, cell, raining, cv);
+            DepthTerms t = {};
+            GateSources gs;
+            if (cv.rec.x >= 0)
+                depth_cell<true, true>(d, cell, dt, raining, c->now, cv, t, &gs);
+            else
+                depth_cell<false, true>(d, cell, dt, raining, c->now, cv, t, &gs);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = gs.v[k];
+        }
+    }
+    __shared__ double s_part[NHF_WAVES][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < width; ++k) {
+        const double w = wave_sum((double)v[k]);
+        if (lane == 0) s_part[wave][k] = w;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < width) {
+        double sum = 0.0;
+        for (int w = 0; w < NHF_WAVES; ++w) sum += s_part[w][threadIdx.x];
+        double2 *acc = reinterpret_cast<double2 *>(g.acc) + (size_t)blockIdx.x * 6 + threadIdx.x;
+        double2 a = *acc;
+        acc_add(a, sum);
+        *acc = a;
+    }
 }
 
 // --------------------------------------------------------------------------- (c) CFL dt
@@ -786,7 +916,7 @@ __global__ __launch_bounds__(1024) void k_finish(Dev d, int mode, int slot, int 
 
 // --------------------------------------------------------------------------- host side
 
-enum { K_PREPARE = 0, K_GRAD, K_FACE, K_FLUX, K_DONOR, K_DEPTH, K_CFL, K_FINISH, K_COUNT };
+enum { K_PREPARE = 0, K_GRAD, K_FACE, K_FLUX, K_DONOR, K_DEPTH, K_CFL, K_FINISH, K_GATE, K_COUNT };
 
 struct nhf_ctx {
     Dev d;
@@ -800,6 +930,7 @@ struct nhf_ctx {
     uint32_t *host_trace;
     double2 *host_acc;
     Ctl *probe_scratch;
+    std::vector<int32_t> gate_chunks;  // host copy of the gate accounting chunks
     hipEvent_t done;    // recorded after a launched batch's trace copy
     int pending_steps;  // steps of the batch in flight, 0 when none
 };
@@ -1019,19 +1150,25 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
     const int cells = d.m.e_num - 1, sides = d.m.s_num - 1;
     LAUNCH(K_PREPARE, k_prepare, dim3(1), dim3(64), 0, s, d, 0);
     for (int slot = 0; slot < steps; ++slot) {
+        if (d.gates_on) LAUNCH(K_GATE, k_gate_move, dim3(1), dim3(1024), 0, s, d, 0, 0);
         if (d.m.n_grad > 0) LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
         if (!d.theta_one) LAUNCH(K_FACE, k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
         if (sides > 0) LAUNCH(K_FLUX, k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d);
-        if (d.cell_block == 128) {
+        if (d.cell_block == 128)
             LAUNCH(K_DONOR, k_donor<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
-            LAUNCH(K_DEPTH, k_depth<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
-        } else if (d.cell_block == 512) {
+        else if (d.cell_block == 512)
             LAUNCH(K_DONOR, k_donor<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
-            LAUNCH(K_DEPTH, k_depth<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
-        } else {
+        else
             LAUNCH(K_DONOR, k_donor<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
+        if (d.gates_on && d.g.n_chunks > 0)
+            LAUNCH(K_GATE, k_gate_account, dim3(d.g.n_chunks), dim3(NHF_BLOCK), 0, s, d);
+        if (d.cell_block == 128)
+            LAUNCH(K_DEPTH, k_depth<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
+        else if (d.cell_block == 512)
+            LAUNCH(K_DEPTH, k_depth<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
+        else
             LAUNCH(K_DEPTH, k_depth<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
-        }
+        if (d.gates_on) LAUNCH(K_GATE, k_gate_move, dim3(1), dim3(1024), 0, s, d, 1, 0);
         LAUNCH(K_CFL, k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
         LAUNCH(K_FINISH, k_finish, dim3(1), dim3(1024), 0, s, d, 1, slot, steps);
     }
@@ -1102,7 +1239,9 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
     hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, s, d, 0);
     const int sides = d.m.s_num - 1;
     for (int k = K_GRAD; k < K_COUNT; ++k) {
-        if ((k == K_GRAD && d.m.n_grad == 0) || (k == K_FACE && d.theta_one) || (k == K_FLUX && sides <= 0)) continue;
+        if ((k == K_GRAD && d.m.n_grad == 0) || (k == K_FACE && d.theta_one) || (k == K_FLUX && sides <= 0) ||
+            k == K_GATE)
+            continue;
         HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < repeats; ++r) {
             switch (k) {
@@ -1199,6 +1338,103 @@ int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side) {
 
 int32_t nhf_sync(nhf_ctx *ctx) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int32_t nhf_set_gates(nhf_ctx *ctx, const nhf_gates *gates) {
+    if (ctx->pending_steps) return set_error_text("nhf_set_gates: a batch is in flight");
+    Dev &d = ctx->d;
+    if (gates) {
+        const nhf_gates &g = *gates;
+        if (g.count < 1 || g.n_cells < 0 || g.n_items < 0 || g.n_chunks < 0 || !(g.opening_seconds > 0.0f))
+            return set_error_text("nhf_set_gates: invalid counts or opening time");
+        if (!g.control || !g.cell_gate || !g.pond || !g.closed || !g.flags || !g.ticks ||
+            (g.n_cells && !g.cells) || (g.n_chunks && (!g.items || !g.chunks || !g.acc)))
+            return set_error_text("nhf_set_gates: an array is NULL");
+        std::vector<int32_t> chunks((size_t)g.n_chunks * 4);
+        if (g.n_chunks)
+            HIP_TRY(hipMemcpy(chunks.data(), g.chunks, sizeof(int32_t) * chunks.size(), hipMemcpyDeviceToHost));
+        for (int c = 0; c < g.n_chunks; ++c) {
+            const int32_t *ch = &chunks[4 * (size_t)c];
+            const int kind = ch[2] & 255, width = kind == 0 ? 2 : 6;
+            if (ch[0] < 0 || ch[1] < 1 || ch[1] > NHF_BLOCK || ch[0] + ch[1] > g.n_items || kind > 1 ||
+                (ch[2] >> 8) - 1 >= g.count || ch[3] < 0 || ch[3] + width > 10 * g.count + 8)
+                return set_error_text("nhf_set_gates: accounting chunk out of range");
+        }
+        // every index the kernels follow is checked once here, on host copies
+        const int e_num = d.m.e_num, s_num = d.m.s_num;
+        std::vector<int32_t> control(g.count), cells(g.n_cells), cell_gate(e_num), items(g.n_items);
+        HIP_TRY(hipMemcpy(control.data(), g.control, sizeof(int32_t) * g.count, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(cell_gate.data(), g.cell_gate, sizeof(int32_t) * e_num, hipMemcpyDeviceToHost));
+        if (g.n_cells) HIP_TRY(hipMemcpy(cells.data(), g.cells, sizeof(int32_t) * g.n_cells, hipMemcpyDeviceToHost));
+        if (g.n_items) HIP_TRY(hipMemcpy(items.data(), g.items, sizeof(int32_t) * g.n_items, hipMemcpyDeviceToHost));
+        for (int k = 0; k < g.count; ++k)
+            if (control[k] < 1 || control[k] >= e_num) return set_error_text("nhf_set_gates: control cell out of range");
+        for (int i = 0; i < g.n_cells; ++i)
+            if (cells[i] < 1 || cells[i] >= e_num || cell_gate[cells[i]] < 0 || cell_gate[cells[i]] >= g.count)
+                return set_error_text("nhf_set_gates: gate cell out of range or without a gate");
+        for (int c = 0; c < g.n_chunks; ++c) {
+            const int32_t *ch = &chunks[4 * (size_t)c];
+            for (int j = ch[0]; j < ch[0] + ch[1]; ++j) {
+                const int id = (ch[2] & 255) == 0 ? (int)((uint32_t)items[j] & 0x7fffffffu) : items[j];
+                if (id < 1 || id >= ((ch[2] & 255) == 0 ? s_num : e_num))
+                    return set_error_text("nhf_set_gates: accounting item out of range");
+            }
+        }
+        ctx->gate_chunks.swap(chunks);
+        d.g = g;
+    } else {
+        ctx->gate_chunks.clear();
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i <= NHF_MAX_BATCH; ++i)
+        if (ctx->exec[i]) {
+            (void)hipGraphExecDestroy(ctx->exec[i]);
+            ctx->exec[i] = nullptr;
+        }
+    d.gates_on = gates != nullptr;
+    return 0;
+}
+
+int32_t nhf_gate_check(nhf_ctx *ctx) {
+    if (!ctx->d.gates_on) return set_error_text("nhf_gate_check: no gates bound");
+    if (ctx->pending_steps) return set_error_text("nhf_gate_check: a batch is in flight");
+    hipLaunchKernelGGL(k_gate_move, dim3(1), dim3(1024), 0, ctx->stream, ctx->d, 0, 1);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhf_flush_gate_budget(nhf_ctx *ctx, double *out) {
+    const Dev &d = ctx->d;
+    if (!d.gates_on) return set_error_text("nhf_flush_gate_budget: no gates bound");
+    if (ctx->pending_steps) return set_error_text("nhf_flush_gate_budget: a batch is in flight");
+    const int terms = 10 * d.g.count + 8, n = d.g.n_chunks;
+    std::vector<double> acc((size_t)n * 12);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) {
+        HIP_TRY(hipMemcpy(acc.data(), d.g.acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemsetAsync(d.g.acc, 0, sizeof(double) * acc.size(), ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    // per term, the chunks' compensated pairs added in chunk order (double-double)
+    std::vector<double> hi(terms, 0.0), lo(terms, 0.0);
+    for (int c = 0; c < n; ++c) {
+        const int32_t *ch = &ctx->gate_chunks[4 * (size_t)c];
+        const int width = (ch[2] & 255) == 0 ? 2 : 6;
+        for (int k = 0; k < width; ++k) {
+            const int t = ch[3] + k;
+            for (int part = 0; part < 2; ++part) {
+                const double x = acc[(size_t)c * 12 + 2 * k + part];
+                const double s = hi[t] + x, bp = s - hi[t];
+                lo[t] += (hi[t] - (s - bp)) + (x - bp);
+                hi[t] = s;
+            }
+        }
+    }
+    for (int t = 0; t < terms; ++t) {
+        out[2 * t] = hi[t];
+        out[2 * t + 1] = lo[t];
+    }
     return 0;
 }
 

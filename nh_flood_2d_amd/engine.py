@@ -182,6 +182,48 @@ class Engine:
             values = self.layout.cells_to_device(values)
         self.device.write(self.buffers[name], values)
 
+    # ----------------------------------------------------------------- gate control
+    def set_gates(self, spec: dict) -> None:
+        """Bind gate control (nhf_set_gates): lists in device order from gate.GateController, fresh
+        flags, ticks and budget accumulators."""
+        dev = self.device
+        count = int(spec['count'])
+        chunks = np.ascontiguousarray(spec['chunks'], np.int32).reshape(-1, 4)
+        bufs = {name: dev.array(np.ascontiguousarray(spec[name], dtype))
+                for name, dtype in (('control', np.int32), ('cells', np.int32), ('cell_gate', np.int32),
+                                    ('pond', np.uint8), ('closed', np.float32), ('items', np.int32))}
+        bufs['chunks'] = dev.array(chunks.ravel())
+        bufs['flags'] = dev.array(np.zeros(3 * count, np.int32))
+        bufs['ticks'] = dev.array(np.zeros(3 * count, np.uint64))
+        bufs['acc'] = dev.array(np.zeros(max(1, len(chunks)) * 12, np.float64))
+        g = abi.GatesStruct(count=count, n_cells=len(spec['cells']), n_items=len(spec['items']),
+                            n_chunks=len(chunks), trigger_level=np.float32(spec['trigger_level']),
+                            opening_seconds=np.float32(spec['opening_seconds']),
+                            open_elevation=np.float32(spec['open_elevation']), min_h=np.float32(spec['min_h']),
+                            formal_tick=int(spec['formal_tick']))
+        for name, buf in bufs.items():
+            setattr(g, name, dev.ptr(buf))
+        self.gate_buffers, self.gate_struct, self.gate_count = bufs, g, count
+        abi.check(self.lib, self.lib.nhf_set_gates(self.ctx, ctypes.byref(g)), 'nhf_set_gates')
+
+    def gate_check(self) -> None:
+        abi.check(self.lib, self.lib.nhf_gate_check(self.ctx), 'nhf_gate_check')
+
+    def gate_state(self):
+        """(flags [count, 3] commanded/flow seen/opened, ticks [count, 3] their clock ticks)."""
+        flags = self.device.read(self.gate_buffers['flags'], np.int32).reshape(-1, 3).copy()
+        ticks = self.device.read(self.gate_buffers['ticks'], np.uint64).reshape(-1, 3).copy()
+        return flags, ticks
+
+    def set_gate_state(self, flags: np.ndarray, ticks: np.ndarray) -> None:
+        self.device.write(self.gate_buffers['flags'], np.asarray(flags, np.int32).ravel())
+        self.device.write(self.gate_buffers['ticks'], np.asarray(ticks, np.uint64).ravel())
+
+    def flush_gate_budget(self) -> np.ndarray:
+        out = np.zeros(2 * (10 * self.gate_count + 8), np.float64)
+        abi.check(self.lib, self.lib.nhf_flush_gate_budget(self.ctx, out.ctypes.data), 'nhf_flush_gate_budget')
+        return out.reshape(-1, 2)
+
     # ----------------------------------------------------------------- pipe coupling
     def set_pipe_q(self, rates: np.ndarray) -> None:
         """Per-cell pipe source rates (m^3/s, reference numbering) for the next coupling window."""
@@ -261,7 +303,7 @@ class Engine:
     def time_kernels(self, repeats: int = 20) -> np.ndarray:
         """Mean microseconds per launch of each step kernel (ids as kernel_times) over `repeats`
         back-to-back launches; leaves the state meaningless (benchmark use)."""
-        us = np.zeros(8, dtype=np.float64)
+        us = np.zeros(9, dtype=np.float64)
         abi.check(self.lib, self.lib.nhf_time_kernels(self.ctx, int(repeats), us.ctypes.data), 'nhf_time_kernels')
         return us
 

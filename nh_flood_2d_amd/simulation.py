@@ -160,7 +160,7 @@ class Simulation:
 
     def __init__(self, mesh: Mesh, land_parameters: dict, forcing: Forcing, settings: RunSettings,
                  checkpoint=None, engine_factory=None, outputs=None, record: dict | None = None,
-                 initial_depth: np.ndarray | None = None, base_layout=None, coupling=None):
+                 initial_depth: np.ndarray | None = None, base_layout=None, coupling=None, gates=None):
         self.mesh = mesh
         self.forcing = forcing
         self.settings = s = settings
@@ -261,6 +261,16 @@ class Simulation:
         else:
             self.engine = (engine_factory or Engine)(self.layout, self.params, state)
         self.engine.reset(self.host_now_ticks)
+        self.gates = gates
+        if gates is not None:
+            # the controller's device lists and state, then check_gates before the first CFL
+            # (solver_compact.py:885-889); a continuation restores the controller first (:723-724)
+            gates.bind(self.engine, self.layout, mesh.z, s.min_h)
+            if self.continuation:
+                gates.restore(meta['gate_controller'])
+            if gates.initial_storage is None:
+                gates.initialize_inventory(state['depth'], area)
+            self.engine.gate_check()
         self.engine.compute_cfl()
         self.steps = self.warmup_steps = 0
         self.previous_steps = meta.get('total_steps', 0) if self.continuation else 0
@@ -326,6 +336,8 @@ class Simulation:
         depth = self.engine.read_state('depth')
         storage = float(np.sum(depth.astype(float) * self.area, dtype=np.float64))
         row = self.budget.output(self.time_text(seconds), seconds, storage, phase)
+        if self.gates is not None:
+            self.record['gate_control'] = self.gates.report(depth, self.area, seconds, self.mesh.z)
         limits, ids = self.engine.limit_capture(reset=True)
         regular, short = np.array(self.interval_dts), np.array(self.interval_event_dts)
         stats = dict(time=self.time_text(seconds), elapsed_seconds=seconds,

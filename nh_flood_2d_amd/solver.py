@@ -26,6 +26,7 @@ from . import __version__
 from .clock import seconds_to_ticks, TICKS_PER_SECOND
 from .config import DomainConfig, ForceConfig
 from .forcing import load_forcing
+from .gate import GateController, append_gate_balance, load_gate_layout
 from .mesh import Mesh, filter_rows, compact_mesh
 from .output import (append_station_samples, read_checkpoint, resolve_station_cells, save_tables,
                      load_tables, write_bed, write_checkpoint, write_depth_frame, write_extrema,
@@ -189,6 +190,9 @@ class FileOutputs:
             self.budget_writer.writeheader()
         self.budget_writer.writerow(row)
         self.budget_file.flush()
+        if sim.gates is not None:
+            append_gate_balance(Path(self.cfg.domain_dir) / 'gate_balance.csv',
+                                self.cfg.start + timedelta(seconds=seconds), seconds, sim.record['gate_control'])
         if self.stations:
             q = sim.engine.read_state('q')
             bed = sim.engine.read_state('bed')
@@ -225,6 +229,8 @@ class FileOutputs:
                                 'requested_discharge_m3': sim.budget.discharge.tolist(),
                                 'device_units': [int(round(t * 2 ** 149)) for t in units.total],
                                 'device_terms_exact': units.state()})
+        if sim.gates is not None:
+            metadata['gate_controller'] = sim.gates.snapshot()
         state = sim.state()
         tick = seconds_to_ticks(sim.host_now_seconds)
         peak = state['peak_tick']
@@ -233,7 +239,8 @@ class FileOutputs:
                          first_time_hi=(peak >> np.uint64(32)).astype(np.uint32),
                          first_time_lo=(peak & np.uint64(0xffffffff)).astype(np.uint32),
                          max_speed=state['max_speed'], clock_hi=tick >> 32, clock_lo=tick & 0xffffffff,
-                         gate_open=False, metadata_json=json.dumps(metadata))
+                         gate_open=sim.gates is not None and sim.gates.any_commanded(),
+                         metadata_json=json.dumps(metadata))
         self.last_checkpoint_wall = perf_counter()
 
     def finish(self, sim: Simulation):
@@ -243,7 +250,7 @@ class FileOutputs:
                       (peak & np.uint64(0xffffffff)).astype(np.uint32), state['max_speed'])
         depth = state['depth']
         self.record.data.update(
-            gate_open=False,
+            gate_open=sim.gates is not None and sim.gates.any_commanded(),
             storage_depth_one_ulp_volume_m3=float(np.sum(np.abs(np.spacing(depth[1:])) * sim.area[1:],
                                                          dtype=np.float64)))
 
@@ -288,8 +295,6 @@ def _device_name(factory) -> str:
 
 def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, pipe_cfg=None,
            end_time=None):
-    if force_cfg.gate is not None and force_cfg.gate.get('enabled', False):
-        raise NotImplementedError('gate control is not part of this build')
     end_time = float(domain_cfg.duration if end_time is None else end_time)
     factory = ENGINE_FACTORY
     record.device(_device_name(factory))
@@ -300,7 +305,7 @@ def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, 
     continuation = bool(meta.get('run_state', False))
     identity = {'ne_sha256': record.data['input_sha256'][str(Path(domain_cfg.ne).resolve())],
                 'ns_sha256': record.data['input_sha256'][str(Path(domain_cfg.ns).resolve())],
-                'gate_config': 'null'}
+                'gate_config': force_cfg.gate.model_dump_json() if force_cfg.gate else 'null'}
     boundary_identity = [dict(name=b.name, kind=b.kind, selected_edges_sha256=hashlib.sha256(
         np.sort(b.side_ids).astype('<i8').tobytes()).hexdigest()) for b in forcing.boundaries]
     if state:
@@ -322,6 +327,13 @@ def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, 
     origin_before_start = meta['clock_origin_before_start_s'] if continuation else domain_cfg.warmup_duration
     if state is not None and ((state.clock_hi << 32) | state.clock_lo) > seconds_to_ticks(end_time + origin_before_start):
         raise ValueError('Checkpoint time is later than the requested simulation end')
+    gate_layout = (load_gate_layout(force_cfg.gate, mesh, forcing.boundaries, mesh.z)
+                   if force_cfg.gate is not None else None)   # checked even when disabled
+    record.data['gate_cell_count'] = int(np.sum(gate_layout.cell_gate >= 0)) if gate_layout else 0
+    gates = (GateController(force_cfg.gate, gate_layout, origin_before_start)
+             if gate_layout is not None and force_cfg.gate.enabled else None)
+    if gates is not None and pipe_cfg is not None:
+        raise ValueError('gate control together with pipe coupling is not supported')
     stations = resolve_station_cells(domain_cfg.stations, mesh)
     settings = RunSettings(afa=domain_cfg.afa, sita=domain_cfg.sita, min_h=domain_cfg.min_h, end_time=end_time,
                            warmup_duration=float(domain_cfg.warmup_duration), yield_step=float(domain_cfg.yield_step),
@@ -349,7 +361,8 @@ def _solve(domain_cfg: DomainConfig, force_cfg: ForceConfig, record: RunRecord, 
             coupling = PipeCoupling(topology, pipe_cfg.coupling_interval, float(domain_cfg.warmup_duration),
                                     float(domain_cfg.duration), actor, domain_cfg.start.isoformat())
         sim = Simulation(mesh, domain_cfg.land_parameters, forcing, settings, checkpoint=state,
-                         engine_factory=factory, outputs=outputs, record=record.data, coupling=coupling)
+                         engine_factory=factory, outputs=outputs, record=record.data, coupling=coupling,
+                         gates=gates)
         record.data['budget_accumulator']['interval_batches'] = settings.batch_size
         setup = perf_counter() - started
         result = sim.run()

@@ -153,6 +153,12 @@ struct nhf_ctx {
     real *donor, *face_qn, *gradients, *cfl, *flow_depth;
     real *eq, *ssq, *increment, *rain_depth, *diag;
     ddsum *budget;
+    /* gate control (core/gate.py), when bound */
+    int gates_on;
+    nhf_gates g;
+    ddsum *gate_budget;  /* [10*count + 8] */
+    real *gate_volume;   /* [s_num] post-limiter q*length*dt of the step */
+    real *gate_sources;  /* [e_num*6] record_cell + collect values of the step */
 };
 
 int32_t nhf_abi_version(void) { return NHF_ABI_VERSION; }
@@ -272,7 +278,7 @@ int32_t nhf_destroy(nhf_ctx *c) {
     void *ptrs[] = {c->coefficients, c->values, c->level_values, c->rain_rates, c->depth, c->bed,
                     c->rain_time, c->capacity, c->max_depth, c->max_speed, c->q, c->donor,
                     c->face_qn, c->gradients, c->cfl, c->flow_depth, c->eq, c->ssq, c->increment,
-                    c->rain_depth, c->diag, c->budget};
+                    c->rain_depth, c->diag, c->budget, c->gate_budget, c->gate_volume, c->gate_sources};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(c);
     return 0;
@@ -585,6 +591,102 @@ static real boundary_neighbor_sum(real q, real left, real right, real left_ext, 
     return result;
 }
 
+/* ---------------------------------------------------------------- gate control (core/gate.py) */
+
+static int gate_source_offset(const nhf_ctx *c, int cell) {
+    const int count = c->g.count;
+    if (c->g.pond[cell]) return 4 * count + 2;
+    if (c->g.cell_gate[cell] >= 0) return 4 * count + 8 + 6 * c->g.cell_gate[cell];
+    return -1;
+}
+
+/* GateController.move (core/gate.py:72-86) at `tick`. */
+static void gate_move(nhf_ctx *c, uint64_t tick) {
+    const nhf_gates *g = &c->g;
+    const uint32_t hi = (uint32_t)(tick >> 32), lo = (uint32_t)tick;
+    for (int i = 0; i < g->n_cells; ++i) {
+        int cell = g->cells[i], k = g->cell_gate[cell];
+        if (g->flags[3 * k] != 1) continue;
+        uint32_t chi = (uint32_t)(g->ticks[3 * k] >> 32), clo = (uint32_t)g->ticks[3 * k];
+        uint32_t borrow = lo < clo ? 1u : 0u;
+        uint32_t high = hi - chi - borrow, low = lo - clo;
+        float since = (float)high / 256.0f + (float)low * 0x1p-40f;
+        float fraction = since / g->opening_seconds;
+        if (!(fraction < 1.0f)) fraction = 1.0f;
+        if (fraction >= 1.0f) {
+            c->bed[cell] = (real)g->open_elevation;
+            if (!g->flags[3 * k + 2]) {
+                g->flags[3 * k + 2] = 1;
+                g->ticks[3 * k + 2] = tick;
+            }
+        } else {
+            float closed = g->closed[cell];
+            c->bed[cell] = (real)(closed + (g->open_elevation - closed) * fraction);
+        }
+    }
+}
+
+/* GateController.check (core/gate.py:88-98): the trigger test at the step start, then move. */
+static void gate_check(nhf_ctx *c) {
+    const nhf_gates *g = &c->g;
+    for (int k = 0; k < g->count; ++k) {
+        int cell = g->control[k];
+        if (c->now >= g->formal_tick && g->flags[3 * k] == 0 && c->depth[cell] >= (real)g->min_h &&
+            c->bed[cell] + c->depth[cell] > (real)g->trigger_level) {
+            g->flags[3 * k] = 1;
+            g->ticks[3 * k] = c->now;
+        }
+    }
+    gate_move(c, c->now);
+}
+
+static void gate_directed(nhf_ctx *c, int term, real into) {
+    dd_add(&c->gate_budget[term], (double)rmax(into, 0));
+    dd_add(&c->gate_budget[term + 1], (double)rmax(-into, 0));
+}
+
+/* record_flux (core/gate.py:115-139) for every side, in side order, after the limiter. */
+static void gate_record_sides(nhf_ctx *c) {
+    const nhf_gates *g = &c->g;
+    const int offset = 4 * g->count;
+    for (int si = 1; si < c->m.s_num; ++si) {
+        int el = c->m.side_cells[2 * si], eh = c->m.side_cells[2 * si + 1];
+        int gl = g->cell_gate[el], gr = g->cell_gate[eh], pl = g->pond[el], pr = g->pond[eh];
+        real volume = c->gate_volume[si];
+        int flow_gate = -1;
+        if (gl >= 0 && gr < 0) {
+            if (pr) {
+                gate_directed(c, 4 * gl, volume);
+                flow_gate = gl;
+            } else {
+                gate_directed(c, 4 * gl + 2, -volume);
+            }
+        } else if (gr >= 0 && gl < 0) {
+            if (pl) {
+                gate_directed(c, 4 * gr, -volume);
+                flow_gate = gr;
+            } else {
+                gate_directed(c, 4 * gr + 2, volume);
+            }
+        } else if (pl != pr) {
+            gate_directed(c, offset, pr ? volume : -volume);
+        }
+        if (flow_gate >= 0 && volume != 0 && !g->flags[3 * flow_gate + 1]) {
+            g->flags[3 * flow_gate + 1] = 1;
+            g->ticks[3 * flow_gate + 1] = c->step_end;
+        }
+    }
+}
+
+/* record_cell + measure_inventory + collect (core/gate.py:149-188), in cell order. */
+static void gate_record_cells(nhf_ctx *c) {
+    for (int ei = 1; ei < c->m.e_num; ++ei) {
+        int offset = gate_source_offset(c, ei);
+        if (offset < 0) continue;
+        for (int k = 0; k < 6; ++k) dd_add(&c->gate_budget[offset + k], (double)c->gate_sources[ei * 6 + k]);
+    }
+}
+
 static void tick(nhf_ctx *c) {
     const nhf_mesh *m = &c->m;
     const real dt = c->dt;
@@ -674,8 +776,10 @@ static void tick(nhf_ctx *c) {
         int donor = c->q[si] >= 0 ? el : eh;
         if (c->q[si] != 0 && c->donor[donor] < 1) limited_faces += 1;
         c->q[si] *= c->donor[donor];
+        if (c->gates_on) c->gate_volume[si] = c->q[si] * (real)m->length[si] * dt;
     }
     c->limited_faces += limited_faces;
+    if (c->gates_on) gate_record_sides(c);
 
     /* sources, Horton capacity and face totals (:525-566) */
 #pragma omp parallel for schedule(static) if (e_num > OMP_MIN_ITEMS)
@@ -762,6 +866,24 @@ static void tick(nhf_ctx *c) {
             dg[1] = floor_ * ea;
             c->depth[ei] = rmax(raw, 0);
         }
+        if (c->gates_on && gate_source_offset(c, ei) >= 0) {
+            /* record_cell (solver_compact.py:595-603, core/gate.py:149-163) and collect (:179-188) */
+            int level = bi >= 0 && m->kind[bi] == 0;
+            real applied_floor = level ? (real)0.0 : floor_;
+            real level_exchange = 0;
+            if (level)
+                level_exchange = (c->depth[ei] - previous) * ea - (ql - qr + qb - qt) * dt - pipe * dt -
+                                 rain_depth * ea + removed * ea;
+            real delta = c->depth[ei] - previous;
+            real rounding = delta - c->increment[ei] + removed - applied_floor;
+            real *gs = &c->gate_sources[ei * 6];
+            gs[0] = rain_depth * ea;
+            gs[1] = removed * ea;
+            gs[2] = applied_floor * ea;
+            gs[3] = pipe * dt;
+            gs[4] = (c->ssq[ei] - pipe) * dt + level_exchange;
+            gs[5] = rounding * ea - level_exchange;
+        }
         real h = rmax(c->depth[ei], (real)0.01);
         real u = 0, v = 0;
         if (c->depth[ei] >= min_h) {
@@ -774,6 +896,7 @@ static void tick(nhf_ctx *c) {
             m->peak_tick[ei] = step_end;
         }
     }
+    if (c->gates_on) gate_record_cells(c);
     /* budget terms, added per cell in cell order (:578-590) */
     for (int ei = 1; ei < e_num; ++ei) {
         const real *dg = &c->diag[ei * 8];
@@ -836,7 +959,7 @@ static void finish_step(nhf_ctx *c) {
 
 /* Kernel timing has no meaning on the CPU: reports zeros. */
 int32_t nhf_time_kernels(nhf_ctx *c, int32_t repeats, double *us) {
-    for (int i = 0; i < 8; ++i) us[i] = 0.0;
+    for (int i = 0; i < 9; ++i) us[i] = 0.0;
     return 0;
 }
 
@@ -873,8 +996,10 @@ int32_t nhf_run_batch(nhf_ctx *c, int32_t steps, uint32_t *trace) {
     for (int s = 0; s < steps; ++s) {
         prepare_step(c);
         if (c->active) {
+            if (c->gates_on) gate_check(c);
             if (c->m.n_grad > 0) update_pressure(c);
             tick(c);
+            if (c->gates_on) gate_move(c, c->step_end); /* finish_gate_bed (core/gate.py:100-101) */
             compute_cfl(c);
         }
         finish_step(c);
@@ -916,6 +1041,41 @@ int32_t nhf_cfl_state(nhf_ctx *c, float *ndt, int32_t *side) {
 
 int32_t nhf_sync(nhf_ctx *c) {
     (void)c;
+    return 0;
+}
+
+int32_t nhf_set_gates(nhf_ctx *c, const nhf_gates *gates) {
+    free(c->gate_budget);
+    free(c->gate_volume);
+    free(c->gate_sources);
+    c->gate_budget = NULL;
+    c->gate_volume = c->gate_sources = NULL;
+    c->gates_on = 0;
+    if (!gates) return 0;
+    if (gates->count < 1 || !(gates->opening_seconds > 0.0f)) return fail("nhf_set_gates: invalid counts or opening time");
+    c->g = *gates;
+    c->gate_budget = (ddsum *)alloc((size_t)(10 * gates->count + 8), sizeof(ddsum));
+    c->gate_volume = (real *)alloc(c->m.s_num, sizeof(real));
+    c->gate_sources = (real *)alloc((size_t)c->m.e_num * 6, sizeof(real));
+    c->gates_on = 1;
+    return 0;
+}
+
+int32_t nhf_gate_check(nhf_ctx *c) {
+    if (!c->gates_on) return fail("nhf_gate_check: no gates bound");
+    sync_host_writes(c);
+    gate_check(c);
+    store_state(c);
+    return 0;
+}
+
+int32_t nhf_flush_gate_budget(nhf_ctx *c, double *out) {
+    if (!c->gates_on) return fail("nhf_flush_gate_budget: no gates bound");
+    for (int t = 0; t < 10 * c->g.count + 8; ++t) {
+        out[2 * t] = c->gate_budget[t].hi;
+        out[2 * t + 1] = c->gate_budget[t].lo;
+        c->gate_budget[t].hi = c->gate_budget[t].lo = 0.0;
+    }
     return 0;
 }
 

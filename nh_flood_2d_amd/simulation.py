@@ -263,6 +263,8 @@ class Simulation:
         self.engine.reset(self.host_now_ticks)
         self.gates = gates
         if gates is not None:
+            if not hasattr(self.engine, 'set_gates'):
+                raise ValueError('gate control needs a single-mesh engine (not a batch slot)')
             # the controller's device lists and state, then check_gates before the first CFL
             # (solver_compact.py:885-889); a continuation restores the controller first (:723-724)
             gates.bind(self.engine, self.layout, mesh.z, s.min_h)

@@ -147,6 +147,8 @@ class FloodServer:
         group_use = {}     # group -> live members
         used = 0
         t0 = perf_counter()
+        inflight = []      # (owner, members) in launch order
+        busy = set()       # ids of owners with a batch in flight
         while pending or waiting or live:
             now = perf_counter() - t0
             while pending and (pending[0].arrival <= now or not realtime):
@@ -156,6 +158,9 @@ class FloodServer:
                 r = waiting[0]
                 if used + r.footprint > self.budget and live:
                     break
+                if r.group is not None and groups.get(r.group) is not None and id(groups[r.group]) in busy \
+                        and groups[r.group].free_slots():
+                    break   # binding a slot waits until the group's batch in flight is consumed
                 waiting.pop(0)
                 if r.group is not None:
                     batch = groups.get(r.group)
@@ -179,11 +184,14 @@ class FloodServer:
                 report.queued[r.key] = perf_counter() - t0 - r.arrival
             report.peak_live = max(report.peak_live, len(live))
             report.peak_bytes = max(report.peak_bytes, used)
-            # plan every live solve; finish the ones that are done
-            planned = []
+            # plan every solve whose engine is idle and launch one batch per such engine; finish the
+            # solves that are done
+            planned = {}
             for key, (sim, r, owner) in list(live.items()):
+                if id(owner) in busy:
+                    continue
                 if sim.plan():
-                    planned.append((key, sim, r, owner))
+                    planned.setdefault(id(owner), (owner, []))[1].append((key, sim))
                     continue
                 report.results[key] = sim.finish()
                 report.latency[key] = perf_counter() - t0 - r.arrival
@@ -199,19 +207,20 @@ class FloodServer:
                         owner.close()
                 del live[key]
                 used -= r.footprint
-            if not planned:
+            for oid, (owner, members) in planned.items():
+                owner.launch_batch(self.batch_size)
+                inflight.append((owner, members))
+                busy.add(oid)
+            if not inflight:
                 if not live and not waiting and pending and realtime:
                     sleep(max(0., min(0.05, pending[0].arrival - (perf_counter() - t0))))
                 continue
-            # one batch per engine, all enqueued before any is waited for
-            launched = {}
-            for key, sim, r, owner in planned:
-                if id(owner) not in launched:
-                    owner.launch_batch(self.batch_size)
-                    launched[id(owner)] = owner
-            traces = {oid: owner.wait_batch() for oid, owner in launched.items()}
-            for key, sim, r, owner in planned:
-                trace = traces[id(owner)]
+            # consume the oldest batch in flight while the others run; its engine is planned and
+            # relaunched at the top of the next pass
+            owner, members = inflight.pop(0)
+            trace = owner.wait_batch()
+            busy.discard(id(owner))
+            for key, sim in members:
                 sim.consume(trace[sim.engine.slot] if owner is not sim.engine else trace)
             report.batches += 1
         report.wall = perf_counter() - t0

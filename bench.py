@@ -486,11 +486,12 @@ def main():
                                                   forcing.rain.rates(forcing.rain.index_at(3600.)).astype(np.float32),
                                                   True), steps=96)
         per_graph = tuning['steps_per_graph']
-        log(f"tuned: cell workgroup {tuning['cell_block']}, CFL grid {tuning['cfl_blocks']}, {per_graph} steps per graph, "
+        log(f"tuned: cell workgroup {tuning['cell_block']} x {tuning.get('cells_per_thread', 1)} cells per thread, "
+            f"CFL grid {tuning['cfl_blocks']}, {per_graph} steps per graph, "
             f"{tuning['us_per_step']:.1f} us/step (candidates {tuning['table']})")
     eng = Engine(layout, params, state, device=device)
     if tuning:
-        eng.set_tuning(tuning['cfl_blocks'], tuning['cell_block'])
+        eng.set_tuning(tuning['cfl_blocks'], tuning['cell_block'], tuning.get('cells_per_thread', 1))
     start = seconds_to_ticks(3600.)
     _set_forcing(eng, forcing, 3600.)
     eng.reset(start)
@@ -584,7 +585,7 @@ def main():
         'kernels': kernels,
         'step': {'algorithmic_bytes': alg['step'],
                  'achieved_gbs': alg['step'] / (event_ms / steps * 1e-3) / 1e9,
-                 'hbm_bytes': sum(v for v in traffic.values()) if traffic else None},
+                 'hbm_bytes': sum(traffic[k] for k in kernels if k in traffic) if traffic else None},
     }
     if args.calibrate_copy:
         with torch.cuda.stream(device.stream):

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 8
+#define NHF_ABI_VERSION 9
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -238,11 +238,12 @@ int32_t nhf_settle_extrema(nhf_ctx *ctx);
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable);
 
 /* Launch tuning (no reference counterpart; used by the autotuner): the number of CFL partial
- * workgroups (a multiple of 8 up to the maximum the workspace holds) and the workgroup size of
- * the two cell passes (128, 256 or 512 threads). Every choice computes the same bits; budget
- * accumulators are kept across a change. out4 of nhf_tuning_range = CFL grid, CFL maximum, cell
- * workgroup size, 0. Captured graphs are rebuilt after a change. */
-int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block);
+ * workgroups (a multiple of 8 up to the maximum the workspace holds), the workgroup size of the
+ * two cell passes (128, 256 or 512 threads) and their cells per thread (1 or 2). Every choice
+ * computes the same state bits; budget accumulators are kept across a change (the sums are
+ * grouped differently, FP64 rounding only). out4 of nhf_tuning_range = CFL grid, CFL maximum,
+ * cell workgroup size, cells per thread. Captured graphs are rebuilt after a change. */
+int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block, int32_t cells_per_thread);
 int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *out4);
 
 /* Per-kernel timing with HIP events on the context's stream (eager mode only). Kernel ids:

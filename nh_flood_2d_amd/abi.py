@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -87,7 +87,7 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_sync': (c_int32, [vp]),
         'nhf_set_graph': (c_int32, [vp, c_int32]),
         'nhf_set_timing': (c_int32, [vp, c_int32]),
-        'nhf_set_tuning': (c_int32, [vp, c_int32, c_int32]),
+        'nhf_set_tuning': (c_int32, [vp, c_int32, c_int32, c_int32]),
         'nhf_time_kernels': (c_int32, [vp, c_int32, vp]),
         'nhf_settle_extrema': (c_int32, [vp]),
         'nhf_set_gates': (c_int32, [vp, POINTER(GatesStruct)]),

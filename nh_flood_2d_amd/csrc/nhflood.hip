@@ -42,6 +42,7 @@ struct Dev {
     double2 *acc;  // per depth-block, per term: (sum, compensation)
     int32_t terms, cfl_blocks, depth_blocks, theta_one, cfl_max;
     int32_t cell_block;  // workgroup size of the two cell passes (128, 256 or 512)
+    int32_t cells_per_thread;  // cells per thread of the two cell passes (1 or 2)
     int32_t acc_owners;  // budget accumulators allocated (wavefronts of the largest grid)
     int32_t gates_on;    // gate control bound (nhf_set_gates)
     nhf_gates g;
@@ -379,8 +380,8 @@ static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float 
     float ds = 1.0f;
     if (outgoing > available && !level) {
         ds = available / outgoing;
-        limited = 1;
-        scale_bits = __float_as_uint(ds);
+        limited += 1;
+        scale_bits = min(scale_bits, __float_as_uint(ds));  // non-negative floats order as their bits
     }
     // Shared-subedge scaling (solver_compact.py:516-524): every side takes the scale of its
     // donor, the lower cell when q >= 0 and the higher cell otherwise. This cell is the lower
@@ -419,30 +420,54 @@ static __device__ __forceinline__ void donor_cell(const Dev &d, int cell, float 
     if (bi >= 0 || m.pipe_q) d.donor[cell] = ds;
 }
 
-template <int BLOCK>
+// First round of the donor pass: every load that needs only the cell index.
+static __device__ __forceinline__ void load_donor_cell(const nhf_mesh &m, int cell, bool raining, DonorCell &v) {
+    v.meta = m.cell_meta[cell];
+    v.h = m.depth[cell];
+    v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
+    v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
+    v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
+    v.rec = cell_record(m, cell);
+}
+
+// CPT cells per thread (1 or 2): thread t of a workgroup takes cells t, t + BLOCK, ... of the
+// workgroup's BLOCK*CPT cells, and issues the first-round loads of all of them before using any.
+template <int BLOCK, int CPT>
 __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
     Ctl *c = d.ctl;
     if (!c->active) return;
     const nhf_mesh &m = d.m;
     const float dt = c->dt;
     const bool raining = c->raining;
-    const int cell = xcd_block() * BLOCK + threadIdx.x + 1;
-    const bool live = cell < m.e_num;
+    const int first = xcd_block() * BLOCK * CPT + threadIdx.x + 1;
     uint32_t limited = 0, faces = 0, scale_bits = __float_as_uint(1.0f);
-    // first round: the cell's own values and its side record
-    DonorCell v = {};
-    if (live) {
-        v.meta = m.cell_meta[cell];
-        v.h = m.depth[cell];
-        v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
-        v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
-        v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
-        v.rec = cell_record(m, cell);
-    }
-    if (__all(!live || v.rec.x >= 0)) {
-        if (live) donor_cell<true>(d, cell, dt, raining, v, limited, faces, scale_bits);
-    } else if (live) {
-        donor_cell<false>(d, cell, dt, raining, v, limited, faces, scale_bits);
+    // first round: the cells' own values and their side records
+    if constexpr (CPT == 1) {   // the plain form keeps every value in registers
+        const bool live = first < m.e_num;
+        DonorCell v = {};
+        if (live) load_donor_cell(m, first, raining, v);
+        if (__all(!live || v.rec.x >= 0)) {
+            if (live) donor_cell<true>(d, first, dt, raining, v, limited, faces, scale_bits);
+        } else if (live) {
+            donor_cell<false>(d, first, dt, raining, v, limited, faces, scale_bits);
+        }
+    } else {   // two cells: both first rounds, then both bodies (named values, no arrays)
+        static_assert(CPT == 2, "one or two cells per thread");
+        const int second = first + BLOCK;
+        const bool live0 = first < m.e_num, live1 = second < m.e_num;
+        DonorCell v0 = {}, v1 = {};
+        if (live0) load_donor_cell(m, first, raining, v0);
+        if (live1) load_donor_cell(m, second, raining, v1);
+        if (__all(!live0 || v0.rec.x >= 0)) {
+            if (live0) donor_cell<true>(d, first, dt, raining, v0, limited, faces, scale_bits);
+        } else if (live0) {
+            donor_cell<false>(d, first, dt, raining, v0, limited, faces, scale_bits);
+        }
+        if (__all(!live1 || v1.rec.x >= 0)) {
+            if (live1) donor_cell<true>(d, second, dt, raining, v1, limited, faces, scale_bits);
+        } else if (live1) {
+            donor_cell<false>(d, second, dt, raining, v1, limited, faces, scale_bits);
+        }
     }
     // limiter counters: one integer atomic per wavefront that limited a cell or face (rare)
     if (__any(limited | faces)) {
@@ -655,7 +680,7 @@ static __device__ __forceinline__ void load_depth_cell(const nhf_mesh &m, int ce
     v.rec = cell_record(m, cell);
 }
 
-template <int BLOCK>
+template <int BLOCK, int CPT>
 __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     Ctl *c = d.ctl;
     if (!c->active) return;
@@ -664,20 +689,44 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     const bool raining = c->raining;
     const uint64_t step_start = c->now;  // the clock commits this step's end in k_finish
     const int b_num = m.b_num;
-    const int cell = xcd_block() * BLOCK + threadIdx.x + 1;
-    const bool live = cell < m.e_num;
+    const int first = xcd_block() * BLOCK * CPT + threadIdx.x + 1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double2 *acc = d.acc + ((size_t)blockIdx.x * (BLOCK / 64) + wave) * d.terms;
-    // first round: the cell's own values and its side record
-    DepthCell v = {};
-    if (live) load_depth_cell(m, cell, raining, v);
-    DepthTerms t = {};
-    t.bi = -1;
-    t.kind = -1;
-    if (__all(!live || v.rec.x >= 0)) {
-        if (live) depth_cell<true>(d, cell, dt, raining, step_start, v, t);
-    } else if (live) {
-        depth_cell<false>(d, cell, dt, raining, step_start, v, t);
+    // first round: the cells' own values and their side records
+    DepthTerms t[CPT];
+    if constexpr (CPT == 1) {   // the plain form keeps every value in registers
+        const bool live = first < m.e_num;
+        DepthCell v = {};
+        if (live) load_depth_cell(m, first, raining, v);
+        t[0] = DepthTerms{};
+        t[0].bi = -1;
+        t[0].kind = -1;
+        if (__all(!live || v.rec.x >= 0)) {
+            if (live) depth_cell<true>(d, first, dt, raining, step_start, v, t[0]);
+        } else if (live) {
+            depth_cell<false>(d, first, dt, raining, step_start, v, t[0]);
+        }
+    } else {   // two cells: both first rounds, then both bodies (named values, no arrays)
+        static_assert(CPT == 2, "one or two cells per thread");
+        const int second = first + BLOCK;
+        const bool live0 = first < m.e_num, live1 = second < m.e_num;
+        DepthCell v0 = {}, v1 = {};
+        if (live0) load_depth_cell(m, first, raining, v0);
+        if (live1) load_depth_cell(m, second, raining, v1);
+        t[0] = DepthTerms{};
+        t[0].bi = t[0].kind = -1;
+        t[1] = DepthTerms{};
+        t[1].bi = t[1].kind = -1;
+        if (__all(!live0 || v0.rec.x >= 0)) {
+            if (live0) depth_cell<true>(d, first, dt, raining, step_start, v0, t[0]);
+        } else if (live0) {
+            depth_cell<false>(d, first, dt, raining, step_start, v0, t[0]);
+        }
+        if (__all(!live1 || v1.rec.x >= 0)) {
+            if (live1) depth_cell<true>(d, second, dt, raining, step_start, v1, t[1]);
+        } else if (live1) {
+            depth_cell<false>(d, second, dt, raining, step_start, v1, t[1]);
+        }
     }
     // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
     // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it. With at most
@@ -687,25 +736,31 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     if (lane_terms && lane < d.terms) held = acc[lane];
     bool touched = false;
     for (int k = 0; k < d.terms; ++k) {
-        float x = 0.0f;
-        if (k == 0)
-            x = t.infil;
-        else if (k == 1)
-            x = t.floor;
-        else if (k == 2)
-            x = t.level_floor;
-        else if (k == 3)
-            x = t.pipe;
-        else if (k < 4 + 5 * b_num) {
-            int tb = (k - 4) / 5, comp = (k - 4) % 5;
-            // a select chain, not an indexed load: the terms stay in registers
-            const float lv = comp == 0 ? t.lv[0] : comp == 1 ? t.lv[1] : comp == 2 ? t.lv[2] : comp == 3 ? t.lv[3] : t.lv[4];
-            if (t.kind == 0 && t.bi == tb) x = lv;
-        } else if (t.kind == 1 && t.bi == k - 4 - 5 * b_num) {
-            x = t.disc;
+        double x = 0.0;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const DepthTerms &tj = t[j];
+            float y = 0.0f;
+            if (k == 0)
+                y = tj.infil;
+            else if (k == 1)
+                y = tj.floor;
+            else if (k == 2)
+                y = tj.level_floor;
+            else if (k == 3)
+                y = tj.pipe;
+            else if (k < 4 + 5 * b_num) {
+                int tb = (k - 4) / 5, comp = (k - 4) % 5;
+                // a select chain, not an indexed load: the terms stay in registers
+                const float lv = comp == 0 ? tj.lv[0] : comp == 1 ? tj.lv[1] : comp == 2 ? tj.lv[2] : comp == 3 ? tj.lv[3] : tj.lv[4];
+                if (tj.kind == 0 && tj.bi == tb) y = lv;
+            } else if (tj.kind == 1 && tj.bi == k - 4 - 5 * b_num) {
+                y = tj.disc;
+            }
+            x += (double)y;
         }
-        if (!__any(x != 0.0f)) continue;
-        double w = wave_sum((double)x);
+        if (!__any(x != 0.0)) continue;
+        double w = wave_sum(x);
         if (lane_terms) {
             if (lane == k) {
                 acc_add(held, w);
@@ -918,6 +973,23 @@ __global__ __launch_bounds__(1024) void k_finish(Dev d, int mode, int slot, int 
 
 enum { K_PREPARE = 0, K_GRAD, K_FACE, K_FLUX, K_DONOR, K_DEPTH, K_CFL, K_FINISH, K_GATE, K_COUNT };
 
+// The cell passes' instantiation for the context's workgroup size and cells per thread.
+#define NHF_CELL_PASS(KERNEL)                                                                       \
+    static void launch_##KERNEL(const Dev &d, hipStream_t s) {                                     \
+        const dim3 grid(d.depth_blocks);                                                           \
+        switch (d.cell_block * 10 + d.cells_per_thread) {                                          \
+        case 1281: hipLaunchKernelGGL((KERNEL<128, 1>), grid, dim3(128), 0, s, d); break;           \
+        case 1282: hipLaunchKernelGGL((KERNEL<128, 2>), grid, dim3(128), 0, s, d); break;           \
+        case 2562: hipLaunchKernelGGL((KERNEL<256, 2>), grid, dim3(256), 0, s, d); break;           \
+        case 5121: hipLaunchKernelGGL((KERNEL<512, 1>), grid, dim3(512), 0, s, d); break;           \
+        case 5122: hipLaunchKernelGGL((KERNEL<512, 2>), grid, dim3(512), 0, s, d); break;           \
+        default: hipLaunchKernelGGL((KERNEL<256, 1>), grid, dim3(256), 0, s, d); break;             \
+        }                                                                                          \
+    }
+NHF_CELL_PASS(k_donor)
+NHF_CELL_PASS(k_depth)
+#undef NHF_CELL_PASS
+
 struct nhf_ctx {
     Dev d;
     hipStream_t stream;
@@ -1051,6 +1123,7 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     d.cfl_blocks = cfl_grid(mesh->s_num);
     d.cfl_max = cfl_max_blocks(mesh->s_num);
     d.cell_block = NHF_BLOCK;
+    d.cells_per_thread = 1;
     d.depth_blocks = cell_grid(mesh->e_num, d.cell_block);
     d.acc_owners = acc_owner_count(mesh->e_num);
     d.theta_one = params->theta == 1.0f;
@@ -1132,14 +1205,14 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
     hipStream_t s = ctx->stream;
     size_t ev = 0;
     ctx->event_kind.clear();
-#define LAUNCH(kind, ...)                                                   \
+#define TIMED(kind, stmt)                                                   \
     do {                                                                    \
         if (timed) {                                                        \
             hipEvent_t e0 = timing_event(ctx, ev++);                        \
             if (!e0) return set_error_text("hipEventCreate failed");        \
             HIP_TRY(hipEventRecord(e0, s));                                 \
         }                                                                   \
-        hipLaunchKernelGGL(__VA_ARGS__);                                    \
+        stmt;                                                               \
         if (timed) {                                                        \
             hipEvent_t e1 = timing_event(ctx, ev++);                        \
             if (!e1) return set_error_text("hipEventCreate failed");        \
@@ -1147,6 +1220,7 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
             ctx->event_kind.push_back(kind);                                \
         }                                                                   \
     } while (0)
+#define LAUNCH(kind, ...) TIMED(kind, hipLaunchKernelGGL(__VA_ARGS__))
     const int cells = d.m.e_num - 1, sides = d.m.s_num - 1;
     LAUNCH(K_PREPARE, k_prepare, dim3(1), dim3(64), 0, s, d, 0);
     for (int slot = 0; slot < steps; ++slot) {
@@ -1154,25 +1228,16 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
         if (d.m.n_grad > 0) LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
         if (!d.theta_one) LAUNCH(K_FACE, k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
         if (sides > 0) LAUNCH(K_FLUX, k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d);
-        if (d.cell_block == 128)
-            LAUNCH(K_DONOR, k_donor<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
-        else if (d.cell_block == 512)
-            LAUNCH(K_DONOR, k_donor<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
-        else
-            LAUNCH(K_DONOR, k_donor<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
+        TIMED(K_DONOR, launch_k_donor(d, s));
         if (d.gates_on && d.g.n_chunks > 0)
             LAUNCH(K_GATE, k_gate_account, dim3(d.g.n_chunks), dim3(NHF_BLOCK), 0, s, d);
-        if (d.cell_block == 128)
-            LAUNCH(K_DEPTH, k_depth<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
-        else if (d.cell_block == 512)
-            LAUNCH(K_DEPTH, k_depth<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
-        else
-            LAUNCH(K_DEPTH, k_depth<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
+        TIMED(K_DEPTH, launch_k_depth(d, s));
         if (d.gates_on) LAUNCH(K_GATE, k_gate_move, dim3(1), dim3(1024), 0, s, d, 1, 0);
         LAUNCH(K_CFL, k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
         LAUNCH(K_FINISH, k_finish, dim3(1), dim3(1024), 0, s, d, 1, slot, steps);
     }
 #undef LAUNCH
+#undef TIMED
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1248,22 +1313,8 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
             case K_GRAD: hipLaunchKernelGGL(k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_FACE: hipLaunchKernelGGL(k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_FLUX: hipLaunchKernelGGL(k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d); break;
-            case K_DONOR:
-                if (d.cell_block == 128)
-                    hipLaunchKernelGGL(k_donor<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
-                else if (d.cell_block == 512)
-                    hipLaunchKernelGGL(k_donor<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
-                else
-                    hipLaunchKernelGGL(k_donor<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
-                break;
-            case K_DEPTH:
-                if (d.cell_block == 128)
-                    hipLaunchKernelGGL(k_depth<128>, dim3(d.depth_blocks), dim3(128), 0, s, d);
-                else if (d.cell_block == 512)
-                    hipLaunchKernelGGL(k_depth<512>, dim3(d.depth_blocks), dim3(512), 0, s, d);
-                else
-                    hipLaunchKernelGGL(k_depth<256>, dim3(d.depth_blocks), dim3(256), 0, s, d);
-                break;
+            case K_DONOR: launch_k_donor(d, s); break;
+            case K_DEPTH: launch_k_depth(d, s); break;
             case K_CFL: hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1); break;
             case K_FINISH: hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, d, 0, 0, 0); break;
             }
@@ -1452,13 +1503,16 @@ int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable) {
     return 0;
 }
 
-int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block) {
+int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block, int32_t cells_per_thread) {
     if (cfl_blocks < 8 || cfl_blocks > ctx->d.cfl_max || (cfl_blocks & 7))
         return set_error_text("nhf_set_tuning: CFL grid out of range or not a multiple of 8");
     if (cell_block != 128 && cell_block != 256 && cell_block != 512)
         return set_error_text("nhf_set_tuning: cell workgroup size must be 128, 256 or 512");
+    if (cells_per_thread != 1 && cells_per_thread != 2)
+        return set_error_text("nhf_set_tuning: cells per thread must be 1 or 2");
     if (ctx->pending_steps) return set_error_text("nhf_set_tuning: a batch is in flight");
-    if (cfl_blocks != ctx->d.cfl_blocks || cell_block != ctx->d.cell_block) {
+    if (cfl_blocks != ctx->d.cfl_blocks || cell_block != ctx->d.cell_block ||
+        cells_per_thread != ctx->d.cells_per_thread) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         for (int i = 0; i <= NHF_MAX_BATCH; ++i)
             if (ctx->exec[i]) {
@@ -1467,7 +1521,8 @@ int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block) {
             }
         ctx->d.cfl_blocks = cfl_blocks;
         ctx->d.cell_block = cell_block;
-        ctx->d.depth_blocks = cell_grid(ctx->d.m.e_num, cell_block);
+        ctx->d.cells_per_thread = cells_per_thread;
+        ctx->d.depth_blocks = cell_grid(ctx->d.m.e_num, cell_block * cells_per_thread);
     }
     return 0;
 }
@@ -1476,7 +1531,7 @@ int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *out4) {
     out4[0] = ctx->d.cfl_blocks;
     out4[1] = ctx->d.cfl_max;
     out4[2] = ctx->d.cell_block;
-    out4[3] = 0;
+    out4[3] = ctx->d.cells_per_thread;
     return 0;
 }
 

@@ -307,14 +307,15 @@ class Engine:
         abi.check(self.lib, self.lib.nhf_time_kernels(self.ctx, int(repeats), us.ctypes.data), 'nhf_time_kernels')
         return us
 
-    def set_tuning(self, cfl_blocks: int, cell_block: int = 256) -> None:
-        rc = self.lib.nhf_set_tuning(self.ctx, int(cfl_blocks), int(cell_block))
+    def set_tuning(self, cfl_blocks: int, cell_block: int = 256, cells_per_thread: int = 1) -> None:
+        rc = self.lib.nhf_set_tuning(self.ctx, int(cfl_blocks), int(cell_block), int(cells_per_thread))
         abi.check(self.lib, rc, 'nhf_set_tuning')
 
     def tuning_range(self) -> dict:
         out = np.zeros(4, dtype=np.int32)
         self.lib.nhf_tuning_range(self.ctx, out.ctypes.data)
-        return dict(cfl_blocks=int(out[0]), cfl_max=int(out[1]), cell_block=int(out[2]))
+        return dict(cfl_blocks=int(out[0]), cfl_max=int(out[1]), cell_block=int(out[2]),
+                    cells_per_thread=int(out[3]))
 
     def kernel_times(self, reset: bool = True):
         ms = np.zeros(8, dtype=np.float64)

@@ -3,8 +3,8 @@
 What it picks, by timing candidates on a scratch engine built from the solve's own layout and
 initial state (the real solve's state, clock and budget are never touched):
 
-* single mesh: the workgroup size of the two cell passes, the CFL partial grid (workgroups of
-  the CFL pass, hence sides per thread) and the steps per captured graph;
+* single mesh: the workgroup size and cells per thread of the two cell passes, the CFL partial
+  grid (workgroups of the CFL pass, hence sides per thread) and the steps per captured graph;
 * scenario batch: the grid of the grid-stride depth pass (cells x slots per thread), the CFL
   grid and the steps per graph.
 
@@ -24,7 +24,8 @@ import numpy as np
 CFL_GRIDS = (256, 512, 1024, 2048, 4096)
 DEPTH_GRIDS = (512, 1024, 2048, 4096, 8192)
 STEPS_PER_GRAPH = (16, 32, 64)
-CELL_BLOCKS = (128, 256, 512)
+# (workgroup size, cells per thread) of the two cell passes
+CELL_SHAPES = ((128, 1), (256, 1), (512, 1), (128, 2), (256, 2))
 
 
 def size_class(layout, slots: int = 1) -> str:
@@ -80,10 +81,11 @@ def pick(times: dict) -> tuple:
     return min(times, key=lambda k: (times[k], list(times).index(k)))
 
 
-def tune_engine(engine, steps: int = 128, rounds: int = 2, cfl_grids=CFL_GRIDS, cell_blocks=CELL_BLOCKS,
+def tune_engine(engine, steps: int = 128, rounds: int = 2, cfl_grids=CFL_GRIDS, cell_shapes=CELL_SHAPES,
                 steps_per_graph=STEPS_PER_GRAPH) -> dict:
-    """Time a single-mesh Engine (already reset, forcing set): the cell workgroup size first (at
-    the default CFL grid, 32 steps per graph), then the CFL grid, then the steps per graph."""
+    """Time a single-mesh Engine (already reset, forcing set): the cell-pass shape (workgroup size
+    and cells per thread) first (at the default CFL grid, 32 steps per graph), then the CFL grid,
+    then the steps per graph."""
     rng = engine.tuning_range()
     grids = [g for g in cfl_grids if g <= rng['cfl_max']] or [rng['cfl_max'] - rng['cfl_max'] % 8 or 8]
     default_grid = rng['cfl_blocks']
@@ -92,26 +94,27 @@ def tune_engine(engine, steps: int = 128, rounds: int = 2, cfl_grids=CFL_GRIDS, 
     engine.set_graph(True)
     times = {}
 
-    def measure(g, cb, pg):
-        engine.set_tuning(g, cb)
+    def measure(g, shape, pg):
+        engine.set_tuning(g, shape[0], shape[1])
         t = _time(engine.synchronize, begin, engine.run_batch, steps, pg, horizon)
-        times[(g, cb, pg)] = min(times.get((g, cb, pg), math.inf), t)
+        times[(g, shape, pg)] = min(times.get((g, shape, pg), math.inf), t)
 
     for _ in range(rounds):                       # interleaved rounds, minimum kept
-        for cb in cell_blocks:
-            measure(default_grid, cb, 32)
-    best_cb = pick({k: v for k, v in times.items() if k[0] == default_grid and k[2] == 32})[1]
+        for shape in cell_shapes:
+            measure(default_grid, shape, 32)
+    best_shape = pick({k: v for k, v in times.items() if k[0] == default_grid and k[2] == 32})[1]
     for _ in range(rounds):
         for g in grids:
-            measure(g, best_cb, 32)
-    best_g = pick({k: v for k, v in times.items() if k[1] == best_cb and k[2] == 32})[0]
+            measure(g, best_shape, 32)
+    best_g = pick({k: v for k, v in times.items() if k[1] == best_shape and k[2] == 32})[0]
     for _ in range(rounds):
         for pg in steps_per_graph:
-            measure(best_g, best_cb, pg)
-    best = pick({k: v for k, v in times.items() if k[0] == best_g and k[1] == best_cb})
-    engine.set_tuning(best[0], best[1])
-    return dict(cfl_blocks=best[0], cell_block=best[1], steps_per_graph=best[2], us_per_step=times[best] * 1e6,
-                table={f'{a}/{b}/{c}': v * 1e6 for (a, b, c), v in times.items()})
+            measure(best_g, best_shape, pg)
+    best = pick({k: v for k, v in times.items() if k[0] == best_g and k[1] == best_shape})
+    engine.set_tuning(best[0], best[1][0], best[1][1])
+    return dict(cfl_blocks=best[0], cell_block=best[1][0], cells_per_thread=best[1][1], steps_per_graph=best[2],
+                us_per_step=times[best] * 1e6,
+                table={f'{a}/{b[0]}x{b[1]}/{c}': v * 1e6 for (a, b, c), v in times.items()})
 
 
 def tune_batch(batch, views, steps: int = 64, rounds: int = 2, depth_grids=DEPTH_GRIDS, cfl_grids=CFL_GRIDS,

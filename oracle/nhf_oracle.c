@@ -964,13 +964,13 @@ int32_t nhf_time_kernels(nhf_ctx *c, int32_t repeats, double *us) {
 }
 
 /* Launch tuning has no meaning on the CPU: accepted and ignored. */
-int32_t nhf_set_tuning(nhf_ctx *c, int32_t cfl_blocks, int32_t cell_block) { return 0; }
+int32_t nhf_set_tuning(nhf_ctx *c, int32_t cfl_blocks, int32_t cell_block, int32_t cells_per_thread) { return 0; }
 
 int32_t nhf_tuning_range(nhf_ctx *c, int32_t *out4) {
     out4[0] = 8;
     out4[1] = 8;
     out4[2] = 256;
-    out4[3] = 0;
+    out4[3] = 1;
     return 0;
 }
 

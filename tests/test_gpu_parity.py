@@ -158,19 +158,20 @@ def test_graph_replays_match_eager(gpu_device):
             np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=f'{name} replay {k}')
 
 
-@pytest.mark.parametrize('cfl_blocks,cell_block', [(512, 128), (4096, 512)])
-def test_launch_tuning_keeps_the_bits(gpu_device, cfl_blocks, cell_block):
+@pytest.mark.parametrize('cfl_blocks,cell_block,cells_per_thread', [(512, 128, 1), (4096, 512, 1), (2048, 256, 2),
+                                                                    (1024, 128, 2)])
+def test_launch_tuning_keeps_the_bits(gpu_device, cfl_blocks, cell_block, cells_per_thread):
     """Every workgroup size and CFL grid the autotuner may pick computes the same state, trace
     and budget as the default launch shapes."""
     from nh_flood_2d_amd.engine import Engine
     sc = synth.small_multires(9, pond=0.3, discharge=1.0)
-    _say(f'[tuning] small_multires seed 9, CFL grid {cfl_blocks}, cell workgroup {cell_block}')
+    _say(f'[tuning] small_multires seed 9, CFL grid {cfl_blocks}, cell workgroup {cell_block} x {cells_per_thread}')
     runs = []
     for tuned in (False, True):
         pair = Pair(sc, gpu_device)
         if tuned:
             top = pair.gpu.tuning_range()['cfl_max']
-            pair.gpu.set_tuning(min(cfl_blocks, top - top % 8), cell_block)
+            pair.gpu.set_tuning(min(cfl_blocks, top - top % 8), cell_block, cells_per_thread)
         rows = [pair.fixed_step(0.05)[0] for _ in range(12)]
         runs.append((np.array(rows), pair.gpu.read_state('depth'), pair.gpu.read_state('q'),
                      pair.gpu.flush_budget()))

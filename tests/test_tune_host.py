@@ -10,7 +10,7 @@ from nh_flood_2d_amd import synth
 from nh_flood_2d_amd.engine import Engine, make_params
 from nh_flood_2d_amd.forcing import cell_forcing_arrays
 from nh_flood_2d_amd.layout import build_layout, initial_state
-from nh_flood_2d_amd.tune import TuningCache, pick, size_class, tune_engine, tuned_engine_parameters
+from nh_flood_2d_amd.tune import CELL_SHAPES, TuningCache, pick, size_class, tune_engine, tuned_engine_parameters
 
 
 def _setup():
@@ -28,7 +28,8 @@ def test_tuner_times_every_candidate_and_keeps_the_state_untouched(tmp_path):
     params = make_params(sc.afa, sc.sita, sc.min_h, 64)
     got = tuned_engine_parameters(layout, params, state, oracle.HostDevice('f32'), cache=cache, steps=16)
     assert got['cfl_blocks'] == 8 and got['steps_per_graph'] in (16, 32, 64)
-    assert len(got['table']) == 5 and got['us_per_step'] > 0 and got['cell_block'] in (128, 256, 512)
+    assert len(got['table']) == len(CELL_SHAPES) + 2 and got['us_per_step'] > 0
+    assert (got['cell_block'], got['cells_per_thread']) in CELL_SHAPES
     for k in before:
         np.testing.assert_array_equal(state[k], before[k])
     # cached per size class, also across processes through the JSON file
@@ -43,7 +44,7 @@ def test_tune_engine_on_a_live_engine_chooses_the_fastest():
     eng.reset(0)
     eng.compute_cfl()
     result = tune_engine(eng, steps=8, rounds=1, steps_per_graph=(8, 16))
-    assert len(result['table']) == 5
+    assert len(result['table']) == len(CELL_SHAPES) + 2
     assert result['us_per_step'] == min(result['table'].values())
 
 

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
 
 struct DepthCell {
     uint32_t meta, pix;
-    float h, size, cap, max_d, max_s, rain_time, pq, pv_before, pv_correction;
+    float h, size, max_d, max_s, rain_time, pq, pv_before, pv_correction;
     int4 rec;
 };
 
@@ -555,7 +555,9 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
     // compensated window volume (solver_compact.py:535-539)
     const float pv_increment = pipe * dt - v.pv_correction;
     const float pv_total = v.pv_before + pv_increment;
-    const float cap = rainq > 0.0f ? fc + (f0 - fc) * det_expf(-hk * (v.rain_time + dt) / 3600.0f) : v.cap;
+    // a cell with rain recomputes its capacity from its rain time; only a cell without reads it
+    const float cap = rainq > 0.0f ? fc + (f0 - fc) * det_expf(-hk * (v.rain_time + dt) / 3600.0f)
+                                   : m.capacity[cell];
     float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
     if (l.short_list) {
 #pragma unroll
@@ -646,7 +648,7 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
         m.rain_time[cell] = v.rain_time + dt;
         m.capacity[cell] = cap;
     }
-    m.depth[cell] = h_new;
+    if (__float_as_uint(h_new) != __float_as_uint(previous)) m.depth[cell] = h_new;
     if (speed > v.max_s) m.max_speed[cell] = speed;
     // Extrema (solver_compact.py:600-606): the maximum depth and the end tick of the step that
     // first reached it. A negative max_depth marks a cell whose depth rose in its last step: its
@@ -667,7 +669,6 @@ static __device__ __forceinline__ void load_depth_cell(const nhf_mesh &m, int ce
     v.meta = m.cell_meta[cell];
     v.h = m.depth[cell];
     v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
-    v.cap = m.capacity[cell];
     v.max_d = m.max_depth[cell];
     v.max_s = m.max_speed[cell];
     v.rain_time = raining ? m.rain_time[cell] : 0.0f;
@@ -731,36 +732,15 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
     // its cells' F32 terms; wavefronts whose cells are all zero for a term skip it. With at most
     // 64 terms, lane t holds the accumulator of term t, loaded once while the sums are formed.
+    // Terms 0-3 (infiltration, floors, pipe) come first; the level components (4 + 5 b + c) and
+    // the applied discharges (4 + 5 B + b) only in a wavefront that holds a boundary cell.
     const bool lane_terms = d.terms <= 64;
     double2 held = make_double2(0.0, 0.0);
     if (lane_terms && lane < d.terms) held = acc[lane];
     bool touched = false;
-    for (int k = 0; k < d.terms; ++k) {
-        double x = 0.0;
-#pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            const DepthTerms &tj = t[j];
-            float y = 0.0f;
-            if (k == 0)
-                y = tj.infil;
-            else if (k == 1)
-                y = tj.floor;
-            else if (k == 2)
-                y = tj.level_floor;
-            else if (k == 3)
-                y = tj.pipe;
-            else if (k < 4 + 5 * b_num) {
-                int tb = (k - 4) / 5, comp = (k - 4) % 5;
-                // a select chain, not an indexed load: the terms stay in registers
-                const float lv = comp == 0 ? tj.lv[0] : comp == 1 ? tj.lv[1] : comp == 2 ? tj.lv[2] : comp == 3 ? tj.lv[3] : tj.lv[4];
-                if (tj.kind == 0 && tj.bi == tb) y = lv;
-            } else if (tj.kind == 1 && tj.bi == k - 4 - 5 * b_num) {
-                y = tj.disc;
-            }
-            x += (double)y;
-        }
-        if (!__any(x != 0.0)) continue;
-        double w = wave_sum(x);
+    auto add_term = [&](int k, double x) {
+        if (!__any(x != 0.0)) return;
+        const double w = wave_sum(x);
         if (lane_terms) {
             if (lane == k) {
                 acc_add(held, w);
@@ -770,6 +750,47 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
             double2 a = acc[k];
             acc_add(a, w);
             acc[k] = a;
+        }
+    };
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
+    bool boundary = false;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        x0 += (double)t[j].infil;
+        x1 += (double)t[j].floor;
+        x2 += (double)t[j].level_floor;
+        x3 += (double)t[j].pipe;
+        boundary = boundary || t[j].kind >= 0;
+    }
+    add_term(0, x0);
+    add_term(1, x1);
+    add_term(2, x2);
+    add_term(3, x3);
+    if (__any(boundary)) {
+        for (int tb = 0; tb < b_num; ++tb) {
+            bool level = false, inflow = false;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                level = level || (t[j].kind == 0 && t[j].bi == tb);
+                inflow = inflow || (t[j].kind == 1 && t[j].bi == tb);
+            }
+            if (__any(level)) {
+#pragma unroll
+                for (int comp = 0; comp < 5; ++comp) {
+                    double x = 0.0;
+#pragma unroll
+                    for (int j = 0; j < CPT; ++j)
+                        x += (double)((t[j].kind == 0 && t[j].bi == tb) ? t[j].lv[comp] : 0.0f);
+                    add_term(4 + 5 * tb + comp, x);
+                }
+            }
+            if (__any(inflow)) {
+                double x = 0.0;
+#pragma unroll
+                for (int j = 0; j < CPT; ++j)
+                    x += (double)((t[j].kind == 1 && t[j].bi == tb) ? t[j].disc : 0.0f);
+                add_term(4 + 5 * b_num + tb, x);
+            }
         }
     }
     if (touched) acc[lane] = held;

@@ -18,6 +18,13 @@ STATE_CELLS = ('depth', 'bed', 'rain_time', 'capacity', 'max_depth', 'max_speed'
 STATE_SIDES = ('q',)
 
 
+def check_extrema(name: str, values) -> None:
+    """The step keeps max_depth >= 0 and reads a negative value as the mark of a cell whose depth
+    is rising (its maximum is its depth), so a negative max_depth from the host is rejected."""
+    if name == 'max_depth' and np.any(np.asarray(values, dtype=np.float32) < 0):
+        raise ValueError(f'{name}: maxima must not be negative')
+
+
 class TorchDevice:
     """Device memory and stream for the HIP library (torch is plumbing only)."""
 
@@ -113,6 +120,7 @@ class Engine:
         e_num, s_num = layout.e_num, layout.s_num
         # state (device order)
         for name in STATE_CELLS:
+            check_extrema(name, state[name])
             self.buffers[name] = dev.array(layout.cells_to_device(state[name]))
         self.buffers['q'] = dev.array(layout.sides_to_device(state['q']))
         for name, values in arr.items():
@@ -176,6 +184,7 @@ class Engine:
             self.settle_extrema()     # a marked cell's maximum is its depth: keep the pair final
         dtype = np.uint64 if name == 'peak_tick' else np.float32
         values = np.asarray(values, dtype=dtype)
+        check_extrema(name, values)
         if name in STATE_SIDES:
             values = self.layout.sides_to_device(values)
         else:

@@ -446,3 +446,27 @@ def test_oracle_takes_host_writes_between_batches():
         eng.run_batch(1)
         np.testing.assert_array_equal(eng.read_state('max_depth'), eng.read_state('depth'))
         eng.close()
+
+
+def test_negative_max_depth_is_rejected():
+    """The step reads a negative max_depth as the mark of a rising cell, so the engine refuses a
+    negative max_depth from the host (at creation and in write_state); other arrays pass."""
+    import oracle
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.engine import Engine, check_extrema, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state
+    check_extrema('max_depth', np.array([0., -0., 2.], np.float32))
+    check_extrema('max_speed', -np.ones(3, np.float32))
+    with pytest.raises(ValueError, match='max_depth'):
+        check_extrema('max_depth', np.array([0., -1e-30, 2.], np.float32))
+    sc = synth.small_multires(2, pond=0.3)
+    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, synth.scenario_forcing(sc)), 1.0)
+    state = initial_state(sc.mesh, layout)
+    bad = dict(state, max_depth=np.full(sc.mesh.e_num, -1., np.float32))
+    with pytest.raises(ValueError, match='max_depth'):
+        Engine(layout, make_params(sc.afa, sc.sita, sc.min_h, 32), bad, device=oracle.HostDevice('f32'))
+    eng = Engine(layout, make_params(sc.afa, sc.sita, sc.min_h, 32), state, device=oracle.HostDevice('f32'))
+    with pytest.raises(ValueError, match='max_depth'):
+        eng.write_state('max_depth', np.full(sc.mesh.e_num, -2., np.float32))
+    eng.close()

@@ -204,9 +204,7 @@ __device__ void face_difference(const Dev &d, int cell, int dir, int axis, float
     usable = ok;
 }
 
-__global__ __launch_bounds__(NHF_BLOCK) void k_gradients(Dev d) {
-    if (!d.ctl->active) return;
-    int slot = blockIdx.x * NHF_BLOCK + threadIdx.x;
+static __device__ __forceinline__ void gradient_slot(const Dev &d, int slot) {
     if (slot >= d.m.n_grad) return;
     int cell = d.m.grad_cell[slot];
     int axes = d.m.grad_axes[slot];
@@ -228,6 +226,11 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_gradients(Dev d) {
         }
         d.grad[2 * slot + axis] = value;
     }
+}
+
+__global__ __launch_bounds__(NHF_BLOCK) void k_gradients(Dev d) {
+    if (!d.ctl->active) return;
+    gradient_slot(d, blockIdx.x * NHF_BLOCK + threadIdx.x);
 }
 
 // --------------------------------------------------------------------------- theta < 1
@@ -902,9 +905,18 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_gate_account(Dev d) {
 // Per-workgroup CFL minimum over a grid-stride range of sides. The key orders by the F32 bits of
 // the bound (non-negative, so bit order is numeric order), then by the reference side id, which
 // makes the argmin the smallest reference id among ties (solver_compact.py:633-641).
-__global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode) {
+// The first `grad_blocks` workgroups (a multiple of 8, so each CFL workgroup keeps its XCD)
+// compute the next step's hanging-edge gradients instead: they need only the depth and bed this
+// step ends with, which are final once the depth pass is done.
+__global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int grad_blocks) {
     Ctl *c = d.ctl;
     if (mode == 1 && !c->active) return;
+    if ((int)blockIdx.x < grad_blocks) {
+        gradient_slot(d, blockIdx.x * NHF_BLOCK + threadIdx.x);
+        return;
+    }
+    const int block = (int)blockIdx.x - grad_blocks;
+    const int xb = (block & 7) * (d.cfl_blocks >> 3) + (block >> 3);
     __shared__ uint64_t s_key[NHF_WAVES];
     __shared__ int32_t s_idx[NHF_WAVES];
     const nhf_mesh &m = d.m;
@@ -913,7 +925,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode) {
     // where the bounds tie, and for the thread's winner at the end.
     uint32_t best_bits = ~0u;
     int32_t idx = -1;
-    for (int si = xcd_block() * NHF_BLOCK + threadIdx.x + 1; si < m.s_num; si += gridDim.x * NHF_BLOCK) {
+    for (int si = xb * NHF_BLOCK + threadIdx.x + 1; si < m.s_num; si += d.cfl_blocks * NHF_BLOCK) {
         int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
         if (lh.x != 0 && lh.y != 0) {
             float zl = m.bed[lh.x], zh = m.bed[lh.y];
@@ -942,8 +954,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode) {
                 key = s_key[w];
                 idx = s_idx[w];
             }
-        d.cfl_key[blockIdx.x] = key;
-        d.cfl_idx[blockIdx.x] = idx;
+        d.cfl_key[block] = key;
+        d.cfl_idx[block] = idx;
     }
 }
 
@@ -1243,10 +1255,16 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
     } while (0)
 #define LAUNCH(kind, ...) TIMED(kind, hipLaunchKernelGGL(__VA_ARGS__))
     const int cells = d.m.e_num - 1, sides = d.m.s_num - 1;
+    const bool fuse_grad = d.m.n_grad > 0 && !d.gates_on;
+    const int grad_blocks = blocks_xcd(d.m.n_grad > 0 ? d.m.n_grad : 1);
     LAUNCH(K_PREPARE, k_prepare, dim3(1), dim3(64), 0, s, d, 0);
     for (int slot = 0; slot < steps; ++slot) {
         if (d.gates_on) LAUNCH(K_GATE, k_gate_move, dim3(1), dim3(1024), 0, s, d, 0, 0);
-        if (d.m.n_grad > 0) LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
+        // gradients: a launch of their own at the batch's first step (the host may have written
+        // the state) and where gates move the bed at the step start; otherwise the previous
+        // step's CFL launch computed them
+        if (d.m.n_grad > 0 && (slot == 0 || !fuse_grad))
+            LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
         if (!d.theta_one) LAUNCH(K_FACE, k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
         if (sides > 0) LAUNCH(K_FLUX, k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d);
         TIMED(K_DONOR, launch_k_donor(d, s));
@@ -1254,7 +1272,8 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
             LAUNCH(K_GATE, k_gate_account, dim3(d.g.n_chunks), dim3(NHF_BLOCK), 0, s, d);
         TIMED(K_DEPTH, launch_k_depth(d, s));
         if (d.gates_on) LAUNCH(K_GATE, k_gate_move, dim3(1), dim3(1024), 0, s, d, 1, 0);
-        LAUNCH(K_CFL, k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
+        const int gb = fuse_grad && slot + 1 < steps ? grad_blocks : 0;
+        LAUNCH(K_CFL, k_cfl, dim3(gb + d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, gb);
         LAUNCH(K_FINISH, k_finish, dim3(1), dim3(1024), 0, s, d, 1, slot, steps);
     }
 #undef LAUNCH
@@ -1336,7 +1355,7 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
             case K_FLUX: hipLaunchKernelGGL(k_flux, dim3(blocks_xcd(sides)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_DONOR: launch_k_donor(d, s); break;
             case K_DEPTH: launch_k_depth(d, s); break;
-            case K_CFL: hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1); break;
+            case K_CFL: hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, 0); break;
             case K_FINISH: hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, d, 0, 0, 0); break;
             }
         }
@@ -1352,7 +1371,7 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
 
 int32_t nhf_compute_cfl(nhf_ctx *ctx) {
     const Dev &d = ctx->d;
-    hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+    hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0);
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, ctx->stream, d, 0, 0, 0);
     HIP_TRY(hipGetLastError());
     return 0;

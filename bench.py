@@ -149,8 +149,13 @@ def trace_summary(trace_csv: str, bench_json: str) -> str:
     cfg = res['config']
     cells, sides = cfg['cells'], cfg['sides']
     steps = res['kernels']['flux']['launches']
-    grid = lambda n: ((n + 255) // 256 + 7) // 8 * 8 * 256
-    unique = {'k_flux': grid(sides), 'k_donor': grid(cells), 'k_depth': grid(cells)}
+    tuning = cfg.get('tuning') or {}
+    block, cpt = tuning.get('cell_block', 256), tuning.get('cells_per_thread', 1)
+    grid = lambda n, per, threads: ((n + per - 1) // per + 7) // 8 * 8 * threads
+    # threads of each pass's grid: the flux pass takes two sides per thread, the cell passes the
+    # tuned workgroup size and cells per thread
+    unique = {'k_flux': grid(sides, 512, 256), 'k_donor': grid(cells, block * cpt, block),
+              'k_depth': grid(cells, block * cpt, block)}
     rows = list(csv.DictReader(open(trace_csv)))
     every = defaultdict(list)
     for r in rows:

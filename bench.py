@@ -439,8 +439,8 @@ def main():
     ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
     ap.add_argument('--serve-requests', type=int, default=12, help='solves of the serving load (0: skip)')
     ap.add_argument('--serve-rate', type=float, default=2.0)
-    ap.add_argument('--serve-min-cells', type=int, default=50_000)
-    ap.add_argument('--serve-max-cells', type=int, default=1_000_000)
+    ap.add_argument('--serve-min-cells', type=int, default=100_000)
+    ap.add_argument('--serve-max-cells', type=int, default=8_000_000)
     ap.add_argument('--serve-min-h', type=float, default=1.0)
     ap.add_argument('--serve-max-h', type=float, default=6.0)
     ap.add_argument('--trace-summary', nargs=2, metavar=('KERNEL_TRACE_CSV', 'BENCH_JSON'),

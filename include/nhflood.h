@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 9
+#define NHF_ABI_VERSION 10
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -331,6 +331,55 @@ int32_t nhb_set_tuning(nhb_ctx *ctx, int32_t depth_blocks, int32_t cfl_blocks);
 int32_t nhb_tuning_range(nhb_ctx *ctx, int32_t *out4);
 int32_t nhb_set_timing(nhb_ctx *ctx, int32_t enable);
 int32_t nhb_kernel_times(nhb_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);
+
+/* ------------------------------------------------------------------------------------------
+ * Meshes of different sizes advanced by the same launches (nhu_*).
+ *
+ * No reference counterpart either: the reference runs one mesh per process. Several meshes
+ * ("parts") are concatenated into one nhf_mesh: part p's cells occupy [cell_begin, cell_end)
+ * with cell_begin = 1 + 1024 k and the range padded to a multiple of 1024 (padding cells have
+ * no sides and are never stepped); its sides likewise; every index inside a part's arrays
+ * (side cells, side lists, cell records, boundary ids in cell_meta, rain pixels, hanging-edge
+ * data) is global; cell 0 and side 0 are the one shared sentinel. Each part keeps its own clock,
+ * dt, forcing, CFL reduction, limiter capture and budget, so a part steps exactly as it would in
+ * a context of its own (nhf_create) on the same arrays; one nhu_run_batch advances every part
+ * with a begun target. Gate control and pipe exchange are not supported here. The nhf_* entry
+ * points that take no part (destroy, sync, graph, timing, tuning of the cell passes, settle
+ * extrema) apply to a parts context too; the others are replaced by the nhu_* forms below.
+ * ------------------------------------------------------------------------------------------ */
+#define NHF_MAX_PARTS 256
+
+typedef struct nhf_part {
+    int32_t cell_begin, cell_end;  /* real cells of the part */
+    int32_t side_begin, side_end;  /* real sides of the part (begin = 1 + 1024 k) */
+    int32_t b_base, b_count;       /* its prescribed boundaries (ids in cell_meta are global) */
+    int32_t pix_base, pix_count;   /* its rain pixels */
+} nhf_part;
+
+/* Workspace bytes of a parts context; parts is a host array of `count` entries that tile the
+ * mesh's cells, sides, boundaries and rain pixels in order. */
+size_t nhu_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params, const nhf_part *parts,
+                           int32_t count);
+/* nhf_create for parts; every part starts idle at tick 0. pipe_q must be NULL. */
+int32_t nhu_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, const nhf_part *parts,
+                   int32_t count, void *workspace, size_t workspace_bytes, void *stream);
+/* nhf_reset / nhf_set_forcing / nhf_begin of one part: coefficients [b_count*2] and rain_rates
+ * [pix_count] of that part only. */
+int32_t nhu_reset(nhf_ctx *ctx, int32_t part, uint64_t now_ticks);
+int32_t nhu_set_forcing(nhf_ctx *ctx, int32_t part, const float *coefficients, const float *rain_rates,
+                        int32_t raining);
+int32_t nhu_begin(nhf_ctx *ctx, int32_t part, uint64_t target_ticks, uint64_t origin_ticks);
+/* nhf_run_batch for every part at once; trace receives count x steps x 9 words. */
+int32_t nhu_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace);
+int32_t nhu_launch_batch(nhf_ctx *ctx, int32_t steps);
+int32_t nhu_wait_batch(nhf_ctx *ctx, uint32_t *trace);
+/* nhf_compute_cfl for every part. */
+int32_t nhu_compute_cfl(nhf_ctx *ctx);
+/* nhf_flush_budget of one part: out receives 2 * (4 + 6 * b_count) doubles. */
+int32_t nhu_flush_budget(nhf_ctx *ctx, int32_t part, double *out);
+/* nhf_limit_capture / nhf_cfl_state of one part (side and cell ids are global). */
+int32_t nhu_limit_capture(nhf_ctx *ctx, int32_t part, float *values, int32_t *ids, int32_t reset);
+int32_t nhu_cfl_state(nhf_ctx *ctx, int32_t part, float *ndt, int32_t *side);
 
 #ifdef __cplusplus
 }

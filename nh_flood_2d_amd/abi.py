@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -65,6 +65,16 @@ class GatesStruct(ctypes.Structure):
                                       'flags', 'ticks', 'acc')]
 
 
+class PartStruct(ctypes.Structure):
+    """nhf_part: one mesh of a parts context (cell, side, boundary and pixel ranges)."""
+    _fields_ = [(name, c_int32) for name in ('cell_begin', 'cell_end', 'side_begin', 'side_end', 'b_base',
+                                             'b_count', 'pix_base', 'pix_count')]
+
+
+MAX_PARTS = 256
+PART_ALIGN = 1024
+
+
 def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
     """Declare every entry point of include/nhflood.h on a loaded library."""
     vp = c_void_p
@@ -120,6 +130,22 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
             'nhb_tuning_range': (c_int32, [vp, vp]),
             'nhb_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
         })
+    if hasattr(lib, 'nhu_create'):
+        sig.update({
+            'nhu_workspace_bytes': (c_size_t, [POINTER(MeshStruct), POINTER(Params), POINTER(PartStruct), c_int32]),
+            'nhu_create': (c_int32, [POINTER(vp), POINTER(MeshStruct), POINTER(Params), POINTER(PartStruct),
+                                     c_int32, vp, c_size_t, vp]),
+            'nhu_reset': (c_int32, [vp, c_int32, c_uint64]),
+            'nhu_set_forcing': (c_int32, [vp, c_int32, vp, vp, c_int32]),
+            'nhu_begin': (c_int32, [vp, c_int32, c_uint64, c_uint64]),
+            'nhu_run_batch': (c_int32, [vp, c_int32, vp]),
+            'nhu_launch_batch': (c_int32, [vp, c_int32]),
+            'nhu_wait_batch': (c_int32, [vp, vp]),
+            'nhu_compute_cfl': (c_int32, [vp]),
+            'nhu_flush_budget': (c_int32, [vp, c_int32, vp]),
+            'nhu_limit_capture': (c_int32, [vp, c_int32, vp, vp, c_int32]),
+            'nhu_cfl_state': (c_int32, [vp, c_int32, vp, vp]),
+        })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -139,7 +165,10 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
             'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
             'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times', 'nhb_set_tuning',
-            'nhb_tuning_range']
+            'nhb_tuning_range',
+            'nhu_workspace_bytes', 'nhu_create', 'nhu_reset', 'nhu_set_forcing', 'nhu_begin', 'nhu_run_batch',
+            'nhu_launch_batch', 'nhu_wait_batch', 'nhu_compute_cfl', 'nhu_flush_budget', 'nhu_limit_capture',
+            'nhu_cfl_state']
 
 _hip_lib = None
 
@@ -159,6 +188,6 @@ def check(lib, rc: int, what: str) -> None:
         raise RuntimeError(f'{what} failed ({rc}): {lib.nhf_last_error().decode(errors="replace")}')
 
 
-__all__ = ['Params', 'MeshStruct', 'MESH_FIELDS', 'GatesStruct', 'BatchState', 'BATCH_STATE_FIELDS', 'MAX_SLOTS', 'bind', 'load_hip_library', 'check',
+__all__ = ['Params', 'MeshStruct', 'MESH_FIELDS', 'GatesStruct', 'PartStruct', 'MAX_PARTS', 'PART_ALIGN', 'BatchState', 'BATCH_STATE_FIELDS', 'MAX_SLOTS', 'bind', 'load_hip_library', 'check',
            'TRACE_COLS', 'MAX_BATCH', 'DONE', 'COMPLETED', 'BAD_CFL', 'BAD_DT', 'OVERSHOOT',
            'c_double', 'c_int64', 'c_uint32']

@@ -31,10 +31,24 @@ thread_local char nhf_error_text[512];
 
 // --------------------------------------------------------------------------- device state
 
+// Parts: several meshes concatenated into one (nhu_create), each with its own clock, forcing and
+// budget. Every part's cells start at 1 + 1024 k and its sides likewise, so a workgroup of a cell
+// or side pass lies in one part and finds it with one load.
+#define NHF_PART_ALIGN 1024
+struct PartDev {
+    int32_t count;                // 0: one mesh (nhf_create)
+    int32_t pad;
+    const int32_t *cell_chunk;    // part of cells 1 + 1024 k .. 1024 (k + 1)
+    const int32_t *side_chunk;    // part of sides 1 + 1024 k .. 1024 (k + 1)
+    const int4 *range;            // per part: cell begin, cell end, side begin, side end
+    const int4 *extra;            // per part: first boundary, boundaries, first CFL workgroup, CFL workgroups
+    const int32_t *cfl_part;      // part of each CFL workgroup
+};
+
 struct Dev {
     nhf_mesh m;
     nhf_params p;
-    Ctl *ctl;
+    Ctl *ctl;  // one per part
     float *coefficients, *values, *level_values, *rain_rates;
     float *donor, *face_qn, *grad;
     uint64_t *cfl_key;
@@ -46,18 +60,37 @@ struct Dev {
     int32_t acc_owners;  // budget accumulators allocated (wavefronts of the largest grid)
     int32_t gates_on;    // gate control bound (nhf_set_gates)
     nhf_gates g;
+    PartDev parts;
 };
 
-// prepare_step (solver_compact.py:364-382), run by one thread.
-__device__ void prepare_slot(const Dev &d, int slot) {
-    Ctl *c = d.ctl;
+// The part of a workgroup whose first cell (side) is `first`; 0 for a single mesh.
+template <bool PARTS>
+static __device__ __forceinline__ int cell_part(const Dev &d, int first) {
+    return PARTS ? d.parts.cell_chunk[(first - 1) >> 10] : 0;
+}
+template <bool PARTS>
+static __device__ __forceinline__ int side_part(const Dev &d, int first) {
+    return PARTS ? d.parts.side_chunk[(first - 1) >> 10] : 0;
+}
+
+// Boundaries [base, base + count) of a part: all of them for a single mesh.
+static __device__ __forceinline__ int2 part_boundaries(const Dev &d, int part) {
+    if (d.parts.count == 0) return make_int2(0, d.m.b_num);
+    const int4 e = d.parts.extra[part];
+    return make_int2(e.x, e.y);
+}
+
+// prepare_step (solver_compact.py:364-382) of one part, run by one thread.
+__device__ void prepare_slot(const Dev &d, int part, int slot) {
+    Ctl *c = d.ctl + part;
     clock_choose(c, c->ndt, d.p.min_dt, slot);
     if (!c->active) return;
     c->limited_cells = 0;
     c->limited_faces = 0;
     c->min_scale_bits = __float_as_uint(1.0f);
     const float dt = c->dt, local = c->local, end_local = c->end_local;
-    for (int bi = 0; bi < d.m.b_num; ++bi) {
+    const int2 bounds = part_boundaries(d, part);
+    for (int bi = bounds.x; bi < bounds.x + bounds.y; ++bi) {
         float c0 = d.coefficients[2 * bi], c1 = d.coefficients[2 * bi + 1];
         d.values[bi] = c0 + c1 * (local + dt / 2.0f);
         d.level_values[bi] = c0 + c1 * end_local;
@@ -81,11 +114,20 @@ __device__ void prepare_slot(const Dev &d, int slot) {
 
 // --------------------------------------------------------------------------- small kernels
 
-__global__ void k_reset(Dev d, uint64_t now) {
+// A part's cells [begin, padded end) and budget accumulators [first, last): the whole mesh for a
+// single mesh. Accumulator k belongs to the wavefront whose first cell is 1 + 64 k.
+static __device__ __forceinline__ int4 part_span(const Dev &d, int part) {
+    if (d.parts.count == 0) return make_int4(0, d.m.e_num, 0, d.acc_owners);
+    const int4 r = d.parts.range[part];
+    const int end = r.x + ((r.y - r.x + NHF_PART_ALIGN - 1) / NHF_PART_ALIGN) * NHF_PART_ALIGN;
+    return make_int4(r.x, end, (r.x - 1) >> 6, (end - 1) >> 6);
+}
+
+__global__ void k_reset(Dev d, int part, uint64_t now) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     int stride = gridDim.x * blockDim.x;
     if (tid == 0) {
-        Ctl *c = d.ctl;
+        Ctl *c = d.ctl + part;
         c->now = c->next = c->origin = c->step_end = now;
         c->dt = c->local = c->end_local = 0.0f;
         c->active = 0;
@@ -94,14 +136,15 @@ __global__ void k_reset(Dev d, uint64_t now) {
         c->min_scale_bits = __float_as_uint(1.0f);
         c->limit_value[0] = INFINITY;
     }
-    for (int i = tid; i < d.acc_owners * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
-    for (int i = tid; i < d.m.e_num; i += stride) d.donor[i] = 0.0f;
+    const int4 span = part_span(d, part);
+    for (int i = span.z * d.terms + tid; i < span.w * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
+    for (int i = span.x + tid; i < span.y; i += stride) d.donor[i] = 0.0f;
     if (d.face_qn)
-        for (int i = tid; i < d.m.e_num * 4; i += stride) d.face_qn[i] = 0.0f;
+        for (int i = 4 * span.x + tid; i < 4 * span.y; i += stride) d.face_qn[i] = 0.0f;
 }
 
-__global__ void k_begin(Dev d, uint64_t target, uint64_t origin) {
-    Ctl *c = d.ctl;
+__global__ void k_begin(Dev d, int part, uint64_t target, uint64_t origin) {
+    Ctl *c = d.ctl + part;
     for (int i = threadIdx.x; i < NHF_MAX_BATCH * NHF_TRACE_COLS; i += blockDim.x) c->trace[i] = 0u;
     if (threadIdx.x == 0) {
         c->next = target;
@@ -110,38 +153,40 @@ __global__ void k_begin(Dev d, uint64_t target, uint64_t origin) {
     }
 }
 
-__global__ void k_set_raining(Dev d, int raining) {
-    if (threadIdx.x == 0) d.ctl->raining = raining;
+__global__ void k_set_raining(Dev d, int part, int raining) {
+    if (threadIdx.x == 0) d.ctl[part].raining = raining;
 }
 
-// First slot of a batch: clear the trace rows, then prepare slot 0.
+// First slot of a batch: clear the trace rows, then prepare slot 0 (workgroup p: part p).
 __global__ void k_prepare(Dev d, int slot) {
-    for (int i = threadIdx.x; i < NHF_MAX_BATCH * NHF_TRACE_COLS; i += blockDim.x) d.ctl->trace[i] = 0u;
+    const int part = blockIdx.x;
+    for (int i = threadIdx.x; i < NHF_MAX_BATCH * NHF_TRACE_COLS; i += blockDim.x) d.ctl[part].trace[i] = 0u;
     __syncthreads();
-    if (threadIdx.x == 0) prepare_slot(d, slot);
+    if (threadIdx.x == 0) prepare_slot(d, part, slot);
 }
 
-__global__ void k_clear_acc(Dev d) {
+__global__ void k_clear_acc(Dev d, int part) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int i = tid; i < d.acc_owners * d.terms; i += gridDim.x * blockDim.x)
+    const int4 span = part_span(d, part);
+    for (int i = span.z * d.terms + tid; i < span.w * d.terms; i += gridDim.x * blockDim.x)
         d.acc[i] = make_double2(0.0, 0.0);
 }
 
 // Writes out the deferred extrema: a cell marked as rising (max_depth < 0) gets its depth as
-// maximum and the clock's current tick (the end of the last step) as peak tick.
+// maximum and its clock's current tick (the end of its last step) as peak tick.
 __global__ __launch_bounds__(NHF_BLOCK) void k_settle_extrema(Dev d) {
     const nhf_mesh &m = d.m;
-    const uint64_t now = d.ctl->now;
     for (int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1; cell < m.e_num; cell += gridDim.x * NHF_BLOCK) {
         if (m.max_depth[cell] < 0.0f) {
+            const int part = d.parts.count ? d.parts.cell_chunk[(cell - 1) >> 10] : 0;
             m.max_depth[cell] = m.depth[cell];
-            m.peak_tick[cell] = now;
+            m.peak_tick[cell] = d.ctl[part].now;
         }
     }
 }
 
-__global__ void k_reset_limit(Dev d) {
-    if (threadIdx.x == 0) d.ctl->limit_value[0] = INFINITY;
+__global__ void k_reset_limit(Dev d, int part) {
+    if (threadIdx.x == 0) d.ctl[part].limit_value[0] = INFINITY;
 }
 
 __global__ void k_clock_probe(const uint64_t *now, const uint64_t *target, const uint64_t *origin,
@@ -204,9 +249,11 @@ __device__ void face_difference(const Dev &d, int cell, int dir, int axis, float
     usable = ok;
 }
 
+template <bool PARTS>
 static __device__ __forceinline__ void gradient_slot(const Dev &d, int slot) {
     if (slot >= d.m.n_grad) return;
     int cell = d.m.grad_cell[slot];
+    if (PARTS && !d.ctl[cell_part<true>(d, cell)].active) return;
     int axes = d.m.grad_axes[slot];
     float h = d.m.depth[cell];
     for (int axis = 0; axis < 2; ++axis) {
@@ -228,17 +275,20 @@ static __device__ __forceinline__ void gradient_slot(const Dev &d, int slot) {
     }
 }
 
+template <bool PARTS>
 __global__ __launch_bounds__(NHF_BLOCK) void k_gradients(Dev d) {
-    if (!d.ctl->active) return;
-    gradient_slot(d, blockIdx.x * NHF_BLOCK + threadIdx.x);
+    if (!PARTS && !d.ctl->active) return;
+    gradient_slot<PARTS>(d, blockIdx.x * NHF_BLOCK + threadIdx.x);
 }
 
 // --------------------------------------------------------------------------- theta < 1
 
+template <bool PARTS>
 __global__ __launch_bounds__(NHF_BLOCK) void k_face_avg(Dev d) {
-    if (!d.ctl->active) return;
+    if (!PARTS && !d.ctl->active) return;
     int cell = blockIdx.x * NHF_BLOCK + threadIdx.x;
     if (cell >= d.m.e_num) return;
+    if (PARTS && cell > 0 && !d.ctl[cell_part<true>(d, cell)].active) return;
     const nhf_mesh &m = d.m;
     uint32_t cnt = m.csr_count[cell];
     int start = m.csr_start[cell];
@@ -314,18 +364,27 @@ static __device__ __forceinline__ float flux_value(const Dev &d, int si, float d
 }
 
 // A workgroup takes NHF_BLOCK * SIDES consecutive sides; thread t the sides t, t + NHF_BLOCK, ...
-template <int SIDES>
+template <int SIDES, bool PARTS>
 __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
-    if (!d.ctl->active) return;
     const nhf_mesh &m = d.m;
-    const int first = xcd_block() * NHF_BLOCK * SIDES + threadIdx.x + 1;
-    if (first >= m.s_num) return;
-    const float dt = d.ctl->dt;
+    const int block_first = xcd_block() * NHF_BLOCK * SIDES + 1;
+    const Ctl *c = d.ctl;
+    int end = m.s_num;
+    if (PARTS) {
+        if (block_first >= m.s_num) return;
+        const int part = side_part<true>(d, block_first);
+        c = d.ctl + part;
+        end = d.parts.range[part].w;
+    }
+    if (!c->active) return;
+    const int first = block_first + threadIdx.x;
+    if (first >= end) return;
+    const float dt = c->dt;
     FluxSide f[SIDES];
 #pragma unroll
     for (int u = 0; u < SIDES; ++u) {
         const int si = first + u * NHF_BLOCK;
-        if (si < m.s_num) {
+        if (si < end) {
             f[u].lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
             f[u].meta = m.side_meta[si];
             f[u].q = m.q[si];
@@ -333,7 +392,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
     }
 #pragma unroll
     for (int u = 0; u < SIDES; ++u) {
-        if (first + u * NHF_BLOCK < m.s_num) {
+        if (first + u * NHF_BLOCK < end) {
             f[u].zl = m.bed[f[u].lh.x];
             f[u].zh = m.bed[f[u].lh.y];
             f[u].dl = m.depth[f[u].lh.x];
@@ -343,12 +402,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
     float nq[SIDES];
 #pragma unroll
     for (int u = 0; u < SIDES; ++u)
-        if (first + u * NHF_BLOCK < m.s_num) nq[u] = flux_value(d, first + u * NHF_BLOCK, dt, f[u]);
+        if (first + u * NHF_BLOCK < end) nq[u] = flux_value(d, first + u * NHF_BLOCK, dt, f[u]);
     // In place, as the reference's sq_t (solver_compact.py:486): no other thread of this pass
     // reads these sides' discharges. k_donor scales them afterwards where their donor is limited.
 #pragma unroll
     for (int u = 0; u < SIDES; ++u)
-        if (first + u * NHF_BLOCK < m.s_num) m.q[first + u * NHF_BLOCK] = nq[u];
+        if (first + u * NHF_BLOCK < end) m.q[first + u * NHF_BLOCK] = nq[u];
 }
 
 // --------------------------------------------------------------------------- donor scale
@@ -476,18 +535,26 @@ static __device__ __forceinline__ void load_donor_cell(const nhf_mesh &m, int ce
 
 // CPT cells per thread (1 or 2): thread t of a workgroup takes cells t, t + BLOCK, ... of the
 // workgroup's BLOCK*CPT cells, and issues the first-round loads of all of them before using any.
-template <int BLOCK, int CPT>
+template <int BLOCK, int CPT, bool PARTS>
 __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
-    Ctl *c = d.ctl;
-    if (!c->active) return;
     const nhf_mesh &m = d.m;
+    const int block_first = xcd_block() * BLOCK * CPT + 1;
+    Ctl *c = d.ctl;
+    int end = m.e_num;
+    if (PARTS) {
+        if (block_first >= m.e_num) return;
+        const int part = cell_part<true>(d, block_first);
+        c = d.ctl + part;
+        end = d.parts.range[part].y;
+    }
+    if (!c->active) return;
     const float dt = c->dt;
     const bool raining = c->raining;
-    const int first = xcd_block() * BLOCK * CPT + threadIdx.x + 1;
+    const int first = block_first + threadIdx.x;
     uint32_t limited = 0, faces = 0, scale_bits = __float_as_uint(1.0f);
     // first round: the cells' own values and their side records
     if constexpr (CPT == 1) {   // the plain form keeps every value in registers
-        const bool live = first < m.e_num;
+        const bool live = first < end;
         DonorCell v = {};
         if (live) load_donor_cell(m, first, raining, v);
         if (__all(!live || v.rec.x >= 0)) {
@@ -498,7 +565,7 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
     } else {   // two cells: both first rounds, then both bodies (named values, no arrays)
         static_assert(CPT == 2, "one or two cells per thread");
         const int second = first + BLOCK;
-        const bool live0 = first < m.e_num, live1 = second < m.e_num;
+        const bool live0 = first < end, live1 = second < end;
         DonorCell v0 = {}, v1 = {};
         if (live0) load_donor_cell(m, first, raining, v0);
         if (live1) load_donor_cell(m, second, raining, v1);
@@ -725,22 +792,34 @@ static __device__ __forceinline__ void load_depth_cell(const nhf_mesh &m, int ce
     v.rec = cell_record(m, cell);
 }
 
-template <int BLOCK, int CPT>
+template <int BLOCK, int CPT, bool PARTS>
 __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
-    Ctl *c = d.ctl;
-    if (!c->active) return;
     const nhf_mesh &m = d.m;
+    const int block_first = xcd_block() * BLOCK * CPT + 1;
+    Ctl *c = d.ctl;
+    int end = m.e_num;
+    int2 bounds = make_int2(0, m.b_num);  // the part's boundaries: first, count
+    if (PARTS) {
+        if (block_first >= m.e_num) return;
+        const int part = cell_part<true>(d, block_first);
+        c = d.ctl + part;
+        end = d.parts.range[part].y;
+        const int4 e = d.parts.extra[part];
+        bounds = make_int2(e.x, e.y);
+    }
+    if (!c->active) return;
     const float dt = c->dt;
     const bool raining = c->raining;
     const uint64_t step_start = c->now;  // the clock commits this step's end in k_finish
-    const int b_num = m.b_num;
-    const int first = xcd_block() * BLOCK * CPT + threadIdx.x + 1;
+    const int b_num = bounds.y;
+    const int first = block_first + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double2 *acc = d.acc + ((size_t)blockIdx.x * (BLOCK / 64) + wave) * d.terms;
+    // accumulator k belongs to the wavefront whose first cell is 1 + 64 k (any workgroup shape)
+    double2 *acc = d.acc + ((size_t)((block_first - 1) >> 6) + wave) * d.terms;
     // first round: the cells' own values and their side records
     DepthTerms t[CPT];
     if constexpr (CPT == 1) {   // the plain form keeps every value in registers
-        const bool live = first < m.e_num;
+        const bool live = first < end;
         DepthCell v = {};
         if (live) load_depth_cell(m, first, raining, v);
         t[0] = DepthTerms{};
@@ -754,7 +833,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     } else {   // two cells: both first rounds, then both bodies (named values, no arrays)
         static_assert(CPT == 2, "one or two cells per thread");
         const int second = first + BLOCK;
-        const bool live0 = first < m.e_num, live1 = second < m.e_num;
+        const bool live0 = first < end, live1 = second < end;
         DepthCell v0 = {}, v1 = {};
         if (live0) load_depth_cell(m, first, raining, v0);
         if (live1) load_depth_cell(m, second, raining, v1);
@@ -815,8 +894,8 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
             bool level = false, inflow = false;
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
-                level = level || (t[j].kind == 0 && t[j].bi == tb);
-                inflow = inflow || (t[j].kind == 1 && t[j].bi == tb);
+                level = level || (t[j].kind == 0 && t[j].bi == bounds.x + tb);
+                inflow = inflow || (t[j].kind == 1 && t[j].bi == bounds.x + tb);
             }
             if (__any(level)) {
 #pragma unroll
@@ -824,7 +903,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
                     double x = 0.0;
 #pragma unroll
                     for (int j = 0; j < CPT; ++j)
-                        x += (double)((t[j].kind == 0 && t[j].bi == tb) ? t[j].lv[comp] : 0.0f);
+                        x += (double)((t[j].kind == 0 && t[j].bi == bounds.x + tb) ? t[j].lv[comp] : 0.0f);
                     add_term(4 + 5 * tb + comp, x);
                 }
             }
@@ -832,7 +911,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
                 double x = 0.0;
 #pragma unroll
                 for (int j = 0; j < CPT; ++j)
-                    x += (double)((t[j].kind == 1 && t[j].bi == tb) ? t[j].disc : 0.0f);
+                    x += (double)((t[j].kind == 1 && t[j].bi == bounds.x + tb) ? t[j].disc : 0.0f);
                 add_term(4 + 5 * b_num + tb, x);
             }
         }
@@ -949,15 +1028,27 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_gate_account(Dev d) {
 // The first `grad_blocks` workgroups (a multiple of 8, so each CFL workgroup keeps its XCD)
 // compute the next step's hanging-edge gradients instead: they need only the depth and bed this
 // step ends with, which are final once the depth pass is done.
+// With parts, each part has its own range of CFL workgroups (a multiple of 8, starting at a
+// multiple of 8) striding over its own sides.
+template <bool PARTS>
 __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int grad_blocks) {
-    Ctl *c = d.ctl;
-    if (mode == 1 && !c->active) return;
+    if (!PARTS && mode == 1 && !d.ctl->active) return;
     if ((int)blockIdx.x < grad_blocks) {
-        gradient_slot(d, blockIdx.x * NHF_BLOCK + threadIdx.x);
+        gradient_slot<PARTS>(d, blockIdx.x * NHF_BLOCK + threadIdx.x);
         return;
     }
     const int block = (int)blockIdx.x - grad_blocks;
-    const int xb = (block & 7) * (d.cfl_blocks >> 3) + (block >> 3);
+    int begin = 1, end = d.m.s_num, count = d.cfl_blocks, lb = block;
+    if (PARTS) {
+        const int part = d.parts.cfl_part[block];
+        if (mode == 1 && !d.ctl[part].active) return;
+        const int4 r = d.parts.range[part], e = d.parts.extra[part];
+        begin = r.z;
+        end = r.w;
+        count = e.w;
+        lb = block - e.z;
+    }
+    const int xb = (lb & 7) * (count >> 3) + (lb >> 3);
     __shared__ uint64_t s_key[NHF_WAVES];
     __shared__ int32_t s_idx[NHF_WAVES];
     const nhf_mesh &m = d.m;
@@ -966,7 +1057,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int grad_blo
     // where the bounds tie, and for the thread's winner at the end.
     uint32_t best_bits = ~0u;
     int32_t idx = -1;
-    for (int si = xb * NHF_BLOCK + threadIdx.x + 1; si < m.s_num; si += d.cfl_blocks * NHF_BLOCK) {
+    for (int si = begin + xb * NHF_BLOCK + threadIdx.x; si < end; si += count * NHF_BLOCK) {
         int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
         if (lh.x != 0 && lh.y != 0) {
             float zl = m.bed[lh.x], zh = m.bed[lh.y];
@@ -1002,14 +1093,22 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_cfl(Dev d, int mode, int grad_blo
 
 // One workgroup: reduce the CFL partials to ndt and its side; in a batch (mode 1) also commit
 // `slot` (finish_step) and prepare `slot + 1` (prepare_step) when it is below `steps`.
+template <bool PARTS>
 __global__ __launch_bounds__(1024) void k_finish(Dev d, int mode, int slot, int steps) {
-    Ctl *c = d.ctl;
+    const int part = PARTS ? (int)blockIdx.x : 0;  // one workgroup per part
+    Ctl *c = d.ctl + part;
     if (mode == 1 && !c->active) return;
     __shared__ uint64_t s_key[16];
     __shared__ int32_t s_idx[16];
     uint64_t key = ~0ull;
     int32_t idx = 0;
-    for (int b = threadIdx.x; b < d.cfl_blocks; b += blockDim.x) {
+    int b0 = 0, b1 = d.cfl_blocks;
+    if (PARTS) {
+        const int4 e = d.parts.extra[part];
+        b0 = e.z;
+        b1 = e.z + e.w;
+    }
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
         uint64_t k2 = d.cfl_key[b];
         if (k2 < key) {
             key = k2;
@@ -1038,7 +1137,7 @@ __global__ __launch_bounds__(1024) void k_finish(Dev d, int mode, int slot, int 
         }
         if (mode == 1) {
             commit_slot(c, slot);
-            if (slot + 1 < steps) prepare_slot(d, slot + 1);
+            if (slot + 1 < steps) prepare_slot(d, part, slot + 1);
         }
     }
 }
@@ -1049,16 +1148,23 @@ enum { K_PREPARE = 0, K_GRAD, K_FACE, K_FLUX, K_DONOR, K_DEPTH, K_CFL, K_FINISH,
 
 // The cell passes' instantiation for the context's workgroup size and cells per thread.
 #define NHF_CELL_PASS(KERNEL)                                                                       \
-    static void launch_##KERNEL(const Dev &d, hipStream_t s) {                                     \
+    template <bool PARTS>                                                                          \
+    static void launch_##KERNEL##_as(const Dev &d, hipStream_t s) {                                \
         const dim3 grid(d.depth_blocks);                                                           \
         switch (d.cell_block * 10 + d.cells_per_thread) {                                          \
-        case 1281: hipLaunchKernelGGL((KERNEL<128, 1>), grid, dim3(128), 0, s, d); break;           \
-        case 1282: hipLaunchKernelGGL((KERNEL<128, 2>), grid, dim3(128), 0, s, d); break;           \
-        case 2562: hipLaunchKernelGGL((KERNEL<256, 2>), grid, dim3(256), 0, s, d); break;           \
-        case 5121: hipLaunchKernelGGL((KERNEL<512, 1>), grid, dim3(512), 0, s, d); break;           \
-        case 5122: hipLaunchKernelGGL((KERNEL<512, 2>), grid, dim3(512), 0, s, d); break;           \
-        default: hipLaunchKernelGGL((KERNEL<256, 1>), grid, dim3(256), 0, s, d); break;             \
+        case 1281: hipLaunchKernelGGL((KERNEL<128, 1, PARTS>), grid, dim3(128), 0, s, d); break;    \
+        case 1282: hipLaunchKernelGGL((KERNEL<128, 2, PARTS>), grid, dim3(128), 0, s, d); break;    \
+        case 2562: hipLaunchKernelGGL((KERNEL<256, 2, PARTS>), grid, dim3(256), 0, s, d); break;    \
+        case 5121: hipLaunchKernelGGL((KERNEL<512, 1, PARTS>), grid, dim3(512), 0, s, d); break;    \
+        case 5122: hipLaunchKernelGGL((KERNEL<512, 2, PARTS>), grid, dim3(512), 0, s, d); break;    \
+        default: hipLaunchKernelGGL((KERNEL<256, 1, PARTS>), grid, dim3(256), 0, s, d); break;      \
         }                                                                                          \
+    }                                                                                              \
+    static void launch_##KERNEL(const Dev &d, hipStream_t s) {                                     \
+        if (d.parts.count)                                                                         \
+            launch_##KERNEL##_as<true>(d, s);                                                      \
+        else                                                                                       \
+            launch_##KERNEL##_as<false>(d, s);                                                     \
     }
 NHF_CELL_PASS(k_donor)
 NHF_CELL_PASS(k_depth)
@@ -1079,12 +1185,17 @@ struct nhf_ctx {
     std::vector<int32_t> gate_chunks;  // host copy of the gate accounting chunks
     hipEvent_t done;    // recorded after a launched batch's trace copy
     int pending_steps;  // steps of the batch in flight, 0 when none
+    int parts;          // 0: one mesh (nhf_create); else the parts of nhu_create
+    std::vector<int4> part_range, part_extra;
+    std::vector<int2> part_pixels;
+    Ctl *host_ctl;      // pinned copy of every part's control block (parts contexts)
 };
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Carve {
-    size_t ctl, coeff, values, level, rain, donor, face_qn, grad, key, idx, acc, total;
+    size_t ctl, cell_chunk, side_chunk, range, cfl_part, coeff, values, level, rain, donor, face_qn, grad, key, idx,
+        acc, total;
 };
 
 static int blocks_for(int n) { return (n + NHF_BLOCK - 1) / NHF_BLOCK; }
@@ -1098,7 +1209,10 @@ static int flux_blocks(int sides) {
     return ((sides + per - 1) / per + 7) & ~7;
 }
 static void launch_k_flux(const Dev &d, int sides, hipStream_t s) {
-    hipLaunchKernelGGL(k_flux<NHF_FLUX_SIDES>, dim3(flux_blocks(sides)), dim3(NHF_BLOCK), 0, s, d);
+    if (d.parts.count)
+        hipLaunchKernelGGL((k_flux<NHF_FLUX_SIDES, true>), dim3(flux_blocks(sides)), dim3(NHF_BLOCK), 0, s, d);
+    else
+        hipLaunchKernelGGL((k_flux<NHF_FLUX_SIDES, false>), dim3(flux_blocks(sides)), dim3(NHF_BLOCK), 0, s, d);
 }
 
 // CFL partials: the workspace holds up to NHF_CFL_MAX_BLOCKS (no more than the sides need); the
@@ -1122,26 +1236,79 @@ static int cell_grid(int e_num, int block) {
     return ((n + block - 1) / block + 7) & ~7;
 }
 
-// Budget accumulators: one per wavefront of the cell pass, for the largest of the supported
-// workgroup sizes' grids, so a change of workgroup size keeps every accumulated term.
+// Budget accumulators: accumulator k belongs to the wavefront whose first cell is 1 + 64 k, so
+// one per 64 cells of the largest grid of any workgroup shape; a change of shape keeps every
+// accumulated term.
 static int acc_owner_count(int e_num) {
     int most = 0;
-    for (int b = 128; b <= 512; b *= 2) {
-        int owners = cell_grid(e_num, b) * (b / 64);
-        most = owners > most ? owners : most;
-    }
+    for (int b = 128; b <= 512; b *= 2)
+        for (int cpt = 1; cpt <= 2; ++cpt) {
+            int owners = cell_grid(e_num, b * cpt) * (b * cpt / 64);
+            most = owners > most ? owners : most;
+        }
     return most;
 }
 
-static Carve carve(const nhf_mesh *m, const nhf_params *p) {
+// Parts of a concatenated mesh (nhu_create): ranges checked, and the device tables.
+struct PartPlan {
+    std::vector<int4> range, extra;  // as PartDev
+    std::vector<int32_t> cell_chunk, side_chunk, cfl_part;
+    std::vector<int2> pixels;        // per part: first rain pixel, pixels
+    int cfl_total = 0, max_b = 0;
+};
+
+static int padded(int n) { return (n + NHF_PART_ALIGN - 1) / NHF_PART_ALIGN * NHF_PART_ALIGN; }
+
+// CFL workgroups of a part with `sides` sides: about six sides per thread, a multiple of 8 in
+// [8, 2048] (the single-mesh default grid).
+static int part_cfl_blocks(int sides) {
+    int n = (sides + 6 * NHF_BLOCK - 1) / (6 * NHF_BLOCK);
+    n = (n + 7) & ~7;
+    return n < 8 ? 8 : n > 2048 ? 2048 : n;
+}
+
+static const char *plan_parts(const nhf_mesh *m, const nhf_part *parts, int count, PartPlan &plan) {
+    if (count < 1 || count > NHF_MAX_PARTS) return "part count out of range";
+    int cell = 1, side = 1, bnd = 0, pix = 0;
+    for (int p = 0; p < count; ++p) {
+        const nhf_part &q = parts[p];
+        if (q.cell_begin != cell || q.cell_end <= q.cell_begin || q.side_begin != side || q.side_end < q.side_begin)
+            return "parts must tile the cells and sides in order, each starting at 1 + 1024 k";
+        if (q.b_base != bnd || q.b_count < 0 || q.pix_base != pix || q.pix_count < 1)
+            return "parts must tile the boundaries and rain pixels in order";
+        cell = q.cell_begin + padded(q.cell_end - q.cell_begin);
+        side = q.side_begin + padded(q.side_end - q.side_begin > 0 ? q.side_end - q.side_begin : 1);
+        bnd += q.b_count;
+        pix += q.pix_count;
+        const int blocks = part_cfl_blocks(q.side_end - q.side_begin);
+        plan.range.push_back(make_int4(q.cell_begin, q.cell_end, q.side_begin, q.side_end));
+        plan.extra.push_back(make_int4(q.b_base, q.b_count, plan.cfl_total, blocks));
+        plan.pixels.push_back(make_int2(q.pix_base, q.pix_count));
+        for (int k = 0; k < blocks; ++k) plan.cfl_part.push_back(p);
+        plan.cfl_total += blocks;
+        if (q.b_count > plan.max_b) plan.max_b = q.b_count;
+        for (int k = 0; k < padded(q.cell_end - q.cell_begin) / NHF_PART_ALIGN; ++k) plan.cell_chunk.push_back(p);
+        for (int k = 0; k < padded(q.side_end - q.side_begin > 0 ? q.side_end - q.side_begin : 1) / NHF_PART_ALIGN; ++k)
+            plan.side_chunk.push_back(p);
+    }
+    if (cell != m->e_num || side != m->s_num) return "the parts' padded ranges must end at e_num and s_num";
+    if (bnd != m->b_num || pix != m->n_pixels) return "the parts' boundaries and pixels must sum to b_num and n_pixels";
+    return nullptr;
+}
+
+static Carve carve(const nhf_mesh *m, const nhf_params *p, const PartPlan *plan = nullptr) {
     Carve c;
     size_t off = 0;
     int b = m->b_num > 0 ? m->b_num : 1;
-    int terms = 4 + 6 * m->b_num;
-    int cfl_blocks = cfl_max_blocks(m->s_num);
-    int depth_blocks = blocks_xcd(m->e_num - 1 > 0 ? m->e_num - 1 : 1);
+    int terms = 4 + 6 * (plan ? plan->max_b : m->b_num);
+    int cfl_blocks = plan ? plan->cfl_total : cfl_max_blocks(m->s_num);
     bool theta_one = p->theta == 1.0f;
-    c.ctl = off; off = align_up(off + sizeof(Ctl));
+    const int parts = plan ? (int)plan->range.size() : 1;
+    c.ctl = off; off = align_up(off + sizeof(Ctl) * parts);
+    c.cell_chunk = off; off = align_up(off + (plan ? sizeof(int32_t) * plan->cell_chunk.size() : 0));
+    c.side_chunk = off; off = align_up(off + (plan ? sizeof(int32_t) * plan->side_chunk.size() : 0));
+    c.range = off; off = align_up(off + (plan ? sizeof(int4) * 2 * parts : 0));
+    c.cfl_part = off; off = align_up(off + (plan ? sizeof(int32_t) * plan->cfl_part.size() : 0));
     c.coeff = off; off = align_up(off + sizeof(float) * 2 * b);
     c.values = off; off = align_up(off + sizeof(float) * b);
     c.level = off; off = align_up(off + sizeof(float) * b);
@@ -1167,13 +1334,13 @@ size_t nhf_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params) {
     return carve(mesh, params).total;
 }
 
-int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, void *workspace,
-                   size_t workspace_bytes, void *stream) {
+static int32_t create_ctx(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, const PartPlan *plan,
+                          void *workspace, size_t workspace_bytes, void *stream) {
     if (!out || !mesh || !params || !workspace) return set_error_text("nhf_create: null argument");
     if (params->batch_capacity < 1 || params->batch_capacity > NHF_MAX_BATCH)
         return set_error_text("nhf_create: batch_capacity out of range");
     if (mesh->e_num < 2 || mesh->s_num < 1) return set_error_text("nhf_create: empty mesh");
-    Carve cv = carve(mesh, params);
+    Carve cv = carve(mesh, params, plan);
     if (workspace_bytes < cv.total) return set_error_text("nhf_create: workspace too small");
     if ((params->theta != 1.0f) && !mesh->open_face)
         return set_error_text("nhf_create: open_face is required when sita != 1");
@@ -1181,12 +1348,26 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
         return set_error_text("nhf_create: hanging reconstruction arrays missing");
     if ((mesh->pipe_volume != nullptr) != (mesh->pipe_correction != nullptr) || (mesh->pipe_volume && !mesh->pipe_q))
         return set_error_text("nhf_create: pipe volume and correction go together, with pipe_q");
+    if (plan && mesh->pipe_q) return set_error_text("nhu_create: pipe exchange is not supported with parts");
     if (const char *name = missing_mesh_array(mesh, true, true)) {
         snprintf(nhf_error_text, sizeof nhf_error_text, "nhf_create: mesh array %s is NULL", name);
         return -2;
     }
-    nhf_ctx *ctx = new nhf_ctx();
     char *ws = (char *)workspace;
+    if (plan) {
+        // the part tables, copied once into the workspace
+        HIP_TRY(hipMemcpy(ws + cv.cell_chunk, plan->cell_chunk.data(), sizeof(int32_t) * plan->cell_chunk.size(),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ws + cv.side_chunk, plan->side_chunk.data(), sizeof(int32_t) * plan->side_chunk.size(),
+                          hipMemcpyHostToDevice));
+        std::vector<int4> table(plan->range);
+        table.insert(table.end(), plan->extra.begin(), plan->extra.end());
+        HIP_TRY(hipMemcpy(ws + cv.range, table.data(), sizeof(int4) * table.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ws + cv.cfl_part, plan->cfl_part.data(), sizeof(int32_t) * plan->cfl_part.size(),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(ws + cv.ctl, 0, sizeof(Ctl) * plan->range.size()));
+    }
+    nhf_ctx *ctx = new nhf_ctx();
     Dev &d = ctx->d;
     d.m = *mesh;
     if (mesh->n_hang == 0) d.m.edge_index = nullptr;
@@ -1202,30 +1383,50 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     d.cfl_key = (uint64_t *)(ws + cv.key);
     d.cfl_idx = (int32_t *)(ws + cv.idx);
     d.acc = (double2 *)(ws + cv.acc);
-    d.terms = 4 + 6 * mesh->b_num;
-    d.cfl_blocks = cfl_grid(mesh->s_num);
-    d.cfl_max = cfl_max_blocks(mesh->s_num);
+    d.terms = 4 + 6 * (plan ? plan->max_b : mesh->b_num);
+    d.cfl_blocks = plan ? plan->cfl_total : cfl_grid(mesh->s_num);
+    d.cfl_max = plan ? plan->cfl_total : cfl_max_blocks(mesh->s_num);
     d.cell_block = NHF_BLOCK;
     d.cells_per_thread = 1;
     d.depth_blocks = cell_grid(mesh->e_num, d.cell_block);
     d.acc_owners = acc_owner_count(mesh->e_num);
     d.theta_one = params->theta == 1.0f;
+    d.parts = PartDev{};
+    ctx->parts = 0;
+    ctx->host_ctl = nullptr;
+    if (plan) {
+        const int n = (int)plan->range.size();
+        d.parts.count = n;
+        d.parts.cell_chunk = (const int32_t *)(ws + cv.cell_chunk);
+        d.parts.side_chunk = (const int32_t *)(ws + cv.side_chunk);
+        d.parts.range = (const int4 *)(ws + cv.range);
+        d.parts.extra = d.parts.range + n;
+        d.parts.cfl_part = (const int32_t *)(ws + cv.cfl_part);
+        ctx->parts = n;
+        ctx->part_range = plan->range;
+        ctx->part_extra = plan->extra;
+        ctx->part_pixels = plan->pixels;
+    }
     ctx->stream = (hipStream_t)stream;
     ctx->graph_enabled = 0;
     ctx->timing = 0;
     for (int i = 0; i <= NHF_MAX_BATCH; ++i) ctx->exec[i] = nullptr;
-    hipError_t err = hipHostMalloc((void **)&ctx->host_trace, sizeof(uint32_t) * NHF_MAX_BATCH * NHF_TRACE_COLS,
-                                   hipHostMallocDefault);
-    if (err != hipSuccess) {
-        delete ctx;
-        return set_error("hipHostMalloc trace", err);
-    }
     ctx->host_acc = nullptr;
     ctx->probe_scratch = nullptr;
     ctx->pending_steps = 0;
+    hipError_t err = hipHostMalloc((void **)&ctx->host_trace, sizeof(uint32_t) * NHF_MAX_BATCH * NHF_TRACE_COLS,
+                                   hipHostMallocDefault);
+    if (err == hipSuccess && plan)
+        err = hipHostMalloc((void **)&ctx->host_ctl, sizeof(Ctl) * plan->range.size(), hipHostMallocDefault);
+    if (err != hipSuccess) {
+        if (ctx->host_trace) (void)hipHostFree(ctx->host_trace);
+        delete ctx;
+        return set_error("hipHostMalloc trace", err);
+    }
     err = hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming);
     if (err != hipSuccess) {
         (void)hipHostFree(ctx->host_trace);
+        if (ctx->host_ctl) (void)hipHostFree(ctx->host_ctl);
         delete ctx;
         return set_error("hipEventCreate", err);
     }
@@ -1233,6 +1434,11 @@ int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params
     memset(ctx->kernel_count, 0, sizeof ctx->kernel_count);
     *out = ctx;
     return 0;
+}
+
+int32_t nhf_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, void *workspace,
+                   size_t workspace_bytes, void *stream) {
+    return create_ctx(out, mesh, params, nullptr, workspace, workspace_bytes, stream);
 }
 
 int32_t nhf_destroy(nhf_ctx *ctx) {
@@ -1243,34 +1449,76 @@ int32_t nhf_destroy(nhf_ctx *ctx) {
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->done);
     if (ctx->host_trace) (void)hipHostFree(ctx->host_trace);
+    if (ctx->host_ctl) (void)hipHostFree(ctx->host_ctl);
     if (ctx->host_acc) (void)hipHostFree(ctx->host_acc);
     delete ctx;
     return 0;
 }
 
-int32_t nhf_reset(nhf_ctx *ctx, uint64_t now_ticks) {
-    hipLaunchKernelGGL(k_reset, dim3(256), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, now_ticks);
+// Entry points of one mesh refuse a parts context and the nhu_* forms a single mesh.
+static int32_t single_only(const nhf_ctx *ctx, const char *name) {
+    if (!ctx) return set_error_text("null context");
+    if (ctx->parts) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "%s: a parts context takes the nhu_* form", name);
+        return -2;
+    }
+    return 0;
+}
+
+static int32_t part_only(const nhf_ctx *ctx, int part, const char *name) {
+    if (!ctx) return set_error_text("null context");
+    if (!ctx->parts || part < 0 || part >= ctx->parts) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "%s: not a parts context or part out of range", name);
+        return -2;
+    }
+    return 0;
+}
+
+static int32_t reset_part(nhf_ctx *ctx, int part, uint64_t now_ticks) {
+    hipLaunchKernelGGL(k_reset, dim3(256), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, part, now_ticks);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+static int32_t forcing_part(nhf_ctx *ctx, int part, const float *coefficients, const float *rain_rates,
+                            int32_t raining) {
+    const Dev &d = ctx->d;
+    int b0 = 0, nb = d.m.b_num, p0 = 0, np = d.m.n_pixels;
+    if (ctx->parts) {
+        b0 = ctx->part_extra[part].x;
+        nb = ctx->part_extra[part].y;
+        p0 = ctx->part_pixels[part].x;
+        np = ctx->part_pixels[part].y;
+    }
+    if (nb > 0 && coefficients)
+        HIP_TRY(hipMemcpyAsync(d.coefficients + 2 * b0, coefficients, sizeof(float) * 2 * nb, hipMemcpyHostToDevice,
+                               ctx->stream));
+    if (rain_rates && np > 0)
+        HIP_TRY(hipMemcpyAsync(d.rain_rates + p0, rain_rates, sizeof(float) * np, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_set_raining, dim3(1), dim3(64), 0, ctx->stream, ctx->d, part, (int)raining);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int32_t begin_part(nhf_ctx *ctx, int part, uint64_t target_ticks, uint64_t origin_ticks) {
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, part, target_ticks, origin_ticks);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhf_reset(nhf_ctx *ctx, uint64_t now_ticks) {
+    if (int32_t rc = single_only(ctx, "nhf_reset")) return rc;
+    return reset_part(ctx, 0, now_ticks);
 }
 
 int32_t nhf_set_forcing(nhf_ctx *ctx, const float *coefficients, const float *rain_rates, int32_t raining) {
-    const Dev &d = ctx->d;
-    if (d.m.b_num > 0 && coefficients)
-        HIP_TRY(hipMemcpyAsync(d.coefficients, coefficients, sizeof(float) * 2 * d.m.b_num,
-                               hipMemcpyHostToDevice, ctx->stream));
-    if (rain_rates && d.m.n_pixels > 0)
-        HIP_TRY(hipMemcpyAsync(d.rain_rates, rain_rates, sizeof(float) * d.m.n_pixels,
-                               hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_set_raining, dim3(1), dim3(64), 0, ctx->stream, ctx->d, (int)raining);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    if (int32_t rc = single_only(ctx, "nhf_set_forcing")) return rc;
+    return forcing_part(ctx, 0, coefficients, rain_rates, raining);
 }
 
 int32_t nhf_begin(nhf_ctx *ctx, uint64_t target_ticks, uint64_t origin_ticks) {
-    hipLaunchKernelGGL(k_begin, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, target_ticks, origin_ticks);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    if (int32_t rc = single_only(ctx, "nhf_begin")) return rc;
+    return begin_part(ctx, 0, target_ticks, origin_ticks);
 }
 
 static hipEvent_t timing_event(nhf_ctx *ctx, size_t i) {
@@ -1304,18 +1552,29 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
         }                                                                   \
     } while (0)
 #define LAUNCH(kind, ...) TIMED(kind, hipLaunchKernelGGL(__VA_ARGS__))
-    const int cells = d.m.e_num - 1, sides = d.m.s_num - 1;
+    const int sides = d.m.s_num - 1;
+    const bool parts = ctx->parts > 0;
+    const int groups = parts ? ctx->parts : 1;  // workgroups of the prepare and finish launches
     const bool fuse_grad = d.m.n_grad > 0 && !d.gates_on;
     const int grad_blocks = blocks_xcd(d.m.n_grad > 0 ? d.m.n_grad : 1);
-    LAUNCH(K_PREPARE, k_prepare, dim3(1), dim3(64), 0, s, d, 0);
+    LAUNCH(K_PREPARE, k_prepare, dim3(groups), dim3(64), 0, s, d, 0);
     for (int slot = 0; slot < steps; ++slot) {
         if (d.gates_on) LAUNCH(K_GATE, k_gate_move, dim3(1), dim3(1024), 0, s, d, 0, 0);
         // gradients: a launch of their own at the batch's first step (the host may have written
         // the state) and where gates move the bed at the step start; otherwise the previous
         // step's CFL launch computed them
-        if (d.m.n_grad > 0 && (slot == 0 || !fuse_grad))
-            LAUNCH(K_GRAD, k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
-        if (!d.theta_one) LAUNCH(K_FACE, k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
+        if (d.m.n_grad > 0 && (slot == 0 || !fuse_grad)) {
+            if (parts)
+                LAUNCH(K_GRAD, k_gradients<true>, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
+            else
+                LAUNCH(K_GRAD, k_gradients<false>, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d);
+        }
+        if (!d.theta_one) {
+            if (parts)
+                LAUNCH(K_FACE, k_face_avg<true>, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
+            else
+                LAUNCH(K_FACE, k_face_avg<false>, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d);
+        }
         if (sides > 0) TIMED(K_FLUX, launch_k_flux(d, sides, s));
         TIMED(K_DONOR, launch_k_donor(d, s));
         if (d.gates_on && d.g.n_chunks > 0)
@@ -1323,8 +1582,13 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
         TIMED(K_DEPTH, launch_k_depth(d, s));
         if (d.gates_on) LAUNCH(K_GATE, k_gate_move, dim3(1), dim3(1024), 0, s, d, 1, 0);
         const int gb = fuse_grad && slot + 1 < steps ? grad_blocks : 0;
-        LAUNCH(K_CFL, k_cfl, dim3(gb + d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, gb);
-        LAUNCH(K_FINISH, k_finish, dim3(1), dim3(1024), 0, s, d, 1, slot, steps);
+        if (parts) {
+            LAUNCH(K_CFL, k_cfl<true>, dim3(gb + d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, gb);
+            LAUNCH(K_FINISH, k_finish<true>, dim3(groups), dim3(1024), 0, s, d, 1, slot, steps);
+        } else {
+            LAUNCH(K_CFL, k_cfl<false>, dim3(gb + d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, gb);
+            LAUNCH(K_FINISH, k_finish<false>, dim3(1), dim3(1024), 0, s, d, 1, slot, steps);
+        }
     }
 #undef LAUNCH
 #undef TIMED
@@ -1332,7 +1596,7 @@ static int32_t launch_batch(nhf_ctx *ctx, int steps, bool timed) {
     return 0;
 }
 
-int32_t nhf_launch_batch(nhf_ctx *ctx, int32_t steps) {
+static int32_t launch_any(nhf_ctx *ctx, int32_t steps) {
     if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhf_launch_batch: steps out of range");
     if (ctx->pending_steps) return set_error_text("nhf_launch_batch: the previous batch was not waited for");
     hipStream_t s = ctx->stream;
@@ -1353,14 +1617,17 @@ int32_t nhf_launch_batch(nhf_ctx *ctx, int32_t steps) {
         int32_t rc = launch_batch(ctx, steps, ctx->timing != 0);
         if (rc) return rc;
     }
-    HIP_TRY(hipMemcpyAsync(ctx->host_trace, ctx->d.ctl->trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps,
-                           hipMemcpyDeviceToHost, s));
+    if (ctx->parts)  // every part's control block, trace rows included, in one copy
+        HIP_TRY(hipMemcpyAsync(ctx->host_ctl, ctx->d.ctl, sizeof(Ctl) * ctx->parts, hipMemcpyDeviceToHost, s));
+    else
+        HIP_TRY(hipMemcpyAsync(ctx->host_trace, ctx->d.ctl->trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps,
+                               hipMemcpyDeviceToHost, s));
     HIP_TRY(hipEventRecord(ctx->done, s));
     ctx->pending_steps = steps;
     return 0;
 }
 
-int32_t nhf_wait_batch(nhf_ctx *ctx, uint32_t *trace) {
+static int32_t wait_any(nhf_ctx *ctx, uint32_t *trace) {
     const int steps = ctx->pending_steps;
     if (!steps) return set_error_text("nhf_wait_batch: no batch in flight");
     ctx->pending_steps = 0;
@@ -1373,8 +1640,23 @@ int32_t nhf_wait_batch(nhf_ctx *ctx, uint32_t *trace) {
             ctx->kernel_count[ctx->event_kind[i]] += 1;
         }
     }
-    if (trace) memcpy(trace, ctx->host_trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps);
+    if (!trace) return 0;
+    const size_t words = sizeof(uint32_t) * NHF_TRACE_COLS * steps;
+    if (ctx->parts)
+        for (int p = 0; p < ctx->parts; ++p) memcpy(trace + (size_t)p * NHF_TRACE_COLS * steps, ctx->host_ctl[p].trace, words);
+    else
+        memcpy(trace, ctx->host_trace, words);
     return 0;
+}
+
+int32_t nhf_launch_batch(nhf_ctx *ctx, int32_t steps) {
+    if (int32_t rc = single_only(ctx, "nhf_launch_batch")) return rc;
+    return launch_any(ctx, steps);
+}
+
+int32_t nhf_wait_batch(nhf_ctx *ctx, uint32_t *trace) {
+    if (int32_t rc = single_only(ctx, "nhf_wait_batch")) return rc;
+    return wait_any(ctx, trace);
 }
 
 int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
@@ -1384,6 +1666,7 @@ int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
 }
 
 int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
+    if (int32_t rc = single_only(ctx, "nhf_time_kernels")) return rc;
     if (repeats < 1) return set_error_text("nhf_time_kernels: repeats must be positive");
     if (ctx->pending_steps) return set_error_text("nhf_time_kernels: a batch is in flight");
     const Dev &d = ctx->d;
@@ -1400,13 +1683,13 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
         HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < repeats; ++r) {
             switch (k) {
-            case K_GRAD: hipLaunchKernelGGL(k_gradients, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d); break;
-            case K_FACE: hipLaunchKernelGGL(k_face_avg, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_GRAD: hipLaunchKernelGGL(k_gradients<false>, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d); break;
+            case K_FACE: hipLaunchKernelGGL(k_face_avg<false>, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_FLUX: launch_k_flux(d, sides, s); break;
             case K_DONOR: launch_k_donor(d, s); break;
             case K_DEPTH: launch_k_depth(d, s); break;
-            case K_CFL: hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, 0); break;
-            case K_FINISH: hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, d, 0, 0, 0); break;
+            case K_CFL: hipLaunchKernelGGL(k_cfl<false>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, 0); break;
+            case K_FINISH: hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(1024), 0, s, d, 0, 0, 0); break;
             }
         }
         HIP_TRY(hipGetLastError());
@@ -1419,30 +1702,51 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
     return 0;
 }
 
-int32_t nhf_compute_cfl(nhf_ctx *ctx) {
+static int32_t compute_cfl_any(nhf_ctx *ctx) {
     const Dev &d = ctx->d;
-    hipLaunchKernelGGL(k_cfl, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, ctx->stream, d, 0, 0, 0);
+    if (ctx->parts) {
+        hipLaunchKernelGGL(k_cfl<true>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0);
+        hipLaunchKernelGGL(k_finish<true>, dim3(ctx->parts), dim3(1024), 0, ctx->stream, d, 0, 0, 0);
+    } else {
+        hipLaunchKernelGGL(k_cfl<false>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0);
+        hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(1024), 0, ctx->stream, d, 0, 0, 0);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int32_t nhf_flush_budget(nhf_ctx *ctx, double *out) {
+int32_t nhf_compute_cfl(nhf_ctx *ctx) {
+    if (int32_t rc = single_only(ctx, "nhf_compute_cfl")) return rc;
+    return compute_cfl_any(ctx);
+}
+
+// Budget of one part (the whole mesh for a single mesh): its accumulators summed in a fixed
+// order in double-double, then cleared.
+static int32_t flush_part(nhf_ctx *ctx, int part, double *out) {
     const Dev &d = ctx->d;
-    const int owners = d.acc_owners;
-    size_t n = (size_t)owners * d.terms;
-    if (!ctx->host_acc) HIP_TRY(hipHostMalloc((void **)&ctx->host_acc, sizeof(double2) * n, hipHostMallocDefault));
-    HIP_TRY(hipMemcpyAsync(ctx->host_acc, d.acc, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
+    int o0 = 0, o1 = d.acc_owners, terms = d.terms;
+    if (ctx->parts) {
+        const int4 r = ctx->part_range[part];
+        o0 = (r.x - 1) >> 6;
+        o1 = (r.x + padded(r.y - r.x) - 1) >> 6;
+        terms = 4 + 6 * ctx->part_extra[part].y;
+    }
+    const size_t n = (size_t)(o1 - o0) * d.terms;
+    if (!ctx->host_acc)
+        HIP_TRY(hipHostMalloc((void **)&ctx->host_acc, sizeof(double2) * (size_t)d.acc_owners * d.terms,
+                              hipHostMallocDefault));
+    HIP_TRY(hipMemcpyAsync(ctx->host_acc, d.acc + (size_t)o0 * d.terms, sizeof(double2) * n, hipMemcpyDeviceToHost,
+                           ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i < n; ++i)
         if (!isfinite(ctx->host_acc[i].x) || !isfinite(ctx->host_acc[i].y)) return 1;
-    for (int t = 0; t < d.terms; ++t) {
-        // two-sum accumulation of every block's (sum, compensation) pair
+    for (int t = 0; t < terms; ++t) {
+        // two-sum accumulation of every wavefront's (sum, compensation) pair
         double hi = 0.0, lo = 0.0;
-        for (int b = 0; b < owners; ++b) {
+        for (int b = 0; b < o1 - o0; ++b) {
             const double2 a = ctx->host_acc[(size_t)b * d.terms + t];
-            const double parts[2] = {a.x, a.y};
-            for (double x : parts) {
+            const double pair[2] = {a.x, a.y};
+            for (double x : pair) {
                 double s = hi + x;
                 double bp = s - hi;
                 lo += (hi - (s - bp)) + (x - bp);
@@ -1452,29 +1756,44 @@ int32_t nhf_flush_budget(nhf_ctx *ctx, double *out) {
         out[2 * t] = hi;
         out[2 * t + 1] = lo;
     }
-    hipLaunchKernelGGL(k_clear_acc, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d);
+    hipLaunchKernelGGL(k_clear_acc, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, part);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int32_t nhf_limit_capture(nhf_ctx *ctx, float *values, int32_t *ids, int32_t reset) {
-    Ctl *ctl = ctx->d.ctl;
+int32_t nhf_flush_budget(nhf_ctx *ctx, double *out) {
+    if (int32_t rc = single_only(ctx, "nhf_flush_budget")) return rc;
+    return flush_part(ctx, 0, out);
+}
+
+static int32_t limit_part(nhf_ctx *ctx, int part, float *values, int32_t *ids, int32_t reset) {
+    Ctl *ctl = ctx->d.ctl + part;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (values) HIP_TRY(hipMemcpy(values, ctl->limit_value, sizeof(float) * 5, hipMemcpyDeviceToHost));
     if (ids) HIP_TRY(hipMemcpy(ids, ctl->limit_ids, sizeof(int32_t) * 3, hipMemcpyDeviceToHost));
     if (reset) {
-        hipLaunchKernelGGL(k_reset_limit, dim3(1), dim3(64), 0, ctx->stream, ctx->d);
+        hipLaunchKernelGGL(k_reset_limit, dim3(1), dim3(64), 0, ctx->stream, ctx->d, part);
         HIP_TRY(hipGetLastError());
     }
     return 0;
 }
 
-int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side) {
-    Ctl *ctl = ctx->d.ctl;
+int32_t nhf_limit_capture(nhf_ctx *ctx, float *values, int32_t *ids, int32_t reset) {
+    if (int32_t rc = single_only(ctx, "nhf_limit_capture")) return rc;
+    return limit_part(ctx, 0, values, ids, reset);
+}
+
+static int32_t cfl_state_part(nhf_ctx *ctx, int part, float *ndt, int32_t *side) {
+    Ctl *ctl = ctx->d.ctl + part;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ndt) HIP_TRY(hipMemcpy(ndt, &ctl->ndt, sizeof(float), hipMemcpyDeviceToHost));
     if (side) HIP_TRY(hipMemcpy(side, &ctl->cfl_arg, sizeof(int32_t), hipMemcpyDeviceToHost));
     return 0;
+}
+
+int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side) {
+    if (int32_t rc = single_only(ctx, "nhf_cfl_state")) return rc;
+    return cfl_state_part(ctx, 0, ndt, side);
 }
 
 int32_t nhf_sync(nhf_ctx *ctx) {
@@ -1483,6 +1802,7 @@ int32_t nhf_sync(nhf_ctx *ctx) {
 }
 
 int32_t nhf_set_gates(nhf_ctx *ctx, const nhf_gates *gates) {
+    if (int32_t rc = single_only(ctx, "nhf_set_gates")) return rc;
     if (ctx->pending_steps) return set_error_text("nhf_set_gates: a batch is in flight");
     Dev &d = ctx->d;
     if (gates) {
@@ -1594,6 +1914,7 @@ int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable) {
 }
 
 int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block, int32_t cells_per_thread) {
+    if (ctx->parts) cfl_blocks = ctx->d.cfl_blocks;  // each part's CFL grid is fixed at creation
     if (cfl_blocks < 8 || cfl_blocks > ctx->d.cfl_max || (cfl_blocks & 7))
         return set_error_text("nhf_set_tuning: CFL grid out of range or not a multiple of 8");
     if (cell_block != 128 && cell_block != 256 && cell_block != 512)
@@ -1674,6 +1995,78 @@ int32_t nhf_clock_probe(const uint64_t *now, const uint64_t *target, const uint6
     }
     (void)hipFree(buf);
     return rc;
+}
+
+// --------------------------------------------------------------------------- parts (nhu_*)
+
+size_t nhu_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params, const nhf_part *parts, int32_t count) {
+    if (!mesh || !params || !parts) return 0;
+    PartPlan plan;
+    if (plan_parts(mesh, parts, count, plan)) return 0;
+    return carve(mesh, params, &plan).total;
+}
+
+int32_t nhu_create(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, const nhf_part *parts,
+                   int32_t count, void *workspace, size_t workspace_bytes, void *stream) {
+    if (!mesh || !parts) return set_error_text("nhu_create: null argument");
+    PartPlan plan;
+    if (const char *why = plan_parts(mesh, parts, count, plan)) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "nhu_create: %s", why);
+        return -2;
+    }
+    return create_ctx(out, mesh, params, &plan, workspace, workspace_bytes, stream);
+}
+
+int32_t nhu_reset(nhf_ctx *ctx, int32_t part, uint64_t now_ticks) {
+    if (int32_t rc = part_only(ctx, part, "nhu_reset")) return rc;
+    return reset_part(ctx, part, now_ticks);
+}
+
+int32_t nhu_set_forcing(nhf_ctx *ctx, int32_t part, const float *coefficients, const float *rain_rates,
+                        int32_t raining) {
+    if (int32_t rc = part_only(ctx, part, "nhu_set_forcing")) return rc;
+    return forcing_part(ctx, part, coefficients, rain_rates, raining);
+}
+
+int32_t nhu_begin(nhf_ctx *ctx, int32_t part, uint64_t target_ticks, uint64_t origin_ticks) {
+    if (int32_t rc = part_only(ctx, part, "nhu_begin")) return rc;
+    return begin_part(ctx, part, target_ticks, origin_ticks);
+}
+
+int32_t nhu_launch_batch(nhf_ctx *ctx, int32_t steps) {
+    if (int32_t rc = part_only(ctx, 0, "nhu_launch_batch")) return rc;
+    return launch_any(ctx, steps);
+}
+
+int32_t nhu_wait_batch(nhf_ctx *ctx, uint32_t *trace) {
+    if (int32_t rc = part_only(ctx, 0, "nhu_wait_batch")) return rc;
+    return wait_any(ctx, trace);
+}
+
+int32_t nhu_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
+    int32_t rc = nhu_launch_batch(ctx, steps);
+    if (rc) return rc;
+    return nhu_wait_batch(ctx, trace);
+}
+
+int32_t nhu_compute_cfl(nhf_ctx *ctx) {
+    if (int32_t rc = part_only(ctx, 0, "nhu_compute_cfl")) return rc;
+    return compute_cfl_any(ctx);
+}
+
+int32_t nhu_flush_budget(nhf_ctx *ctx, int32_t part, double *out) {
+    if (int32_t rc = part_only(ctx, part, "nhu_flush_budget")) return rc;
+    return flush_part(ctx, part, out);
+}
+
+int32_t nhu_limit_capture(nhf_ctx *ctx, int32_t part, float *values, int32_t *ids, int32_t reset) {
+    if (int32_t rc = part_only(ctx, part, "nhu_limit_capture")) return rc;
+    return limit_part(ctx, part, values, ids, reset);
+}
+
+int32_t nhu_cfl_state(nhf_ctx *ctx, int32_t part, float *ndt, int32_t *side) {
+    if (int32_t rc = part_only(ctx, part, "nhu_cfl_state")) return rc;
+    return cfl_state_part(ctx, part, ndt, side);
 }
 
 }  // extern "C"

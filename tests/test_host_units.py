@@ -409,7 +409,7 @@ def test_hip_library_loads_and_exports_every_declared_symbol():
     import re
     from pathlib import Path
     header = (Path(__file__).resolve().parents[1] / 'include' / 'nhflood.h').read_text()
-    declared = set(re.findall(r'\b(nh[fb]_[a-z_]+)\s*\(', header))
+    declared = set(re.findall(r'\b(nh[fbu]_[a-z_]+)\s*\(', header))
     lib = abi.load_hip_library()
     assert declared == set(abi.EXPORTED)
     for name in declared:

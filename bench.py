@@ -361,6 +361,101 @@ def batched_bench(args, device) -> dict:
     }
 
 
+def uneven_bench(args, device) -> dict:
+    """Uneven-size batch: --uneven-meshes meshes with cell counts log-uniform in
+    [--uneven-min-cells, --uneven-max-cells], 1-3 refinement levels (finest cells 5 m), seeded
+    fractal DEMs, 2-4 land classes with Horton infiltration, rain at its peak, a tide on half of
+    them and a 0.3 m pond on half. All advance together in one union (one launch sequence per step
+    for every mesh, each with its own dt; autotuned cell-pass shape and steps per graph, one
+    hipGraph per batch); the same steps are then timed one mesh at a time on single-mesh engines."""
+    import torch
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, make_params
+    from nh_flood_2d_amd.layout import initial_state
+    from nh_flood_2d_amd.union import UnionEngine
+    t0 = time.time()
+    items = synth.uneven_batch(args.seed + 21, args.uneven_meshes, args.uneven_min_cells, args.uneven_max_cells,
+                               workers=args.uneven_workers)
+    cells = np.array([sc.mesh.e_num - 1 for sc, _ in items], dtype=np.int64)
+    levels = [int(np.max((lay.arrays['cell_meta'][1:] >> 16) & 15)) + 1 for _, lay in items]
+    log(f'uneven workload: {len(items)} meshes, {int(cells.sum())} cells, from {int(cells.min())} to '
+        f'{int(cells.max())}, levels {levels} ({time.time() - t0:.1f} s)')
+    sc0 = items[0][0]
+    params = make_params(sc0.afa, sc0.sita, sc0.min_h, 64)
+
+    def state_of(sc, layout):
+        st = initial_state(sc.mesh, layout)
+        if sc.initial_depth is not None:
+            st['depth'] = sc.initial_depth.copy()
+        return st
+
+    t_peak = sc0.duration / 3
+    start = seconds_to_ticks(t_peak)
+    horizon = seconds_to_ticks(t_peak + 1e6)
+    union = UnionEngine([lay for _, lay in items], params, device=device)
+    views = []
+    for p, (sc, lay) in enumerate(items):
+        v = union.view(p)(lay, params, state_of(sc, lay))
+        _set_forcing(v, synth.scenario_forcing(sc), t_peak)
+        v.reset(start)
+        views.append(v)
+    union.compute_cfl()
+    union.set_graph(True)
+    utune, per_graph = None, 32
+    if args.tune:
+        from nh_flood_2d_amd.tune import tune_union
+        utune = tune_union(union, views, steps=32, rounds=1)
+        per_graph = utune['steps_per_graph']
+        log(f"uneven tuned: cell workgroup {utune['cell_block']} x {utune['cells_per_thread']}, "
+            f"{per_graph} steps per graph, {utune['us_per_step']:.0f} us/step")
+    now = [start] * len(views)    # forcing origins; the parts' clocks continue from the tuning steps
+    _run_protocol(views, union.run_batch, args.uneven_warmup, horizon, now, per_graph)
+    union.synchronize()
+    w0 = time.perf_counter()
+    _run_protocol(views, union.run_batch, args.uneven_steps, horizon, now, per_graph)
+    union.synchronize()
+    union_wall = time.perf_counter() - w0
+    updates = int(cells.sum()) * args.uneven_steps
+    union_value = updates / union_wall
+    union.close()
+    del union, views
+    torch.cuda.synchronize()
+    # the same steps one mesh at a time (tuned like the union's cell passes)
+    plain_wall = 0.0
+    for sc, lay in items:
+        eng = Engine(lay, params, state_of(sc, lay), device=device)
+        if utune:
+            eng.set_tuning(eng.tuning_range()['cfl_blocks'], utune['cell_block'], utune['cells_per_thread'])
+        _set_forcing(eng, synth.scenario_forcing(sc), t_peak)
+        eng.reset(start)
+        eng.compute_cfl()
+        eng.set_graph(True)
+        one = lambda n, eng=eng: eng.run_batch(n)[None]
+        mine = [start]
+        _run_protocol([eng], one, args.uneven_warmup, horizon, mine, per_graph)
+        eng.synchronize()
+        w0 = time.perf_counter()
+        _run_protocol([eng], one, args.uneven_steps, horizon, mine, per_graph)
+        eng.synchronize()
+        plain_wall += time.perf_counter() - w0
+        eng.close()
+        del eng
+    plain_value = updates / plain_wall
+    log(f'uneven: {len(items)} meshes x {args.uneven_steps} steps in {union_wall:.3f} s, '
+        f'{union_value / 1e9:.2f} G cell-updates/s; one mesh at a time {plain_wall:.3f} s, '
+        f'{plain_value / 1e9:.2f} G; ratio {union_value / plain_value:.2f}')
+    return {
+        'workload': f'{len(items)} meshes, {int(cells.sum())} cells in all ({int(cells.min())} to {int(cells.max())}, '
+                    f'log-uniform {args.uneven_min_cells}-{args.uneven_max_cells}), 1-3 levels (finest 5 m), '
+                    'fractal DEMs, 2-4 land classes with Horton, rain at peak, tide on half, pond on half, '
+                    'own dt per mesh',
+        'meshes': len(items), 'cells': int(cells.sum()), 'value': union_value, 'unit': 'cell-updates/s',
+        'plain_value': plain_value, 'uneven_vs_plain': union_value / plain_value, 'steps': args.uneven_steps,
+        'wall_s': union_wall, 'plain_wall_s': plain_wall, 'tuning': utune, 'batch_steps': per_graph,
+    }
+
+
 def served_bench(args) -> dict:
     """Serving load: Poisson arrivals (rate --serve-rate per second) of --serve-requests solves, each a
     seeded two-level mesh with log-uniform cell count in [--serve-min-cells, --serve-max-cells],
@@ -437,6 +532,12 @@ def main():
                     help='skip the autotuner (default launch grids, 32 steps per graph)')
     ap.add_argument('--batch-slots', type=int, default=64, help='scenarios of the same-mesh batch (0: skip)')
     ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
+    ap.add_argument('--uneven-meshes', type=int, default=32, help='meshes of the uneven-size batch (0: skip)')
+    ap.add_argument('--uneven-min-cells', type=int, default=50_000)
+    ap.add_argument('--uneven-max-cells', type=int, default=4_000_000)
+    ap.add_argument('--uneven-steps', type=int, default=256)
+    ap.add_argument('--uneven-warmup', type=int, default=64)
+    ap.add_argument('--uneven-workers', type=int, default=16, help='processes that build the uneven meshes')
     ap.add_argument('--serve-requests', type=int, default=12, help='solves of the serving load (0: skip)')
     ap.add_argument('--serve-rate', type=float, default=2.0)
     ap.add_argument('--serve-min-cells', type=int, default=100_000)
@@ -602,6 +703,8 @@ def main():
         result['copy_calibration_bytes'] = 2 * src.numel() * 4
     if args.batch_slots:
         result['batched'] = batched_bench(args, device)
+    if args.uneven_meshes:
+        result['uneven'] = uneven_bench(args, device)
     if args.serve_requests:
         result['served'] = served_bench(args)
     if not args.no_cpu:

@@ -492,6 +492,92 @@ def two_level_fractal(seed: int = 0, blocks: int = 790, block_size: float = 10.,
                     np.zeros(mesh.e_num, dtype=np.int32), boundaries, duration, 0.5, 1.0, 0.02)
 
 
+def uneven_scenario(seed: int, cells: float, levels: int, classes: int = 4, finest: float = 5.,
+                    duration: float = 7200.) -> Scenario:
+    """One mesh of an uneven-size batch: about `cells` cells over `levels` (1-3) refinement levels
+    whose finest cells are `finest` m (so meshes of different levels share one size base), the
+    lowest 25 % of a seeded diamond-square DEM (amplitude 20 m) refined one level and, with three
+    levels, its lowest 10 % two; `classes` (2-4) land classes of the Horton table (class 0
+    impervious); a triangular hyetograph with peak U[20, 150] mm/h; a tide boundary on the left
+    edge for half of the seeds; a dry start or a 0.3 m pond on the lowest quarter."""
+    if not 1 <= levels <= 3 or not 2 <= classes <= 4:
+        raise ValueError('1-3 levels and 2-4 land classes')
+    rng = np.random.default_rng(seed)
+    per_block = {1: 1.0, 2: 0.75 + 0.25 * 4, 3: 0.65 + 0.25 * 4 + 0.10 * 16}[levels]
+    blocks = max(8, int(round(np.sqrt(cells / per_block))))
+    block_size = finest * (1 << (levels - 1))
+    extent = blocks * block_size
+    power = int(np.ceil(np.log2(blocks))) + 1
+    fractal = diamond_square(min(power, 11), 0.6, rng)
+    bc = (np.arange(blocks) + 0.5) * block_size
+    BX, BY = np.meshgrid(bc, bc)
+    zb = sample_grid(fractal, extent, BX.ravel(), BY.ravel()).reshape(BX.shape)
+    lv = np.zeros_like(zb, dtype=np.int64)
+    if levels >= 2:
+        lv[zb <= np.quantile(zb, 0.25)] = 1
+    if levels == 3:
+        lv[zb <= np.quantile(zb, 0.10)] = 2
+    lv = balance_levels(lv)
+    class_field = diamond_square(min(power, 9), 0.7, rng)
+    cuts = np.linspace(0, 1, classes + 1)[1:-1]
+
+    def bed_fn(x, y):
+        return (20.0 * sample_grid(fractal, extent, x, y) + 0.0005 * x).astype(np.float32)
+
+    def land_fn(x, y):
+        v = sample_grid(class_field, extent, x, y)
+        return np.searchsorted(np.quantile(v, cuts), v).astype(np.uint8)
+
+    mesh = quadtree_mesh(lv, block_size, bed_fn, land_fn)
+    table = {0: dict(manning_n=0.015, infiltration='none'),
+             1: dict(manning_n=0.03, infiltration=dict(f0_mm_h=60, fc_mm_h=10, k_per_h=2)),
+             2: dict(manning_n=0.05, infiltration=dict(f0_mm_h=90, fc_mm_h=15, k_per_h=3)),
+             3: dict(manning_n=0.1, infiltration=dict(f0_mm_h=120, fc_mm_h=20, k_per_h=4))}
+    land = {k: table[k] for k in range(classes)}
+    starts, ends, mm = triangular_hyetograph(float(rng.uniform(20., 150.)), duration / 3, duration, 300.)
+    boundaries = []
+    if rng.uniform() < 0.5:
+        side_ids, _, faces, _, _ = mesh.exterior_edges()
+        knots = np.arange(0., duration + 300., 300.)
+        boundaries.append(BoundarySpec('tide', 'level', np.sort(side_ids[faces == 0]),
+                                       SeriesSpec(knots, 1.0 * np.sin(2 * np.pi * knots / 44640.))))
+    depth = None
+    if rng.uniform() < 0.5:
+        low = mesh.z <= np.quantile(mesh.z[1:], 0.25)
+        depth = np.where(low, np.float32(0.3), np.float32(0.)).astype(np.float32)
+        depth[0] = 0.
+    return Scenario(f'uneven_{levels}l_{mesh.e_num - 1}_cells', mesh, land, starts, ends, mm[:, None],
+                    np.zeros(mesh.e_num, dtype=np.int32), boundaries, duration, 0.5, 1.0, 0.02, depth)
+
+
+def _uneven_item(args):
+    seed, cells, levels, classes = args
+    from .forcing import cell_forcing_arrays
+    from .layout import build_layout
+    sc = uneven_scenario(seed, cells, levels, classes)
+    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, scenario_forcing(sc)), sc.sita)
+    return sc, layout
+
+
+def uneven_batch(seed: int, count: int = 32, min_cells: float = 50_000, max_cells: float = 4_000_000,
+                 workers: int = 16) -> list:
+    """(scenario, layout) of `count` meshes with cell counts log-uniform in [min_cells, max_cells],
+    1-3 levels and 2-4 land classes drawn per mesh, built in `workers` processes (spawned, so a
+    parent that already holds a GPU context is not forked)."""
+    rng = np.random.default_rng(seed)
+    jobs = [(seed * 1000 + i, float(np.exp(rng.uniform(np.log(min_cells), np.log(max_cells)))),
+             int(rng.integers(1, 4)), int(rng.integers(2, 5))) for i in range(count)]
+    order = sorted(range(count), key=lambda i: -jobs[i][1])     # largest first
+    if workers <= 1:
+        built = {i: _uneven_item(jobs[i]) for i in order}
+    else:
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context('spawn')) as pool:
+            built = dict(zip(order, pool.map(_uneven_item, [jobs[i] for i in order])))
+    return [built[i] for i in range(count)]
+
+
 def scenario_variants(base: Scenario, count: int, seed: int = 0, peak_range=(20., 150.),
                       manning_sigma: float = 0.2, pond: float = 0.3) -> list[Scenario]:
     """`count` scenarios on the mesh of `base`: rain peak uniform in `peak_range` (mm/h, same

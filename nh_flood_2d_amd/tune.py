@@ -6,7 +6,9 @@ initial state (the real solve's state, clock and budget are never touched):
 * single mesh: the workgroup size and cells per thread of the two cell passes, the CFL partial
   grid (workgroups of the CFL pass, hence sides per thread) and the steps per captured graph;
 * scenario batch: the grid of the grid-stride depth pass (cells x slots per thread), the CFL
-  grid and the steps per graph.
+  grid and the steps per graph;
+* meshes of different sizes in one union: the workgroup size and cells per thread of the cell
+  passes and the steps per graph (each part's CFL grid follows its side count).
 
 The choice changes launch shapes only: every candidate computes the same bits. Results are kept
 per size class (log2 of the cell count, slots, theta, 2:1 or aligned) in a :class:`TuningCache`,
@@ -177,3 +179,35 @@ def tuned_engine_parameters(layout, params, state, device, forcing=None, cache: 
     if cache is not None:
         cache.put(key, result)
     return result
+
+
+def tune_union(union, views, steps: int = 64, rounds: int = 2, cell_shapes=CELL_SHAPES,
+               steps_per_graph=STEPS_PER_GRAPH) -> dict:
+    """Time a UnionEngine whose parts are bound (views reset, forcing set): the cell-pass shape at
+    32 steps per graph, then the steps per graph. The union's own state advances while it is timed
+    (every candidate computes the same bits, so only the timing depends on the candidate)."""
+    horizon = 1 << 62
+
+    def begin(target, origin):
+        for v in views:
+            v.begin(target, origin)
+    union.set_graph(True)
+    times = {}
+
+    def measure(shape, pg):
+        union.set_tuning(shape[0], shape[1])
+        t = _time(union.synchronize, begin, union.run_batch, steps, pg, horizon)
+        times[(shape, pg)] = min(times.get((shape, pg), math.inf), t)
+
+    for _ in range(rounds):
+        for shape in cell_shapes:
+            measure(shape, 32)
+    best_shape = pick({k: v for k, v in times.items() if k[1] == 32})[0]
+    for _ in range(rounds):
+        for pg in steps_per_graph:
+            measure(best_shape, pg)
+    best = pick({k: v for k, v in times.items() if k[0] == best_shape})
+    union.set_tuning(best[0][0], best[0][1])
+    return dict(cell_block=best[0][0], cells_per_thread=best[0][1], steps_per_graph=best[1],
+                us_per_step=times[best] * 1e6,
+                table={f'{a[0]}x{a[1]}/{b}': v * 1e6 for (a, b), v in times.items()})

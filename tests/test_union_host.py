@@ -121,3 +121,19 @@ def test_concatenation_steps_like_each_part_alone(mixed):
         assert sc.initial_depth is None or np.any(mine != 0)   # the ponded parts move water
     for e in engines:
         e.close()
+
+
+def test_uneven_meshes_have_the_drawn_levels_and_classes():
+    """The uneven-size batch's meshes: 1-3 levels with 5 m finest cells (one size base, so a union
+    keeps the sizes-from-levels path), 2-4 land classes, and about the requested cell count."""
+    for levels in (1, 2, 3):
+        sc = synth.uneven_scenario(levels, 6000, levels, classes=levels + 1)
+        sizes = np.unique(sc.mesh.geometry()[1][1:])
+        np.testing.assert_allclose(sizes, 5.0 * 2.0 ** np.arange(len(sizes)))
+        assert len(sizes) == levels
+        assert len(np.unique(sc.mesh.land[1:])) == levels + 1
+        assert 0.6 * 6000 < sc.mesh.e_num - 1 < 1.6 * 6000
+    items = synth.uneven_batch(3, 4, 2000, 9000, workers=1)
+    plan = UnionPlan([lay for _, lay in items])
+    assert plan.size_base == 5.0
+    assert [lay.e_num for _, lay in items] == [sc.mesh.e_num for sc, _ in items]

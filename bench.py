@@ -14,6 +14,9 @@ Prints one JSON line (rank 0, N = 1 only):
   roofline     the dominant phase's algorithmic bytes / its mean duration (HIP events), with
                its measured HBM bytes (rocprofv3 counters) as `traffic`
   cpu_baseline the CPU oracle (C, OpenMP) on the host cores, on a bounded sample of steps
+  batched      64 scenarios on one shared 1 M-cell mesh (nhb_*), against one of them alone
+  uneven       32 meshes of 50 k-4 M cells in one union (nhu_*), against one mesh at a time
+  served       Poisson arrivals of 100 k-8 M-cell solves, p50/p99 latency, against one at a time
 """
 from __future__ import annotations
 

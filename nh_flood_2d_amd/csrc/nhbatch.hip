@@ -260,33 +260,29 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_face_avg(BDev d) {
 
 // --------------------------------------------------------------------------- (a) face flux
 
-template <bool W>
-__global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
-    const int sh = W ? 6 : d.shift;
-    size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
-    int si = entity_of<W>(t, sh) + 1;
-    int s = (int)(t & ((1u << sh) - 1));
-    if (si >= d.m.s_num) return;
-    const float dt = d.dt[s];
-    if (dt == 0.0f) return;
+// A side's loaded values for one slot.
+struct BFlux {
+    int2 lh;
+    uint32_t meta;
+    float q, zl, zh, dl, dh;
+};
+
+// The new discharge of side si for slot s (k_flux's arithmetic, solver_compact.py:439-486).
+static __device__ __forceinline__ float bflux_value(const BDev &d, int s, int sh, int si, float dt, const BFlux &f) {
     const nhf_mesh &m = d.m;
     const float g = d.p.g;
-    int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
-    uint32_t meta = m.side_meta[si];
-    float bf = (meta & NHF_SIDE_EXTERIOR) ? 0.0f : 1.0f;
-    float zl = m.bed[lh.x], zh = m.bed[lh.y];
-    float dl = d.depth[IX(lh.x)], dh = d.depth[IX(lh.y)];
-    float hl = zl + dl, hr = zh + dh;
-    float q = d.q[IX(si)];
-    float wh = fmx(hl, hr) - fmx(fmx(zl, zh), -999.0f);
+    const float bf = (f.meta & NHF_SIDE_EXTERIOR) ? 0.0f : 1.0f;
+    const float hl = f.zl + f.dl, hr = f.zh + f.dh;
+    const float q = f.q;
+    const float wh = fmx(hl, hr) - fmx(fmx(f.zl, f.zh), -999.0f);
     // a face below min_h carries nothing (solver_compact.py:462,484): its arithmetic is skipped
     float nq = 0.0f;
     if (!(wh < d.p.min_h)) {
         float difference = hr - hl;
         if (m.edge_index) {
             int idx = m.edge_index[si];
-            if (idx >= 0 && dl > d.p.min_h && dh > d.p.min_h) {
-                int axis = (meta & NHF_SIDE_ORIENT_MASK) == 2 ? 1 : 0;
+            if (idx >= 0 && f.dl > d.p.min_h && f.dh > d.p.min_h) {
+                int axis = (f.meta & NHF_SIDE_ORIENT_MASK) == 2 ? 1 : 0;
                 float o0 = m.hang_offset[2 * idx], o1 = m.hang_offset[2 * idx + 1];
                 if (o0 != 0.0f) difference -= d.grad[IX(2 * m.hang_slot[2 * idx] + axis)] * o0;
                 if (o1 != 0.0f) difference += d.grad[IX(2 * m.hang_slot[2 * idx + 1] + axis)] * o1;
@@ -294,64 +290,121 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
         }
         float dq = ((-g * difference) / m.sdc[si]) * fmx(wh, 0.0f) * dt;
         float n = d.manning ? d.manning[IX(si)] : m.manning[si];
-        float f = 1.0f;
-        if (q != 0.0f) f = 1.0f + g * dt * (n * n) * fabsf(q / det_powf(fmx(wh, 0.00001f), 7.0f / 3.0f));
+        float fr = 1.0f;
+        if (q != 0.0f) fr = 1.0f + g * dt * (n * n) * fabsf(q / det_powf(fmx(wh, 0.00001f), 7.0f / 3.0f));
         float neighbours = 0.0f;
         if (!d.theta_one) {
-            int vertical = (meta & NHF_SIDE_ORIENT_MASK) == 2;
+            int vertical = (f.meta & NHF_SIDE_ORIENT_MASK) == 2;
             int fa = vertical ? 0 : 2, fb = vertical ? 1 : 3;
-            neighbours = neighbour_sum(q, d.face_qn[IX(4 * lh.x + fa)], d.face_qn[IX(4 * lh.y + fb)],
-                                       m.open_face[4 * lh.x + fa], m.open_face[4 * lh.y + fb]);
+            neighbours = neighbour_sum(q, d.face_qn[IX(4 * f.lh.x + fa)], d.face_qn[IX(4 * f.lh.y + fb)],
+                                       m.open_face[4 * f.lh.x + fa], m.open_face[4 * f.lh.y + fb]);
         }
-        nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / f;
+        nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / fr;
     }
-    nq *= bf;
-    d.q[IX(si)] = nq;  // in place, as in k_flux; kb_donor scales it where its donor is limited
+    return nq * bf;
+}
+
+// K consecutive sides per thread (per wavefront with 64 slots): every side's loads are issued
+// before any arithmetic, since a lone side is one short chain of dependent loads.
+template <bool W, int K>
+__global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
+    const int sh = W ? 6 : d.shift;
+    size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
+    const int first = entity_of<W>(t, sh) * K + 1;
+    const int s = (int)(t & ((1u << sh) - 1));
+    if (first >= d.m.s_num) return;
+    const float dt = d.dt[s];
+    if (dt == 0.0f) return;
+    const nhf_mesh &m = d.m;
+    BFlux f[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int si = first + k;
+        if (si < m.s_num) {
+            f[k].lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
+            f[k].meta = m.side_meta[si];
+            f[k].q = d.q[IX(si)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (first + k < m.s_num) {
+            f[k].zl = m.bed[f[k].lh.x];
+            f[k].zh = m.bed[f[k].lh.y];
+            f[k].dl = d.depth[IX(f[k].lh.x)];
+            f[k].dh = d.depth[IX(f[k].lh.y)];
+        }
+    }
+    float nq[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (first + k < m.s_num) nq[k] = bflux_value(d, s, sh, first + k, dt, f[k]);
+    // in place, as in k_flux; kb_donor scales them where their donor is limited
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (first + k < m.s_num) d.q[IX(first + k)] = nq[k];
 }
 
 // --------------------------------------------------------------------------- donor scale
 
-template <bool W>
-__global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
-    const int sh = W ? 6 : d.shift;
-    size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
-    int cell = entity_of<W>(t, sh) + 1;
-    int s = (int)(t & ((1u << sh) - 1));
-    if (cell >= d.m.e_num) return;
-    const float dt = d.dt[s];
-    if (dt == 0.0f) return;
-    const nhf_mesh &m = d.m;
-    Ctl *c = &d.ctl[s];
-    const uint32_t meta = m.cell_meta[cell];
-    int bi = meta_boundary(meta);
-    float discharge = 0.0f;
-    bool level = false;
-    if (bi >= 0) {
-        int kind = m.kind[bi];
-        if (kind == 1) discharge = d.values[s * d.b_num + bi] * m.bweight[cell];
-        level = kind == 0;
-    }
-    float rainq = d.raining[s] ? d.rain_rates[(size_t)s * d.n_pixels + (d.n_pixels > 1 ? m.rain_index[cell] : 0)] : 0.0f;
+// One cell's values for kb_donor, gathered in two rounds before any arithmetic.
+struct BDonor {
+    uint32_t meta;
+    int4 rec;
+    float h;
+    int pix;
     CellList l;
-    cell_list(m, cell_record(m, cell), l);
-    const float size = l.regular ? cell_size(m, meta, cell) : 0.0f;
+    float qf[NHF_LIST_MAX];
+    float discharge, rainq;
+    int bi;
+    bool level;
+};
+
+// First round: what needs the cell index only.
+static __device__ __forceinline__ void bdonor_first(const BDev &d, int s, int sh, int cell, BDonor &v) {
+    const nhf_mesh &m = d.m;
+    v.meta = m.cell_meta[cell];
+    v.rec = cell_record(m, cell);
+    v.h = d.depth[IX(cell)];
+    v.pix = d.n_pixels > 1 ? m.rain_index[cell] : 0;
+}
+
+// Second round: the side discharges of a short list, the boundary values and the rain rate.
+static __device__ __forceinline__ void bdonor_second(const BDev &d, int s, int sh, int cell, BDonor &v) {
+    const nhf_mesh &m = d.m;
+    cell_list(m, v.rec, v.l);
+    if (v.l.short_list) gather_list(v.l, [&](int side) { return d.q[IX(side)]; }, v.qf);
+    v.bi = meta_boundary(v.meta);
+    v.discharge = 0.0f;
+    v.level = false;
+    if (v.bi >= 0) {
+        int kind = m.kind[v.bi];
+        if (kind == 1) v.discharge = d.values[s * d.b_num + v.bi] * m.bweight[cell];
+        v.level = kind == 0;
+    }
+    v.rainq = d.raining[s] ? d.rain_rates[(size_t)s * d.n_pixels + v.pix] : 0.0f;
+}
+
+// The donor scale of the cell (solver_compact.py:491-514) and the in-place scaling of the sides
+// it is the donor of (:516-524), only where the scale is not 1.
+static __device__ __forceinline__ void bdonor_finish(const BDev &d, int s, int sh, int cell, float dt, Ctl *c,
+                                                     BDonor &v) {
+    const nhf_mesh &m = d.m;
+    const CellList &l = v.l;
+    const float size = l.regular ? cell_size(m, v.meta, cell) : 0.0f;
     const float area = l.regular ? size * size : m.area[cell];
-    float positive = (fmx(0.0f, 0.0f) + fmx(discharge, 0.0f)) * dt;
-    float available = d.depth[IX(cell)] * area + positive + rainq * dt * area;
-    float outgoing = (fmx(-0.0f, 0.0f) + fmx(-discharge, 0.0f)) * dt;
+    float positive = (fmx(0.0f, 0.0f) + fmx(v.discharge, 0.0f)) * dt;
+    float available = v.h * area + positive + v.rainq * dt * area;
+    float outgoing = (fmx(-0.0f, 0.0f) + fmx(-v.discharge, 0.0f)) * dt;
     const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
     const uint32_t cnt = l.cnt;
-    const bool short_list = l.short_list;
-    const int *ids = l.ids;
-    float qf[NHF_LIST_MAX];
-    if (short_list) {
+    if (l.short_list) {
         float len[NHF_LIST_MAX];
-        gather_list(l, [&](int side) { return d.q[IX(side)]; }, qf);
         list_lengths(m, l, size, len);
 #pragma unroll
         for (int j = 0; j < NHF_LIST_MAX; ++j) {
             if (j < total) {
-                float sq = qf[j] * len[j];
+                float sq = v.qf[j] * len[j];
                 if (!(entry_face(j, n0, n01, n012) & 1)) sq = -sq;
                 outgoing += fmx(sq, 0.0f) * dt;
             }
@@ -370,7 +423,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     }
     float ds = 1.0f;
     bool limited = false;
-    if (outgoing > available && !level) {
+    if (outgoing > available && !v.level) {
         ds = available / outgoing;
         limited = true;
     }
@@ -378,12 +431,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     // (the argument of k_donor covers the neighbour that reads a side while it is scaled)
     uint32_t faces = 0;
     if (ds != 1.0f) {
-        if (short_list) {
+        if (l.short_list) {
 #pragma unroll
             for (int j = 0; j < NHF_LIST_MAX; ++j) {
-                if (j < total && (qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
-                    d.q[IX(ids[j])] = qf[j] * ds;
-                    if (qf[j] != 0.0f && ds < 1.0f) faces += 1;
+                if (j < total && (v.qf[j] >= 0.0f) == (bool)(entry_face(j, n0, n01, n012) & 1)) {
+                    d.q[IX(l.ids[j])] = v.qf[j] * ds;
+                    if (v.qf[j] != 0.0f && ds < 1.0f) faces += 1;
                 }
             }
         } else {
@@ -402,12 +455,38 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
             }
         }
     }
-    if (bi >= 0) d.bdonor[IX(d.bslot[cell])] = ds;
+    if (v.bi >= 0) d.bdonor[IX(d.bslot[cell])] = ds;
     if (limited) {
         atomicAdd(&c->limited_cells, 1u);
         atomicMin(&c->min_scale_bits, __float_as_uint(ds));
     }
     if (faces) atomicAdd(&c->limited_faces, faces);
+}
+
+// K consecutive cells per thread (per wavefront with 64 slots): both load rounds of every cell
+// are issued before any arithmetic or store. A later cell may read a side before an earlier cell
+// of the same thread scales it, which is one of the orders k_donor's argument already covers.
+template <bool W, int K>
+__global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
+    const int sh = W ? 6 : d.shift;
+    size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
+    const int first = entity_of<W>(t, sh) * K + 1;
+    const int s = (int)(t & ((1u << sh) - 1));
+    if (first >= d.m.e_num) return;
+    const float dt = d.dt[s];
+    if (dt == 0.0f) return;
+    Ctl *c = &d.ctl[s];
+    const int e_num = d.m.e_num;
+    BDonor v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (first + k < e_num) bdonor_first(d, s, sh, first + k, v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (first + k < e_num) bdonor_second(d, s, sh, first + k, v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (first + k < e_num) bdonor_finish(d, s, sh, first + k, dt, c, v[k]);
 }
 
 // --------------------------------------------------------------------------- (b) depth update
@@ -416,7 +495,174 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
 // keeps one slot and sums its cells' infiltration and floor terms in FP64 registers; the block
 // then adds them to its own compensated accumulators. Boundary cells leave their terms in bvals
 // for kb_finish.
-template <bool W>
+// One (cell, slot) of kb_depth: its values gathered in two rounds before any arithmetic.
+struct BDepth {
+    uint32_t meta;
+    int4 rec;
+    int pix, j;
+    float h, cap, rain_time, max_s, max_d;
+    CellList l;
+    float qv[NHF_LIST_MAX];
+    int kind, bi;
+    float ds, bvalue, bweight, level_value, bed, rainq, f0, fc, hk, area_stored;
+};
+
+// First round: what needs the cell index only.
+static __device__ __forceinline__ void bdepth_first(const BDev &d, int s, int sh, int cell, bool raining,
+                                                    BDepth &v) {
+    const nhf_mesh &m = d.m;
+    v.meta = m.cell_meta[cell];
+    v.rec = cell_record(m, cell);
+    v.pix = d.n_pixels > 1 ? m.rain_index[cell] : 0;
+    v.j = d.bslot[cell];
+    v.h = d.depth[IX(cell)];
+    v.cap = d.capacity[IX(cell)];
+    v.rain_time = raining ? d.rain_time[IX(cell)] : 0.0f;
+    v.max_s = d.max_speed[IX(cell)];
+    v.max_d = d.max_depth[IX(cell)];
+}
+
+// Second round: side discharges of a short list, boundary values, rain rate, Horton parameters.
+static __device__ __forceinline__ void bdepth_second(const BDev &d, int s, int sh, int cell, bool raining,
+                                                     BDepth &v) {
+    const nhf_mesh &m = d.m;
+    cell_list(m, v.rec, v.l);
+    if (v.l.short_list) gather_list(v.l, [&](int side) { return d.q[IX(side)]; }, v.qv);
+    v.bi = meta_boundary(v.meta);
+    v.kind = -1;
+    v.ds = 1.0f;
+    if (v.bi >= 0) {
+        v.kind = m.kind[v.bi];
+        v.ds = d.bdonor[IX(v.j)];
+        v.bvalue = d.values[s * d.b_num + v.bi];
+        v.bweight = m.bweight[cell];
+        v.level_value = d.level_values[s * d.b_num + v.bi];
+        v.bed = m.bed[cell];
+    }
+    v.rainq = raining ? d.rain_rates[(size_t)s * d.n_pixels + v.pix] : 0.0f;
+    if (raining) {
+        const float *hp = m.class_horton + 3 * meta_class(v.meta);
+        v.f0 = hp[0];
+        v.fc = hp[1];
+        v.hk = hp[2];
+    }
+    v.area_stored = v.l.regular ? 0.0f : m.area[cell];
+}
+
+// The update of one (cell, slot): sources, Horton, face totals, continuity, infiltration,
+// floor, level cells and extrema (k_depth's arithmetic, solver_compact.py:525-613), stores last.
+static __device__ __forceinline__ void bdepth_finish(const BDev &d, int s, int sh, int cell, float dt,
+                                                     uint64_t step_end, BDepth &v, double &a_infil,
+                                                     double &a_floor) {
+    const nhf_mesh &m = d.m;
+    const int bi = v.bi, kind = v.kind, j = v.j;
+    const float ds = v.ds;
+    float pipe = fmx(0.0f, 0.0f) + fmn(0.0f, 0.0f) * ds;
+    float discharge = 0.0f;
+    if (kind == 1) {
+        float requested = v.bvalue * v.bweight;
+        discharge = fmx(requested, 0.0f) + fmn(requested, 0.0f) * ds;
+    }
+    float ssq = pipe + discharge;
+    const float rainq = v.rainq;
+    float cap = v.cap, rt = 0.0f;
+    if (rainq > 0.0f) {
+        rt = v.rain_time + dt;
+        cap = v.fc + (v.f0 - v.fc) * det_expf(-v.hk * rt / 3600.0f);
+    }
+    float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
+    const CellList &l = v.l;
+    const float reg_size = l.regular ? cell_size(m, v.meta, cell) : 0.0f;
+    const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
+    const uint32_t cnt = l.cnt;
+    if (l.short_list) {
+        float len[NHF_LIST_MAX];
+        list_lengths(m, l, reg_size, len);
+#pragma unroll
+        for (int jj = 0; jj < NHF_LIST_MAX; ++jj) {
+            if (jj < total) {
+                const float p = v.qv[jj] * len[jj];
+                const int dir = entry_face(jj, n0, n01, n012);
+                if (dir == 0)
+                    ql += p;
+                else if (dir == 1)
+                    qr += p;
+                else if (dir == 2)
+                    qb += p;
+                else
+                    qt += p;
+            }
+        }
+    } else {
+        float fsum[4];
+        int k = k0;
+        for (int dir = 0; dir < 4; ++dir) {
+            int n = face_n(cnt, dir);
+            float acc = 0.0f;
+            for (int jj = 0; jj < n; ++jj, ++k) {
+                int side = m.isl[k];
+                acc += d.q[IX(side)] * m.length[side];
+            }
+            fsum[dir] = acc;
+        }
+        ql = fsum[0];
+        qr = fsum[1];
+        qb = fsum[2];
+        qt = fsum[3];
+    }
+    float tq = ql - qr + qb - qt + ssq;
+    const float area = l.regular ? reg_size * reg_size : v.area_stored;
+    float rain_depth = rainq * dt;
+    float increment = tq * dt / area + rain_depth;
+    float previous = v.h;
+    float raw = previous + increment;
+    float removed = fmn(cap * dt, fmx(raw, 0.0f));
+    raw -= removed;
+    float floor_v = fmx(-raw, 0.0f);
+    a_infil += (double)(removed * area);
+    float h_new;
+    float hv_;
+    if (kind == 0) {
+        float target = fmx(v.level_value - v.bed, 0.0f);
+        float *bv = d.bvals;
+        bv[IX(6 * j + 0)] = floor_v * area;
+        bv[IX(6 * j + 1)] = (target - previous) * area;
+        bv[IX(6 * j + 2)] = -(ql - qr + qb - qt) * dt;
+        bv[IX(6 * j + 3)] = -pipe * dt;
+        bv[IX(6 * j + 4)] = -rain_depth * area;
+        bv[IX(6 * j + 5)] = removed * area;
+        h_new = target;
+    } else {
+        a_floor += (double)(floor_v * area);
+        if (kind == 1) d.bvals[IX(6 * j + 0)] = discharge * dt;
+        h_new = fmx(raw, 0.0f);
+    }
+    if (rainq > 0.0f) {
+        d.rain_time[IX(cell)] = rt;
+        d.capacity[IX(cell)] = cap;
+    }
+    d.depth[IX(cell)] = h_new;
+    hv_ = fmx(h_new, 0.01f);
+    float u = 0.0f, w = 0.0f;
+    if (h_new >= d.p.min_h) {
+        const float size = l.regular ? reg_size : cell_size(m, v.meta, cell);
+        u = (ql + qr) / size / hv_ / 2.0f;
+        w = (qb + qt) / size / hv_ / 2.0f;
+    }
+    float speed = sqrtf(u * u + w * w);
+    if (speed > v.max_s) d.max_speed[IX(cell)] = speed;
+    if (h_new > v.max_d) {
+        d.max_depth[IX(cell)] = h_new;
+        d.peak_tick[IX(cell)] = step_end;
+    }
+}
+
+// Grid-stride over (cell, slot): the stride is a multiple of the slot count, so each thread
+// keeps one slot and sums its cells' infiltration and floor terms in FP64 registers, in the
+// order of its cells; the block then adds them to its own compensated accumulators. U strides
+// at a time: both load rounds of all U cells come before their arithmetic and stores. Boundary
+// cells leave their terms in bvals for kb_finish.
+template <bool W, int U>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
     const int sh = W ? 6 : d.shift;
     const int S = 1 << sh;
@@ -428,117 +674,20 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
     if (dt != 0.0f) {
         const size_t total = (size_t)(m.e_num - 1) << sh;
         const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
-        const int raining = d.raining[s];
+        const bool raining = d.raining[s] != 0;
         const uint64_t step_end = c->step_end;
-        for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += stride) {
-            const int cell = entity_of<W>(t, sh) + 1;
-            const uint32_t meta = m.cell_meta[cell];
-            const int bi = meta_boundary(meta);
-            const int kind = bi >= 0 ? m.kind[bi] : -1;
-            const int j = bi >= 0 ? d.bslot[cell] : -1;
-            float ds = bi >= 0 ? d.bdonor[IX(j)] : 1.0f;
-            float pipe = fmx(0.0f, 0.0f) + fmn(0.0f, 0.0f) * ds;
-            float discharge = 0.0f;
-            if (kind == 1) {
-                float requested = d.values[s * d.b_num + bi] * m.bweight[cell];
-                discharge = fmx(requested, 0.0f) + fmn(requested, 0.0f) * ds;
-            }
-            float ssq = pipe + discharge;
-            float rainq = raining ? d.rain_rates[(size_t)s * d.n_pixels + (d.n_pixels > 1 ? m.rain_index[cell] : 0)] : 0.0f;
-            float cap;
-            if (rainq > 0.0f) {
-                float rt = d.rain_time[IX(cell)] + dt;
-                d.rain_time[IX(cell)] = rt;
-                const float *hp = m.class_horton + 3 * meta_class(meta);
-                float f0 = hp[0], fc = hp[1], hk = hp[2];
-                cap = fc + (f0 - fc) * det_expf(-hk * rt / 3600.0f);
-                d.capacity[IX(cell)] = cap;
-            } else {
-                cap = d.capacity[IX(cell)];
-            }
-            float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
-            CellList l;
-            cell_list(m, cell_record(m, cell), l);
-            const float reg_size = l.regular ? cell_size(m, meta, cell) : 0.0f;
-            const int n0 = l.n0, n01 = l.n01, n012 = l.n012, total = l.total, k0 = l.k0;
-            const uint32_t cnt = l.cnt;
-            if (l.short_list) {
-                const int *ids = l.ids;
-                float qv[NHF_LIST_MAX], len[NHF_LIST_MAX];
-                gather_list(l, [&](int side) { return d.q[IX(side)]; }, qv);
-                list_lengths(m, l, reg_size, len);
+        for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += U * stride) {
+            BDepth v[U];
 #pragma unroll
-                for (int jj = 0; jj < NHF_LIST_MAX; ++jj) {
-                    if (jj < total) {
-                        const float p = qv[jj] * len[jj];
-                        const int dir = entry_face(jj, n0, n01, n012);
-                        if (dir == 0)
-                            ql += p;
-                        else if (dir == 1)
-                            qr += p;
-                        else if (dir == 2)
-                            qb += p;
-                        else
-                            qt += p;
-                    }
-                }
-            } else {
-                float fsum[4];
-                int k = k0;
-                for (int dir = 0; dir < 4; ++dir) {
-                    int n = face_n(cnt, dir);
-                    float acc = 0.0f;
-                    for (int jj = 0; jj < n; ++jj, ++k) {
-                        int side = m.isl[k];
-                        acc += d.q[IX(side)] * m.length[side];
-                    }
-                    fsum[dir] = acc;
-                }
-                ql = fsum[0];
-                qr = fsum[1];
-                qb = fsum[2];
-                qt = fsum[3];
-            }
-            float tq = ql - qr + qb - qt + ssq;
-            const float area = l.regular ? reg_size * reg_size : m.area[cell];
-            float rain_depth = rainq * dt;
-            float increment = tq * dt / area + rain_depth;
-            float previous = d.depth[IX(cell)];
-            float raw = previous + increment;
-            float removed = fmn(cap * dt, fmx(raw, 0.0f));
-            raw -= removed;
-            float floor_v = fmx(-raw, 0.0f);
-            a_infil += (double)(removed * area);
-            float h_new;
-            if (kind == 0) {
-                float target = fmx(d.level_values[s * d.b_num + bi] - m.bed[cell], 0.0f);
-                float *bv = d.bvals;
-                bv[IX(6 * j + 0)] = floor_v * area;
-                bv[IX(6 * j + 1)] = (target - previous) * area;
-                bv[IX(6 * j + 2)] = -(ql - qr + qb - qt) * dt;
-                bv[IX(6 * j + 3)] = -pipe * dt;
-                bv[IX(6 * j + 4)] = -rain_depth * area;
-                bv[IX(6 * j + 5)] = removed * area;
-                h_new = target;
-            } else {
-                a_floor += (double)(floor_v * area);
-                if (kind == 1) d.bvals[IX(6 * j + 0)] = discharge * dt;
-                h_new = fmx(raw, 0.0f);
-            }
-            d.depth[IX(cell)] = h_new;
-            float hv = fmx(h_new, 0.01f);
-            float u = 0.0f, v = 0.0f;
-            if (h_new >= d.p.min_h) {
-                const float size = l.regular ? reg_size : cell_size(m, meta, cell);
-                u = (ql + qr) / size / hv / 2.0f;
-                v = (qb + qt) / size / hv / 2.0f;
-            }
-            float speed = sqrtf(u * u + v * v);
-            if (speed > d.max_speed[IX(cell)]) d.max_speed[IX(cell)] = speed;
-            if (h_new > d.max_depth[IX(cell)]) {
-                d.max_depth[IX(cell)] = h_new;
-                d.peak_tick[IX(cell)] = step_end;
-            }
+            for (int u = 0; u < U; ++u)
+                if (t + u * stride < total) bdepth_first(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, v[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (t + u * stride < total) bdepth_second(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, v[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (t + u * stride < total)
+                    bdepth_finish(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, dt, step_end, v[u], a_infil, a_floor);
         }
     }
     // per-slot block totals: lanes of one slot within a wavefront, then the wavefronts
@@ -570,7 +719,10 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
 
 // --------------------------------------------------------------------------- (c) CFL dt
 
-template <bool W>
+// U strides per round: the side records of all U sides first, then their bed, depth, length,
+// discharge and id loads, then the bounds (the minimum of the total order does not depend on the
+// order the sides are taken in).
+template <bool W, int U>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
     const int sh = W ? 6 : d.shift;
     const int S = 1 << sh;
@@ -583,18 +735,40 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
         const float afa = d.p.afa, g = d.p.g;
         const size_t total = (size_t)(m.s_num - 1) << sh;
         const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
-        for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += stride) {
-            const int si = entity_of<W>(t, sh) + 1;
-            int2 lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
-            if (lh.x != 0 && lh.y != 0) {
-                float zl = m.bed[lh.x], zh = m.bed[lh.y];
-                float head = fmx(zl + d.depth[IX(lh.x)], zh + d.depth[IX(lh.y)]);
-                float h = fmx(head - fmx(fmx(zl, zh), -999.0f), 0.01f);
-                float cfl = afa * m.length[si] / (sqrtf(g * h) + fabsf(d.q[IX(si)]) / h);
-                uint64_t k2 = ((uint64_t)__float_as_uint(cfl) << 32) | m.side_key[si];
-                if (k2 < key) {
-                    key = k2;
-                    idx = si;
+        for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += U * stride) {
+            int si[U];
+            int2 lh[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool live = t + u * stride < total;
+                si[u] = live ? entity_of<W>(t + u * stride, sh) + 1 : 0;
+                lh[u] = live ? reinterpret_cast<const int2 *>(m.side_cells)[si[u]] : make_int2(0, 0);
+            }
+            float zl[U], zh[U], dl[U], dh[U], len[U], qs[U];
+            uint32_t ref[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (lh[u].x != 0 && lh[u].y != 0) {
+                    zl[u] = m.bed[lh[u].x];
+                    zh[u] = m.bed[lh[u].y];
+                    dl[u] = d.depth[IX(lh[u].x)];
+                    dh[u] = d.depth[IX(lh[u].y)];
+                    len[u] = m.length[si[u]];
+                    qs[u] = d.q[IX(si[u])];
+                    ref[u] = m.side_key[si[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (lh[u].x != 0 && lh[u].y != 0) {
+                    float head = fmx(zl[u] + dl[u], zh[u] + dh[u]);
+                    float h = fmx(head - fmx(fmx(zl[u], zh[u]), -999.0f), 0.01f);
+                    float cfl = afa * len[u] / (sqrtf(g * h) + fabsf(qs[u]) / h);
+                    uint64_t k2 = ((uint64_t)__float_as_uint(cfl) << 32) | ref[u];
+                    if (k2 < key) {
+                        key = k2;
+                        idx = si[u];
+                    }
                 }
             }
         }
@@ -772,6 +946,24 @@ static size_t align_b(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static int b_blocks(size_t n) { return (int)((n + NHF_BLOCK - 1) / NHF_BLOCK); }
 
+// Sides per wavefront of the 64-slot flux pass and cells per wavefront of its donor pass
+// (measured on the 1 M-cell batch mesh: flux 821 / 609 / 533 / 579 us for 1 / 2 / 4 / 8 sides,
+// donor 445 / 333 / 284 us for 1 / 2 / 4 cells).
+#ifndef NHB_FLUX_SIDES
+#define NHB_FLUX_SIDES 4
+#endif
+#ifndef NHB_DONOR_CELLS
+#define NHB_DONOR_CELLS 4
+#endif
+// Grid strides per load round of the 64-slot depth pass (1 / 2 / 3: 559 / 745 / 966 us; two or
+// more lower the occupancy) and of its CFL pass.
+#ifndef NHB_DEPTH_ROUND
+#define NHB_DEPTH_ROUND 1
+#endif
+#ifndef NHB_CFL_ROUND
+#define NHB_CFL_ROUND 2
+#endif
+
 static int b_blocks_xcd(size_t n) { return (b_blocks(n) + 7) & ~7; }
 
 // Grids of the grid-stride kernels: by default about 2048 workgroups (8 per CU), fewer on small
@@ -879,12 +1071,14 @@ static int32_t b_launch_batch(nhb_ctx *ctx, int steps, bool timed) {
             BLAUNCH(KB_FACE, w ? kb_face_avg<true> : kb_face_avg<false>, dim3(b_blocks((size_t)d.m.e_num * S)),
                     dim3(NHF_BLOCK), 0, s, d);
         if (sides > 0)
-            BLAUNCH(KB_FLUX, w ? kb_flux<true> : kb_flux<false>, dim3(b_blocks_xcd(sides * S)), dim3(NHF_BLOCK), 0, s,
+            BLAUNCH(KB_FLUX, (w ? kb_flux<true, NHB_FLUX_SIDES> : kb_flux<false, 1>),
+                    dim3(b_blocks_xcd((w ? (sides + NHB_FLUX_SIDES - 1) / NHB_FLUX_SIDES : sides) * S)), dim3(NHF_BLOCK), 0, s,
                     d);
-        BLAUNCH(KB_DONOR, w ? kb_donor<true> : kb_donor<false>, dim3(b_blocks_xcd(cells * S)), dim3(NHF_BLOCK), 0, s,
+        BLAUNCH(KB_DONOR, (w ? kb_donor<true, NHB_DONOR_CELLS> : kb_donor<false, 1>),
+                dim3(b_blocks_xcd((w ? (cells + NHB_DONOR_CELLS - 1) / NHB_DONOR_CELLS : cells) * S)), dim3(NHF_BLOCK), 0, s,
                 d);
-        BLAUNCH(KB_DEPTH, w ? kb_depth<true> : kb_depth<false>, dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
-        BLAUNCH(KB_CFL, w ? kb_cfl<true> : kb_cfl<false>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
+        BLAUNCH(KB_DEPTH, (w ? kb_depth<true, NHB_DEPTH_ROUND> : kb_depth<false, 1>), dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
+        BLAUNCH(KB_CFL, (w ? kb_cfl<true, NHB_CFL_ROUND> : kb_cfl<false, 1>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
         BLAUNCH(KB_FINISH, kb_finish, dim3((unsigned)S), dim3(NHF_BLOCK), 0, s, d, 1, step, steps);
     }
 #undef BLAUNCH
@@ -1166,9 +1360,9 @@ int32_t nhb_compute_cfl(nhb_ctx *ctx) {
     int32_t rc = apply_stage(ctx);
     if (rc) return rc;
     if (ctx->slots == 64)
-        hipLaunchKernelGGL(kb_cfl<true>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+        hipLaunchKernelGGL((kb_cfl<true, NHB_CFL_ROUND>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
     else
-        hipLaunchKernelGGL(kb_cfl<false>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+        hipLaunchKernelGGL((kb_cfl<false, 1>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
     hipLaunchKernelGGL(kb_finish, dim3(ctx->slots), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0, 0);
     HIP_TRY(hipGetLastError());
     return 0;

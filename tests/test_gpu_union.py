@@ -23,7 +23,10 @@ def _say(text):
     print(text, flush=True)
 
 
-def _scenarios():
+def _scenarios(theta: float = 1.0):
+    if theta != 1.0:      # the face-average pass and open-boundary fractions
+        return [synth.small_multires(1, blocks=20, pond=0.3, duration=240., theta=theta),
+                synth.small_multires(2, blocks=12, pond=0.3, discharge=0.5, duration=300., theta=theta)]
     out = [synth.small_multires(1, blocks=20, pond=0.3, duration=300.),
            synth.scenario_variants(synth.two_level_fractal(41, blocks=16, duration=240.), 1, seed=5)[0],
            synth.small_multires(2, blocks=12, pond=0.3, discharge=0.5, duration=360.)]
@@ -33,10 +36,11 @@ def _scenarios():
     return out
 
 
-@pytest.mark.parametrize('cell_shape', [(256, 1), (128, 2)])
-def test_union_parts_match_each_mesh_alone(gpu_device, cell_shape):
-    scenarios = _scenarios()
-    _say(f'[union] {len(scenarios)} meshes, cells {[s.mesh.e_num - 1 for s in scenarios]}, shape {cell_shape}')
+@pytest.mark.parametrize('cell_shape,theta', [((256, 1), 1.0), ((128, 2), 1.0), ((256, 1), 0.7)])
+def test_union_parts_match_each_mesh_alone(gpu_device, cell_shape, theta):
+    scenarios = _scenarios(theta)
+    _say(f'[union] {len(scenarios)} meshes, cells {[s.mesh.e_num - 1 for s in scenarios]}, shape {cell_shape}, '
+         f'sita {theta}')
 
     def plain(layout, params, state, pipe_q=None):
         return Engine(layout, params, state, device=gpu_device, pipe_q=pipe_q)

@@ -137,3 +137,78 @@ def test_uneven_meshes_have_the_drawn_levels_and_classes():
     plan = UnionPlan([lay for _, lay in items])
     assert plan.size_base == 5.0
     assert [lay.e_num for _, lay in items] == [sc.mesh.e_num for sc, _ in items]
+
+
+def test_part_tables_are_checked_by_the_library():
+    """nhu_workspace_bytes plans without touching a GPU: it accepts parts that tile the mesh in
+    order from 1 + 1024 k and refuses gaps, overlaps, misaligned starts and wrong totals."""
+    import ctypes
+    lib = abi.load_hip_library()
+    items = parts(mixed=False)
+    plan = UnionPlan([it[2] for it in items])
+    m = abi.MeshStruct()
+    m.e_num, m.s_num, m.r_len = plan.e_num, plan.s_num, plan.r_len
+    m.b_num, m.n_pixels, m.n_grad, m.n_hang = plan.b_num, plan.n_pixels, plan.n_grad, plan.n_hang
+    params = make_params(0.5, 1.0, 0.02, 32)
+    good = plan.part_struct()
+    assert lib.nhu_workspace_bytes(ctypes.byref(m), ctypes.byref(params), good, len(good)) > 0
+
+    def refused(edit):
+        bad = plan.part_struct()
+        edit(bad)
+        return lib.nhu_workspace_bytes(ctypes.byref(m), ctypes.byref(params), bad, len(bad)) == 0
+    assert refused(lambda t: setattr(t[1], 'cell_begin', t[1].cell_begin + 1))       # misaligned
+    assert refused(lambda t: setattr(t[0], 'cell_end', t[0].cell_end + 2000))        # overlaps the next
+    assert refused(lambda t: setattr(t[1], 'side_begin', t[1].side_begin + 1024))    # gap in the sides
+    assert refused(lambda t: setattr(t[1], 'b_base', t[1].b_base + 1))              # boundaries not tiled
+    assert refused(lambda t: setattr(t[0], 'pix_count', 0))                          # no rain pixel
+    assert lib.nhu_workspace_bytes(ctypes.byref(m), ctypes.byref(params), good, 1) == 0   # totals short
+    assert lib.nhu_workspace_bytes(ctypes.byref(m), ctypes.byref(params), good, 0) == 0
+
+
+def test_concatenation_with_theta_below_one_steps_like_each_part_alone():
+    """sita = 0.7 adds the face-average pass and the open-boundary fractions (per cell, 4 each)."""
+    scs = [synth.small_multires(4, blocks=16, pond=0.3, theta=0.7),
+           synth.small_multires(5, blocks=10, pond=0.3, theta=0.7, discharge=0.4)]
+    items = []
+    for sc in scs:
+        forcing = synth.scenario_forcing(sc)
+        layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, forcing), sc.sita)
+        state = initial_state(sc.mesh, layout)
+        state['depth'] = sc.initial_depth.astype(np.float32).copy()
+        items.append((sc, forcing, layout, state))
+    plan = UnionPlan([it[2] for it in items])
+    assert 'open_face' in plan.arrays
+    params = make_params(0.5, 0.7, 0.02, 32)
+    alone = [Engine(layout, params, state, device=oracle.HostDevice('f32')) for _, _, layout, state in items]
+    whole = Engine(union_layout(plan), params, union_state(plan, items), device=oracle.HostDevice('f32'))
+    engines = alone + [whole]
+    for e in engines:
+        e.reset(0)
+        e.compute_cfl()
+    now = 0
+    for _ in range(6):
+        t = now / 2 ** 40
+        coeffs, rates = [], []
+        for (sc, forcing, layout, _), e in zip(items, alone):
+            c = coefficients_at(forcing, t)[:layout.b_num].ravel()
+            r = forcing.rain.rates(forcing.rain.index_at(t)).astype(np.float32)
+            e.set_forcing(c, r, True)
+            coeffs.append(c)
+            rates.append(r)
+        whole.set_forcing(np.concatenate(coeffs).astype(np.float32), np.concatenate(rates), True)
+        step = min(0.5, 0.5 * min(e.cfl_state()[0] for e in engines))
+        target = seconds_to_ticks(t + step)
+        for e in engines:
+            e.begin(target, now)
+            rows = e.run_batch(1)
+            assert rows[0, 3] == abi.COMPLETED
+        now = (int(rows[0, 1]) << 32) | int(rows[0, 2])
+    for p, ((_, _, layout, _), e) in enumerate(zip(items, alone)):
+        c0, s0 = int(plan.cell_begin[p]), int(plan.side_begin[p])
+        assert (layout.cells_to_device(e.read_state('depth'))[1:].tobytes()
+                == whole.read_state('depth')[c0:c0 + layout.e_num - 1].tobytes())
+        assert (layout.sides_to_device(e.read_state('q'))[1:].tobytes()
+                == whole.read_state('q')[s0:s0 + layout.s_num - 1].tobytes())
+    for e in engines:
+        e.close()

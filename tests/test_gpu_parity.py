@@ -93,13 +93,13 @@ def test_fixed_dt_steps_match_oracle(gpu_device, name):
     _say(f'[fixed-dt] {name}: max|dh|={worst_h:.3g} max|dq|={worst_q:.3g}')
 
 
-def _simulate(sc, factory, end=600.):
+def _simulate(sc, factory, end=600., yield_step=150.):
     forcing = synth.scenario_forcing(sc)
     from nh_flood_2d_amd.output import seed_checkpoint
     seed = None
     if sc.initial_depth is not None:
         seed = seed_checkpoint(sc.initial_depth, sc.mesh.z, np.zeros(sc.mesh.s_num, np.float32))
-    settings = RunSettings(afa=sc.afa, sita=sc.sita, min_h=sc.min_h, end_time=end, yield_step=150.)
+    settings = RunSettings(afa=sc.afa, sita=sc.sita, min_h=sc.min_h, end_time=end, yield_step=yield_step)
     frames = []
 
     class Grab:
@@ -141,6 +141,30 @@ def test_adaptive_run_frames_match_oracle(gpu_device, name):
         for key in ('infiltration_m3', 'bed_clamp_nonlevel_m3', 'residual_m3'):
             assert abs(rg[key] - rc[key]) <= 1e-9 * max(abs(rc['storage_m3']), 1.), (key, rg[key], rc[key])
     _say(f'[adaptive] {name}: steps gpu={sim_g.steps} oracle={sim_c.steps}')
+
+
+def test_plain_config_full_run_matches_oracle(gpu_device):
+    """The CPU-runnable configuration at full size and duration: 256 x 256 cells of 10 m, 50 mm/h
+    for 1800 s on a dry start, closed boundaries, 3600 s with a frame every 300 s. The adaptive-dt
+    bar is 1e-4 m and 0.1 % flooded area per frame; the HIP step gives the oracle's bits."""
+    from nh_flood_2d_amd.engine import Engine
+    sc = synth.plain_uniform(0, n=256, dx=10.)
+    _say('[adaptive] plain 256 x 256, 3600 s')
+
+    def gpu_factory(layout, params, state):
+        return Engine(layout, params, state, device=gpu_device)
+    sim_g, fg = _simulate(sc, gpu_factory, end=3600., yield_step=300.)
+    sim_c, fc = _simulate(sc, oracle.engine_factory('f32'), end=3600., yield_step=300.)
+    assert [f[0] for f in fg] == [f[0] for f in fc] and len(fg) == 13
+    assert sim_g.record['dt_sequence_sha256'] == sim_c.record['dt_sequence_sha256']
+    _, _, _, area = sc.mesh.geometry()
+    for (t, dg, rg), (_, dc, rc) in zip(fg, fc):
+        assert np.max(np.abs(dg - dc)) <= 1e-4, t
+        assert abs(area[dg > 0.02].sum() - area[dc > 0.02].sum()) <= 1e-3 * max(area[dc > 0.02].sum(), 1.), t
+        np.testing.assert_array_equal(dg.view(np.uint32), dc.view(np.uint32))
+        assert abs(rg['residual_m3'] - rc['residual_m3']) <= 1e-9 * max(abs(rc['storage_m3']), 1.), t
+    assert fg[-1][1].max() > 0.1      # the rain has ponded in the channel and behind the hills
+    _say(f'[adaptive] plain 256 x 256: {sim_g.steps} steps, wet area {area[fg[-1][1] > 0.02].sum():.0f} m2')
 
 
 def test_graph_replays_match_eager(gpu_device):

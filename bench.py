@@ -17,6 +17,7 @@ Prints one JSON line (rank 0, N = 1 only):
                one thread
   precision    max |dh|, max |dq| and relative volume / infiltration error against FP64
   batched      64 scenarios on one shared 1 M-cell mesh (nhb_*), against one of them alone
+  batched_small the same on the 256 x 256 grid of the CPU-runnable configuration
   uneven       32 meshes of 50 k-4 M cells in one union (nhu_*), against one mesh at a time
   served       Poisson arrivals of 100 k-8 M-cell solves, p50/p99 latency, against one at a time
 """
@@ -338,9 +339,10 @@ def _run_protocol(engines, run_batch, steps: int, horizon: int, now: list, batch
     return total
 
 
-def batched_bench(args, device) -> dict:
+def batched_bench(args, device, base=None, label: str = 'batch', t_start: float = 1800.) -> dict:
     """Same-mesh scenario batch: `--batch-slots` scenarios (rain peak, Manning factor and start
-    state drawn per scenario) on one shared two-level mesh, against one of them on the plain path."""
+    state drawn per scenario) on one shared mesh (by default the two-level 1 M-cell mesh), against
+    one of them on the plain path."""
     import torch
     from nh_flood_2d_amd import synth
     from nh_flood_2d_amd.batch import BatchEngine
@@ -349,15 +351,15 @@ def batched_bench(args, device) -> dict:
     from nh_flood_2d_amd.forcing import cell_forcing_arrays
     from nh_flood_2d_amd.layout import build_layout, initial_state, relayout
     t0 = time.time()
-    base = synth.two_level_fractal(args.seed, blocks=args.batch_blocks)
+    if base is None:
+        base = synth.two_level_fractal(args.seed, blocks=args.batch_blocks)
     variants = synth.scenario_variants(base, args.batch_slots, seed=args.seed + 1)
     mesh = base.mesh
     cells = mesh.e_num - 1
     cf = cell_forcing_arrays(mesh, synth.scenario_forcing(variants[0]))
     layout0 = build_layout(mesh, variants[0].land_parameters, cf, base.sita)
     params = make_params(base.afa, base.sita, base.min_h, 64)
-    log(f'batch workload: {len(variants)} scenarios x {cells} cells, {mesh.s_num - 1} sides ({time.time() - t0:.1f} s)')
-    t_start = 1800.
+    log(f'{label} workload: {len(variants)} scenarios x {cells} cells, {mesh.s_num - 1} sides ({time.time() - t0:.1f} s)')
     start = seconds_to_ticks(t_start)
     horizon = seconds_to_ticks(t_start + 1e5)
 
@@ -402,7 +404,7 @@ def batched_bench(args, device) -> dict:
         from nh_flood_2d_amd.tune import tune_batch
         btune = tune_batch(batch, views, steps=32, rounds=1)
         per_graph = btune['steps_per_graph']
-        log(f"batch tuned: depth grid {btune['depth_blocks']}, CFL grid {btune['cfl_blocks']}, "
+        log(f"{label} tuned: depth grid {btune['depth_blocks']}, CFL grid {btune['cfl_blocks']}, "
             f"{per_graph} steps per graph, {btune['us_per_step']:.0f} us/step")
     now = [start] * len(views)   # forcing origins; the slots' clocks continue from the tuning steps
     _run_protocol(views, batch.run_batch, args.warmup, horizon, now, per_graph)
@@ -422,12 +424,12 @@ def batched_bench(args, device) -> dict:
     kernels = {name: dict(avg_us=ms[i] / count[i] * 1e3, launches=int(count[i]))
                for i, name in enumerate(KERNELS) if count[i]}
     sim_seconds = [(n - start) / 2 ** 40 for n in now]
-    log(f'batched: {steps_batch} scenario-steps in {batch_wall:.3f} s, {batch_value / 1e9:.2f} G cell-updates/s; '
+    log(f'{label}ed: {steps_batch} scenario-steps in {batch_wall:.3f} s, {batch_value / 1e9:.2f} G cell-updates/s; '
         f'plain: {plain_value / 1e9:.2f} G; ratio {batch_value / plain_value:.2f}')
-    log('batch kernels: ' + ', '.join(f"{k} {v['avg_us']:.1f} us" for k, v in kernels.items()))
+    log(f'{label} kernels: ' + ', '.join(f"{k} {v['avg_us']:.1f} us" for k, v in kernels.items()))
     batch.close()
     return {
-        'workload': f'{len(variants)} scenarios on one shared 2-level (10/5 m) fractal-DEM mesh, {cells} cells, '
+        'workload': f'{len(variants)} scenarios on one shared mesh ({base.name}), {cells} cells, '
                     f'{mesh.s_num - 1} sides; rain peak U[20,150] mm/h, Manning x lognormal(0.2), dry or 0.3 m '
                     'pond start, own dt per scenario',
         'scenarios': len(variants), 'cells': cells, 'value': batch_value, 'unit': 'cell-updates/s',
@@ -609,6 +611,8 @@ def main():
                     help='skip the autotuner (default launch grids, 32 steps per graph)')
     ap.add_argument('--batch-slots', type=int, default=64, help='scenarios of the same-mesh batch (0: skip)')
     ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
+    ap.add_argument('--no-small-batch', dest='small_batch', action='store_false',
+                    help='skip the batch on the 256 x 256 grid of the CPU-runnable configuration')
     ap.add_argument('--uneven-meshes', type=int, default=32, help='meshes of the uneven-size batch (0: skip)')
     ap.add_argument('--uneven-min-cells', type=int, default=50_000)
     ap.add_argument('--uneven-max-cells', type=int, default=4_000_000)
@@ -780,6 +784,12 @@ def main():
         result['copy_calibration_bytes'] = 2 * src.numel() * 4
     if args.batch_slots:
         result['batched'] = batched_bench(args, device)
+        if args.small_batch:
+            # the same batch on the CPU-runnable configuration's 256 x 256 grid of 10 m cells, where
+            # the plain path is bound by launches rather than bytes
+            from nh_flood_2d_amd import synth
+            result['batched_small'] = batched_bench(args, device, base=synth.plain_uniform(args.seed, n=256, dx=10.),
+                                                    label='small batch', t_start=900.)
     if args.uneven_meshes:
         result['uneven'] = uneven_bench(args, device)
     if args.serve_requests:

@@ -15,7 +15,6 @@ Prints one JSON line (rank 0, N = 1 only):
                its measured HBM bytes (rocprofv3 counters) as `traffic`
   cpu_baseline the CPU oracle (C, OpenMP) on the host cores, on a bounded sample of steps, and on
                one thread
-  precision    max |dh|, max |dq| and relative volume / infiltration error against FP64
   batched      64 scenarios on one shared 1 M-cell mesh (nhb_*), against one of them alone
   batched_small the same on the 256 x 256 grid of the CPU-runnable configuration
   uneven       32 meshes of 50 k-4 M cells in one union (nhu_*), against one mesh at a time
@@ -259,53 +258,6 @@ def cpu_baseline(sc, forcing, depth, seconds_budget: float = 10.0):
                        f'{wall:.1f} s wall, C oracle with {threads} OpenMP threads',
                 single_thread_value=cells * steps1 / wall1,
                 single_thread_sample=f'{steps1} steps, {wall1:.1f} s wall, 1 thread')
-
-
-def precision_vs_fp64(sc, forcing, depth, device, steps: int = 8) -> dict:
-    """Error of the HIP F32 step against an FP64 run of the same kernels (the C oracle built with
-    double arithmetic) on the benchmark mesh: `steps` steps from the same wet state, each forced
-    to the same dt (half the smaller CFL bound), then max |dh|, max |dq|, and the relative
-    differences of the stored volume and of the flushed infiltration."""
-    import oracle
-    from nh_flood_2d_amd.clock import seconds_to_ticks
-    from nh_flood_2d_amd.engine import Engine, make_params
-    from nh_flood_2d_amd.forcing import cell_forcing_arrays
-    from nh_flood_2d_amd.layout import build_layout, initial_state
-    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, forcing), sc.sita)
-    state = initial_state(sc.mesh, layout)
-    state['depth'] = depth.copy()
-    params = make_params(sc.afa, sc.sita, sc.min_h, 32)
-    gpu = Engine(layout, params, state, device=device)
-    f64 = Engine(layout, params, state, device=oracle.HostDevice('f64'))
-    now = seconds_to_ticks(3600.)
-    for e in (gpu, f64):
-        _set_forcing(e, forcing, 3600.)
-        e.reset(now)
-        e.compute_cfl()
-    infil = np.zeros(2)
-    for _ in range(steps):
-        step = 0.5 * min(e.cfl_state()[0] for e in (gpu, f64))
-        target = now + seconds_to_ticks(step)
-        rows = []
-        for e in (gpu, f64):
-            e.begin(target, now)
-            rows.append(e.run_batch(1))
-        if not all(r[0, 3] == 1 for r in rows):
-            raise RuntimeError('precision run: a step did not complete')
-        now = (int(rows[0][0, 1]) << 32) | int(rows[0][0, 2])
-    for i, e in enumerate((gpu, f64)):
-        infil[i] = float(e.flush_budget()[0].sum())
-    _, _, _, area = sc.mesh.geometry()
-    hg, hc = gpu.read_state('depth').astype(np.float64), f64.read_state('depth').astype(np.float64)
-    qg, qc = gpu.read_state('q').astype(np.float64), f64.read_state('q').astype(np.float64)
-    vol_g, vol_c = float(np.sum(hg * area)), float(np.sum(hc * area))
-    gpu.close()
-    f64.close()
-    return dict(reference='FP64 run of the same kernels (C oracle built with double)', steps=steps,
-                dt='forced to half the smaller CFL bound, the same in both runs',
-                max_abs_depth_m=float(np.max(np.abs(hg - hc))), max_abs_q_m2s=float(np.max(np.abs(qg - qc))),
-                relative_volume_error=abs(vol_g - vol_c) / vol_c,
-                relative_infiltration_error=float(abs(infil[0] - infil[1]) / max(abs(infil[1]), 1e-300)))
 
 
 def _set_forcing(eng, forcing, t: float):
@@ -797,9 +749,6 @@ def main():
     if not args.no_cpu:
         del eng
         torch.cuda.synchronize()
-        result['precision'] = precision_vs_fp64(sc, forcing, depth, device)
-        log('precision vs FP64: ' + ', '.join(f'{k} {v:.3g}' for k, v in result['precision'].items()
-                                              if isinstance(v, float)))
         result['cpu_baseline'] = cpu_baseline(sc, forcing, depth, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 

@@ -192,6 +192,19 @@ def trace_summary(trace_csv: str, bench_json: str) -> str:
     for name in batch:
         d = np.array(every[name]) / 1e3
         lines.append(f'| {name} | {len(d)} | {d.mean():.1f} | {d.min():.1f} | {d.max():.1f} |')
+    # the parts instantiations of the single-mesh kernels (meshes of different sizes in one union)
+    parts = defaultdict(list)
+    for r in rows:
+        full = r['Kernel_Name']
+        name = kernel_base_name(full)
+        if name.startswith('k_') and re.search(r'<[^>]*\btrue\b', full):
+            parts[name].append(int(r['End_Timestamp']) - int(r['Start_Timestamp']))
+    if parts:
+        lines += ['', '| Union kernel (parts) | Dispatches | Average (us) | Min (us) | Max (us) |',
+                  '| --- | ---: | ---: | ---: | ---: |']
+        for name in sorted(parts, key=lambda k: -sum(parts[k])):
+            d = np.array(parts[name]) / 1e3
+            lines.append(f'| {name} | {len(d)} | {d.mean():.1f} | {d.min():.1f} | {d.max():.1f} |')
     return '\n'.join(lines) + '\n'
 
 

@@ -739,14 +739,26 @@ def main():
                  'achieved_gbs': alg['step'] / (event_ms / steps * 1e-3) / 1e9,
                  'hbm_bytes': sum(traffic[k] for k in kernels if k in traffic) if traffic else None},
     }
+    # what a plain 256 MiB device copy reaches on this part (read + write bytes over its average
+    # duration by HIP events on the engine's stream), beside the roofline's published peak
+    with torch.cuda.stream(device.stream):
+        src = torch.ones(64 << 20, dtype=torch.float32, device=device.device)
+        dst = torch.empty_like(src)
+        for _ in range(3 if args.calibrate_copy else 2):
+            dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            dst.copy_(src)
+        e1.record()
+    device.stream.synchronize()
+    copy_gbs = 20 * 2 * src.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    result['roofline']['copy_gbs'] = copy_gbs
+    result['roofline']['frac_of_copy'] = dp['achieved_gbs'] / copy_gbs
+    log(f'device copy: {copy_gbs:.0f} GB/s; {dominant} at {dp["achieved_gbs"] / copy_gbs:.2f} of it')
     if args.calibrate_copy:
-        with torch.cuda.stream(device.stream):
-            src = torch.ones(64 << 20, dtype=torch.float32, device=device.device)
-            dst = torch.empty_like(src)
-            for _ in range(3):
-                dst.copy_(src)
-        device.stream.synchronize()
         result['copy_calibration_bytes'] = 2 * src.numel() * 4
+    del src, dst
     if args.batch_slots:
         result['batched'] = batched_bench(args, device)
         if args.small_batch:

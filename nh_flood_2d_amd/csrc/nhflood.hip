@@ -10,9 +10,12 @@
 //   k_depth       sources, Horton capacity, face totals, continuity, infiltration removal,
 //                 positivity floor, level cells, extrema and the per-wavefront FP64 water
 //                 budget                                               (solver_compact.py:525-613)
-//   k_cfl         CFL bound per interior side, wave-shuffle + LDS min/argmin, then the last
-//                 workgroup reduces the partials, commits the clock and prepares the next slot
-//                 (solver_compact.py:617-641, 364-391; core/clock.py:94-181)
+//   k_cfl         CFL bound per interior side, wave-shuffle + LDS min/argmin per workgroup
+//                                                                      (solver_compact.py:617-641)
+//   k_finish      one workgroup (per part) reduces the partials, commits the clock and prepares
+//                 the next slot                (solver_compact.py:364-391; core/clock.py:94-181)
+// k_finish stays a launch of its own: a last-workgroup-done reduction inside k_cfl needs an
+// agent-scope release per workgroup, an L2 write-back on each XCD, and made k_cfl 2.4x slower.
 // The clock, dt and every reduction stay on the device; the host reads one trace per batch.
 //
 // Layout: structure of arrays, cells and sides renumbered along a Morton curve by the host so a

@@ -1235,6 +1235,7 @@ int32_t nhb_destroy(nhb_ctx *ctx) {
 }
 
 static int32_t check_slot(const nhb_ctx *ctx, int32_t slot) {
+    if (!ctx) return set_error_text("nhb: null context");
     if (slot < 0 || slot >= ctx->slots) return set_error_text("nhb: slot out of range");
     return 0;
 }
@@ -1303,6 +1304,7 @@ static int32_t apply_stage(nhb_ctx *ctx) {
 }
 
 int32_t nhb_launch_batch(nhb_ctx *ctx, int32_t steps) {
+    if (!ctx) return set_error_text("nhb_launch_batch: null context");
     if (steps < 1 || steps > ctx->d.p.batch_capacity) return set_error_text("nhb_launch_batch: steps out of range");
     if (ctx->pending_steps) return set_error_text("nhb_launch_batch: the previous batch was not waited for");
     int32_t rc = apply_stage(ctx);
@@ -1332,6 +1334,7 @@ int32_t nhb_launch_batch(nhb_ctx *ctx, int32_t steps) {
 }
 
 int32_t nhb_wait_batch(nhb_ctx *ctx, uint32_t *trace) {
+    if (!ctx) return set_error_text("nhb_wait_batch: null context");
     const int steps = ctx->pending_steps;
     if (!steps) return set_error_text("nhb_wait_batch: no batch in flight");
     ctx->pending_steps = 0;
@@ -1354,12 +1357,14 @@ int32_t nhb_wait_batch(nhb_ctx *ctx, uint32_t *trace) {
 }
 
 int32_t nhb_run_batch(nhb_ctx *ctx, int32_t steps, uint32_t *trace) {
+    if (!ctx) return set_error_text("nhb_run_batch: null context");
     int32_t rc = nhb_launch_batch(ctx, steps);
     if (rc) return rc;
     return nhb_wait_batch(ctx, trace);
 }
 
 int32_t nhb_compute_cfl(nhb_ctx *ctx) {
+    if (!ctx) return set_error_text("nhb_compute_cfl: null context");
     const BDev &d = ctx->d;
     int32_t rc = apply_stage(ctx);
     if (rc) return rc;
@@ -1412,16 +1417,19 @@ int32_t nhb_cfl_state(nhb_ctx *ctx, int32_t slot, float *ndt, int32_t *side) {
 }
 
 int32_t nhb_sync(nhb_ctx *ctx) {
+    if (!ctx) return set_error_text("nhb_sync: null context");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
 int32_t nhb_set_graph(nhb_ctx *ctx, int32_t enable) {
+    if (!ctx) return set_error_text("nhb_set_graph: null context");
     ctx->graph_enabled = enable != 0;
     return 0;
 }
 
 int32_t nhb_set_tuning(nhb_ctx *ctx, int32_t depth_blocks, int32_t cfl_blocks) {
+    if (!ctx) return set_error_text("nhb_set_tuning: null context");
     BDev &d = ctx->d;
     if (depth_blocks < 8 || depth_blocks > d.depth_max || (depth_blocks & 7) || cfl_blocks < 8 ||
         cfl_blocks > d.cfl_max || (cfl_blocks & 7))
@@ -1441,6 +1449,7 @@ int32_t nhb_set_tuning(nhb_ctx *ctx, int32_t depth_blocks, int32_t cfl_blocks) {
 }
 
 int32_t nhb_tuning_range(nhb_ctx *ctx, int32_t *out4) {
+    if (!ctx) return set_error_text("nhb_tuning_range: null context");
     out4[0] = ctx->d.depth_blocks;
     out4[1] = ctx->d.depth_max;
     out4[2] = ctx->d.cfl_blocks;
@@ -1449,11 +1458,13 @@ int32_t nhb_tuning_range(nhb_ctx *ctx, int32_t *out4) {
 }
 
 int32_t nhb_set_timing(nhb_ctx *ctx, int32_t enable) {
+    if (!ctx) return set_error_text("nhb_set_timing: null context");
     ctx->timing = enable != 0;
     return 0;
 }
 
 int32_t nhb_kernel_times(nhb_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset) {
+    if (!ctx) return set_error_text("nhb_kernel_times: null context");
     for (int i = 0; i < n && i < KB_COUNT; ++i) {
         ms[i] = ctx->kernel_ms[i];
         count[i] = ctx->kernel_count[i];

@@ -1663,6 +1663,7 @@ int32_t nhf_wait_batch(nhf_ctx *ctx, uint32_t *trace) {
 }
 
 int32_t nhf_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
+    if (!ctx) return set_error_text("nhf_run_batch: null context");
     int32_t rc = nhf_launch_batch(ctx, steps);
     if (rc) return rc;
     return nhf_wait_batch(ctx, trace);
@@ -1800,6 +1801,7 @@ int32_t nhf_cfl_state(nhf_ctx *ctx, float *ndt, int32_t *side) {
 }
 
 int32_t nhf_sync(nhf_ctx *ctx) {
+    if (!ctx) return set_error_text("nhf_sync: null context");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -1869,6 +1871,7 @@ int32_t nhf_gate_check(nhf_ctx *ctx) {
 }
 
 int32_t nhf_flush_gate_budget(nhf_ctx *ctx, double *out) {
+    if (!ctx) return set_error_text("nhf_flush_gate_budget: null context");
     const Dev &d = ctx->d;
     if (!d.gates_on) return set_error_text("nhf_flush_gate_budget: no gates bound");
     if (ctx->pending_steps) return set_error_text("nhf_flush_gate_budget: a batch is in flight");
@@ -1903,6 +1906,7 @@ int32_t nhf_flush_gate_budget(nhf_ctx *ctx, double *out) {
 }
 
 int32_t nhf_settle_extrema(nhf_ctx *ctx) {
+    if (!ctx) return set_error_text("nhf_settle_extrema: null context");
     if (ctx->pending_steps) return set_error_text("nhf_settle_extrema: a batch is in flight");
     const Dev &d = ctx->d;
     hipLaunchKernelGGL(k_settle_extrema, dim3(blocks_for(d.m.e_num) < 2048 ? blocks_for(d.m.e_num) : 2048),
@@ -1912,11 +1916,13 @@ int32_t nhf_settle_extrema(nhf_ctx *ctx) {
 }
 
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable) {
+    if (!ctx) return set_error_text("nhf_set_graph: null context");
     ctx->graph_enabled = enable != 0;
     return 0;
 }
 
 int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block, int32_t cells_per_thread) {
+    if (!ctx) return set_error_text("nhf_set_tuning: null context");
     if (ctx->parts) cfl_blocks = ctx->d.cfl_blocks;  // each part's CFL grid is fixed at creation
     if (cfl_blocks < 8 || cfl_blocks > ctx->d.cfl_max || (cfl_blocks & 7))
         return set_error_text("nhf_set_tuning: CFL grid out of range or not a multiple of 8");
@@ -1942,6 +1948,7 @@ int32_t nhf_set_tuning(nhf_ctx *ctx, int32_t cfl_blocks, int32_t cell_block, int
 }
 
 int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *out4) {
+    if (!ctx) return set_error_text("nhf_tuning_range: null context");
     out4[0] = ctx->d.cfl_blocks;
     out4[1] = ctx->d.cfl_max;
     out4[2] = ctx->d.cell_block;
@@ -1950,11 +1957,13 @@ int32_t nhf_tuning_range(nhf_ctx *ctx, int32_t *out4) {
 }
 
 int32_t nhf_set_timing(nhf_ctx *ctx, int32_t enable) {
+    if (!ctx) return set_error_text("nhf_set_timing: null context");
     ctx->timing = enable != 0;
     return 0;
 }
 
 int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset) {
+    if (!ctx) return set_error_text("nhf_kernel_times: null context");
     for (int i = 0; i < n && i < K_COUNT; ++i) {
         ms[i] = ctx->kernel_ms[i];
         count[i] = ctx->kernel_count[i];
@@ -2047,6 +2056,7 @@ int32_t nhu_wait_batch(nhf_ctx *ctx, uint32_t *trace) {
 }
 
 int32_t nhu_run_batch(nhf_ctx *ctx, int32_t steps, uint32_t *trace) {
+    if (!ctx) return set_error_text("nhu_run_batch: null context");
     int32_t rc = nhu_launch_batch(ctx, steps);
     if (rc) return rc;
     return nhu_wait_batch(ctx, trace);

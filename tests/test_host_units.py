@@ -421,6 +421,24 @@ def test_hip_library_loads_and_exports_every_declared_symbol():
     assert lib.nhf_workspace_bytes(ctypes.byref(m), ctypes.byref(p)) > 0
 
 
+def test_entry_points_refuse_a_null_context():
+    """Every context entry point returns an error code and a message for a NULL context before
+    touching the device (destroy accepts NULL, as free does)."""
+    lib = abi.load_hip_library()
+    trace = (ctypes.c_uint32 * 9)()
+    calls = {
+        'nhf_run_batch': (1, trace), 'nhf_sync': (), 'nhf_set_graph': (1,), 'nhf_set_timing': (1,),
+        'nhf_settle_extrema': (), 'nhf_compute_cfl': (), 'nhf_set_tuning': (2048, 256, 1),
+        'nhb_launch_batch': (1,), 'nhb_wait_batch': (trace,), 'nhb_compute_cfl': (), 'nhb_sync': (),
+        'nhb_set_graph': (1,), 'nhb_set_timing': (1,), 'nhb_set_tuning': (8, 8), 'nhb_reset': (0, 0),
+        'nhu_run_batch': (1, trace), 'nhu_compute_cfl': (),
+    }
+    for name, args in calls.items():
+        assert getattr(lib, name)(None, *args) < 0, name
+        assert b'null context' in lib.nhf_last_error(), name
+    assert lib.nhf_destroy(None) == 0 and lib.nhb_destroy(None) == 0
+
+
 def test_oracle_takes_host_writes_between_batches():
     """A host write between batches (an extrema reset) is what the next batch starts from."""
     import oracle

@@ -50,7 +50,11 @@ def _compare(variants, alone, results, sims):
         _close_rows(results[i], row)
 
 
-@pytest.mark.parametrize('case', ['two_level', 'multires_tide_river', 'multires_theta'])
+# 64 slots take the wide kernels (several sides or cells per wavefront, one wavefront holding
+# the 64 slots of each); fewer slots take the narrow ones. The wide cases bind more scenarios than
+# slots (every lane busy, slots reused) or few scenarios with boundaries, hanging edges and sita < 1.
+@pytest.mark.parametrize('case', ['two_level', 'multires_tide_river', 'multires_theta', 'two_level_wide',
+                                  'multires_tide_river_wide', 'multires_theta_wide'])
 def test_batch_matches_each_scenario_alone(gpu_device, case):
     if case == 'two_level':
         base = synth.two_level_fractal(5, blocks=48, duration=600.)
@@ -58,9 +62,18 @@ def test_batch_matches_each_scenario_alone(gpu_device, case):
     elif case == 'multires_tide_river':
         base = synth.small_multires(3, duration=600., discharge=1.5)
         variants, slots = synth.scenario_variants(base, 3, seed=8), 2
-    else:
+    elif case == 'multires_theta':
         base = synth.small_multires(4, duration=480., theta=0.7, pond=0.2)
         variants, slots = synth.scenario_variants(base, 3, seed=9), 4
+    elif case == 'two_level_wide':
+        base = synth.two_level_fractal(12, blocks=20, duration=300.)
+        variants, slots = synth.scenario_variants(base, 70, seed=13), 64
+    elif case == 'multires_tide_river_wide':
+        base = synth.small_multires(14, duration=480., discharge=1.5)
+        variants, slots = synth.scenario_variants(base, 5, seed=15), 64
+    else:
+        base = synth.small_multires(16, duration=420., theta=0.7, pond=0.2)
+        variants, slots = synth.scenario_variants(base, 5, seed=17), 64
     _say(f'[batch] {case}: {base.mesh.e_num - 1} cells, {len(variants)} scenarios, {slots} slots; alone runs')
     alone = [_run_alone(v, gpu_device, 120.) for v in variants]
     _say(f'[batch] {case}: batched run')

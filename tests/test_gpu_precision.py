@@ -1,5 +1,6 @@
-"""Error of the HIP F32 step against an FP64 run of the same kernels (the oracle built with double
-arithmetic), on the benchmark's 1.58 M-cell multi-resolution mesh.
+"""The benchmark's 1.58 M-cell multi-resolution mesh: the HIP step against the F32 oracle (bit for
+bit) and its error against an FP64 run of the same kernels (the oracle built with double
+arithmetic).
 
 Both start from the benchmark's wet state at the peak of the rain and take 8 steps, each forced to
 the same dt (half the smaller CFL bound). Bars: max |dh| 1e-4 m, the flushed infiltration budget
@@ -20,23 +21,49 @@ from parity import coefficients_at
 pytestmark = pytest.mark.gpu
 
 
-def test_benchmark_mesh_against_fp64(gpu_device):
+def _benchmark_engines(gpu_device, precision):
+    """The benchmark mesh's wet state at the rain peak (3600 s) on the HIP engine and on the
+    oracle with `precision` arithmetic, forced and with their first CFL bound computed."""
     sc = synth.multires_channel(0)
-    print(f'[precision] {sc.mesh.e_num - 1} cells against FP64', flush=True)
     forcing = synth.scenario_forcing(sc)
     layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, forcing), sc.sita)
     state = initial_state(sc.mesh, layout)
     state['depth'][1:] = np.maximum(np.float32(2.0) - sc.mesh.z[1:], np.float32(0.01))
     params = make_params(sc.afa, sc.sita, sc.min_h, 32)
     gpu = Engine(layout, params, state, device=gpu_device)
-    f64 = Engine(layout, params, state, device=oracle.HostDevice('f64'))
+    cpu = Engine(layout, params, state, device=oracle.HostDevice(precision))
     now = seconds_to_ticks(3600.)
     coeff = coefficients_at(forcing, 3600.)[:layout.b_num].ravel()
     rates = forcing.rain.rates(forcing.rain.index_at(3600.)).astype(np.float32)
-    for e in (gpu, f64):
+    for e in (gpu, cpu):
         e.set_forcing(coeff, rates, True)
         e.reset(now)
         e.compute_cfl()
+    return sc, gpu, cpu, now
+
+
+def test_benchmark_mesh_matches_the_f32_oracle(gpu_device):
+    """The benchmark's whole step at its full size, 32 adaptive steps in one batch: the same trace
+    (dt bits, clock words, limiter counts) and the same state bits as the F32 oracle."""
+    sc, gpu, f32, now = _benchmark_engines(gpu_device, 'f32')
+    print(f'[full size] {sc.mesh.e_num - 1} cells against the F32 oracle', flush=True)
+    rows = []
+    for e in (gpu, f32):
+        e.begin(now + seconds_to_ticks(600.), now)
+        rows.append(e.run_batch(32))
+    np.testing.assert_array_equal(rows[0], rows[1])
+    assert np.all(rows[0][:, 3] == 1)
+    for name in ('depth', 'q', 'max_depth', 'max_speed', 'capacity', 'rain_time'):
+        a, b = gpu.read_state(name), f32.read_state(name)
+        assert a.tobytes() == b.tobytes(), name
+    np.testing.assert_allclose(gpu.flush_budget().sum(axis=1), f32.flush_budget().sum(axis=1), rtol=1e-9, atol=1e-9)
+    gpu.close()
+    f32.close()
+
+
+def test_benchmark_mesh_against_fp64(gpu_device):
+    sc, gpu, f64, now = _benchmark_engines(gpu_device, 'f64')
+    print(f'[precision] {sc.mesh.e_num - 1} cells against FP64', flush=True)
     for _ in range(8):
         target = now + seconds_to_ticks(0.5 * min(e.cfl_state()[0] for e in (gpu, f64)))
         rows = []

@@ -304,6 +304,53 @@ def _run_protocol(engines, run_batch, steps: int, horizon: int, now: list, batch
     return total
 
 
+# The reference's only published step time: a 3,850-cell 4 m mesh, gates disabled, 0.212545 ms per
+# step of synchronized device-graph wall time excluding the first two batches, on an Apple M1 Max
+# with Taichi 1.7.4 Metal (docs/gate-validation/metal-performance-v2.json:92-95).
+REFERENCE_SMALL_MS = 0.212545
+
+
+def small_mesh_bench(args, device) -> dict:
+    """Time per step on a mesh of the size of the reference's published timing: 62 x 62 cells of
+    4 m (3,844 cells), seeded DEM with hills and a channel, rain, wet start, plain path with one
+    graph per 32-step batch; the first two batches are excluded as in the reference's definition."""
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state
+    sc = synth.plain_uniform(args.seed, n=62, dx=4.)
+    forcing = synth.scenario_forcing(sc)
+    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, forcing), sc.sita)
+    state = initial_state(sc.mesh, layout)
+    state['depth'][1:] = np.float32(0.05)
+    eng = Engine(layout, make_params(sc.afa, sc.sita, sc.min_h, 32), state, device=device)
+    t_start = 600.
+    _set_forcing(eng, forcing, t_start)
+    eng.reset(seconds_to_ticks(t_start))
+    eng.compute_cfl()
+    eng.set_graph(True)
+    now, horizon = [seconds_to_ticks(t_start)], seconds_to_ticks(t_start + 1e5)
+    one = lambda n: eng.run_batch(n)[None]
+    _run_protocol([eng], one, 64, horizon, now)          # the two excluded batches
+    eng.synchronize()
+    steps = 32 * 40
+    w0 = time.perf_counter()
+    done = _run_protocol([eng], one, steps, horizon, now)
+    eng.synchronize()
+    ms = (time.perf_counter() - w0) / done * 1e3
+    cells = sc.mesh.e_num - 1
+    log(f'small mesh: {cells} cells of 4 m, {ms * 1e3:.1f} us per step ({cells / ms / 1e6:.3f} G cell-updates/s); '
+        f'the reference on an M1 Max with Metal: {REFERENCE_SMALL_MS * 1e3:.1f} us')
+    eng.close()
+    return {'workload': f'{cells} cells, uniform 4 m grid (62 x 62), seeded hills and channel, rain, wet start, '
+                        'plain path, one graph per 32-step batch, first two batches excluded',
+            'cells': cells, 'steps': done, 'ms_per_step': ms, 'value': cells / (ms * 1e-3), 'unit': 'cell-updates/s',
+            'reference_ms_per_step': REFERENCE_SMALL_MS,
+            'reference': 'published: 3,850-cell 4 m mesh, gates disabled, Apple M1 Max, Taichi 1.7.4 Metal',
+            'ratio_to_reference': REFERENCE_SMALL_MS / ms}
+
+
 def batched_bench(args, device, base=None, label: str = 'batch', t_start: float = 1800.) -> dict:
     """Same-mesh scenario batch: `--batch-slots` scenarios (rain peak, Manning factor and start
     state drawn per scenario) on one shared mesh (by default the two-level 1 M-cell mesh), against
@@ -578,6 +625,8 @@ def main():
     ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
     ap.add_argument('--no-small-batch', dest='small_batch', action='store_false',
                     help='skip the batch on the 256 x 256 grid of the CPU-runnable configuration')
+    ap.add_argument('--no-small-mesh', dest='small_mesh', action='store_false',
+                    help='skip the time per step on a mesh of the size of the reference\'s published timing')
     ap.add_argument('--uneven-meshes', type=int, default=32, help='meshes of the uneven-size batch (0: skip)')
     ap.add_argument('--uneven-min-cells', type=int, default=50_000)
     ap.add_argument('--uneven-max-cells', type=int, default=4_000_000)
@@ -759,6 +808,8 @@ def main():
     if args.calibrate_copy:
         result['copy_calibration_bytes'] = 2 * src.numel() * 4
     del src, dst
+    if args.small_mesh:
+        result['small_mesh'] = small_mesh_bench(args, device)
     if args.batch_slots:
         result['batched'] = batched_bench(args, device)
         if args.small_batch:

@@ -121,8 +121,15 @@ def pmc_summary(fetch_csv: str, write_csv: str) -> str:
     copy_bytes = float(64 << 20) * 4
     for counter, path in (('FETCH_SIZE', fetch_csv), ('WRITE_SIZE', write_csv)):
         per_kernel, copies = defaultdict(list), []
-        for row in csv.DictReader(open(path)):
+        rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r['Dispatch_Id']))
+        # the single-mesh section runs before any batch kernel: later k_* dispatches belong to
+        # other sections (a plain engine beside a batch, the parts of a union)
+        first_batch = min((int(r['Dispatch_Id']) for r in rows if kernel_base_name(r['Kernel_Name']).startswith('kb_')),
+                          default=None)
+        for row in rows:
             name = kernel_base_name(row['Kernel_Name'])
+            if (name.startswith('k_') and first_batch is not None and int(row['Dispatch_Id']) > first_batch):
+                continue
             value = float(row['Counter_Value']) * 1024.0
             if 'copyBuffer' in name and int(row['Grid_Size']) >= 131072:
                 copies.append(value)

@@ -1,4 +1,4 @@
-"""File-based entry points: ``run_simulation``, ``solver``, ``warmup_solver``.
+"""File-based entry points: ``run_simulation``, ``solver``, ``warmup_solver``, ``solver_coupled``.
 
 Same signatures, return value (the final mass-balance row) and side effects as
 ``src/nh_flood_2d/core/solver_compact.py:152-176,1011-1016``: a manifest in ``domain_dir``, the
@@ -281,6 +281,15 @@ def warmup_solver(domain_cfg: DomainConfig, force_cfg: ForceConfig, pipe_cfg=Non
         record.data.update(run_kind='warmup', simulation_end=domain_cfg.start.isoformat())
         record.hash_inputs(domain_cfg, force_cfg)
         return _solve(domain_cfg, force_cfg, record, pipe_cfg, end_time=0.)
+
+
+def solver_coupled(domain_cfg: DomainConfig, force_cfg: ForceConfig, pipe_cfg=None):
+    """Run an already preprocessed domain with optional pipe coupling.
+
+    The reference keeps this name as an alias of ``solver`` (core/coupled/driver.py:12-20): the
+    pipe exchange is part of the one evolution loop, selected by ``pipe_cfg``.
+    """
+    return solver(domain_cfg, force_cfg, pipe_cfg)
 
 
 def _device_name(factory) -> str:

@@ -148,7 +148,7 @@ def run(path: Path, mesh: Mesh, *, duration, rain, boundaries=(), warmup=0, land
         sita=1., afa=.5, restart='', stations=None, warmup_only=False, preprocess=True, pipe=None):
     """Write the mesh and run the file-based solver; returns (row, manifest, domain config)."""
     from nh_flood_2d_amd.config import DomainConfig, ForceConfig
-    from nh_flood_2d_amd import solver
+    from nh_flood_2d_amd import solver_compact as solver
     write_mesh(path, mesh)
     land = land or {10: dict(manning_n=.03, infiltration='none')}
     dc = DomainConfig(ne=str(path / 'ne.txt'), ns=str(path / 'ns.txt'), epsg_code=EPSG,
@@ -170,7 +170,7 @@ def run(path: Path, mesh: Mesh, *, duration, rain, boundaries=(), warmup=0, land
 def seed(path: Path, mesh: Mesh, depth, q=None, bed=None, rain_time=None, infiltration=None):
     """A run_state=False checkpoint carrying an initial state for the mesh written at path."""
     from nh_flood_2d_amd.output import write_checkpoint
-    from nh_flood_2d_amd.solver import sha256
+    from nh_flood_2d_amd.solver_compact import sha256
     write_mesh(path, mesh)
     depth = np.asarray(depth, np.float32)
     zeros = np.zeros_like(depth)

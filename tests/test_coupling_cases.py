@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from nh_flood_2d_amd import solver
+from nh_flood_2d_amd import solver_compact as solver
 from nh_flood_2d_amd.coupling import (ORIFICE_COEFFICIENT, PipeConfig, PipeCoupling, PipeTopology, StorageActor,
                                       compute_exchange, map_nodes, parse_inp)
 import cases
@@ -137,7 +137,7 @@ def _coupled(tmp_path, depth0, q0=None, duration=10, interval=5, area=10.):
         def advance(self, boundaries, seconds):
             sent.append((boundaries, seconds))
             return super().advance(boundaries, seconds)
-    import nh_flood_2d_amd.solver as s
+    import nh_flood_2d_amd.solver_compact as s
     old = s.PIPE_ACTOR_FACTORY
     s.PIPE_ACTOR_FACTORY = lambda cfg, t, begin, end: Actor(t, area=area)
     try:

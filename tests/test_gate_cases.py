@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle
-from nh_flood_2d_amd import solver
+from nh_flood_2d_amd import solver_compact as solver
 from nh_flood_2d_amd.config import DomainConfig, ForceConfig, GateConfig
 from nh_flood_2d_amd.gate import load_gate_layout
 from nh_flood_2d_amd.output import read_checkpoint, write_checkpoint

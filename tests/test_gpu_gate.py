@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from nh_flood_2d_amd import solver
+from nh_flood_2d_amd import solver_compact as solver
 from nh_flood_2d_amd.engine import Engine
 import test_gate_cases as gc
 

@@ -83,8 +83,12 @@ def test_fixed_dt_steps_match_oracle(gpu_device, name):
             np.testing.assert_array_equal(pair.gpu.read_state('peak_tick'), pair.cpu.read_state('peak_tick'),
                                           err_msg=f'peak_tick step {step}')
     np.testing.assert_array_equal(pair.gpu.read_state('peak_tick'), pair.cpu.read_state('peak_tick'))
-    for state, tol in (('capacity', 1e-9), ('rain_time', 1e-6), ('max_depth', H_TOL), ('max_speed', 1e-3)):
+    for state, tol in (('capacity', 1e-9), ('rain_time', 1e-6), ('max_depth', H_TOL), ('max_speed', 1e-5)):
         assert pair.diff(state) <= tol, (state, pair.diff(state))
+        # the same operations in the same order: equal bits (max_speed's formula is pinned on
+        # the CPU against the host velocities and the uniform-flow speed)
+        np.testing.assert_array_equal(pair.gpu.read_state(state).view(np.uint32),
+                                      pair.cpu.read_state(state).view(np.uint32), err_msg=state)
     # the clock fields and the water budget
     bg, bc = pair.gpu.flush_budget(), pair.cpu.flush_budget()
     tg, tc = bg.sum(axis=1), bc.sum(axis=1)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from nh_flood_2d_amd import solver
+from nh_flood_2d_amd import solver_compact as solver
 from nh_flood_2d_amd.config import DomainConfig, ForceConfig
 from nh_flood_2d_amd.hydraulics import hanging_geometry
 from nh_flood_2d_amd.output import list_depth_frames, read_checkpoint, read_depth_frame

@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 import oracle
-from nh_flood_2d_amd import simulation, solver
+from nh_flood_2d_amd import simulation
+from nh_flood_2d_amd import solver_compact as solver
 from nh_flood_2d_amd.engine import Engine
 from nh_flood_2d_amd.output import read_checkpoint
 import cases

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from nh_flood_2d_amd import solver
+from nh_flood_2d_amd import solver_compact as solver
 from nh_flood_2d_amd.engine import Engine
 from nh_flood_2d_amd.output import list_depth_frames, read_checkpoint, read_depth_frame, read_extrema
 import cases

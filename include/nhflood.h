@@ -269,6 +269,34 @@ int32_t nhf_clock_probe(const uint64_t *now, const uint64_t *target, const uint6
                         uint64_t *out_ticks, float *out_times);
 
 /* ------------------------------------------------------------------------------------------
+ * One batch with no context and no allocation.
+ *
+ * No counterpart in the reference. It is for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. nhf_run_static(...) computes
+ * what this sequence on a fresh context computes, bit for bit:
+ *   nhf_reset(now_ticks); nhf_set_forcing(coefficients, rain_rates, raining); nhf_compute_cfl;
+ *   nhf_begin(target_ticks, origin_ticks); nhf_run_batch(steps, trace);
+ *   nhf_flush_budget(budget) when budget != NULL; nhf_settle_extrema.
+ * The mesh and its state arrays are the caller's device arrays, as for nhf_create, and are
+ * updated in place; coefficients, rain_rates, trace and budget are host arrays. The library's
+ * scratch (control block, CFL partials, donor scales, face averages, gradients, budget
+ * accumulators) is one file-scope device array of NHF_STATIC_SCRATCH_BYTES per GPU, held from the
+ * library's load until the process ends; a mesh whose scratch would not fit is refused with an
+ * error (about 10.5 B per cell at sita 1 with one boundary, 16 B more per cell at sita < 1).
+ * The call makes no hipMalloc, hipFree, hipHostMalloc or event, stream or graph creation; it
+ * launches on `stream` (NULL: the null stream) and returns after synchronising it.
+ * Every call on one GPU uses the same scratch and leaves its contents behind: each call clears
+ * what must start at zero, and calls that overlap on one GPU (several host threads, or streams
+ * running at once) race and give wrong results. Gate control and parts are not supported.
+ * Returns 1 (not an error code) when a budget total is non-finite, as nhf_flush_budget.
+ * ------------------------------------------------------------------------------------------ */
+#define NHF_STATIC_SCRATCH_BYTES (64u << 20)
+
+int32_t nhf_run_static(const nhf_mesh *mesh, const nhf_params *params, uint64_t now_ticks, uint64_t target_ticks,
+                       uint64_t origin_ticks, const float *coefficients, const float *rain_rates, int32_t raining,
+                       int32_t steps, uint32_t *trace, double *budget, void *stream);
+
+/* ------------------------------------------------------------------------------------------
  * Scenario batch: up to NHB_MAX_SLOTS scenarios ("slots") on one shared mesh, advanced by the
  * same launches, each with its own state, forcing, clock, dt, CFL, trace and budget.
  *

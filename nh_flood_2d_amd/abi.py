@@ -107,6 +107,9 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
         'nhf_clock_probe': (c_int32, [vp, vp, vp, vp, c_float, c_int32, vp, vp, vp]),
     }
+    if hasattr(lib, 'nhf_run_static'):
+        sig['nhf_run_static'] = (c_int32, [POINTER(MeshStruct), POINTER(Params), c_uint64, c_uint64, c_uint64,
+                                           vp, vp, c_int32, c_int32, vp, vp, vp])
     if hasattr(lib, 'nhb_create'):
         sig.update({
             'nhb_workspace_bytes': (c_size_t, [POINTER(MeshStruct), POINTER(BatchState), POINTER(Params), c_int32]),
@@ -161,7 +164,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
             'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe',
             'nhf_set_tuning', 'nhf_tuning_range', 'nhf_time_kernels', 'nhf_settle_extrema',
-            'nhf_set_gates', 'nhf_gate_check', 'nhf_flush_gate_budget',
+            'nhf_set_gates', 'nhf_gate_check', 'nhf_flush_gate_budget', 'nhf_run_static',
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
             'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
             'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times', 'nhb_set_tuning',

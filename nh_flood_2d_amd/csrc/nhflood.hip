@@ -24,6 +24,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "nhf_device.h"
@@ -166,6 +167,39 @@ __global__ void k_prepare(Dev d, int slot) {
     for (int i = threadIdx.x; i < NHF_MAX_BATCH * NHF_TRACE_COLS; i += blockDim.x) d.ctl[part].trace[i] = 0u;
     __syncthreads();
     if (threadIdx.x == 0) prepare_slot(d, part, slot);
+}
+
+// The interval budget of nhf_flush_budget reduced on the device (nhf_run_static, which has no
+// host buffer to take the accumulators to): workgroup t sums term t's per-wavefront (sum,
+// compensation) pairs in wavefront order with the host's two-sum, staged through LDS a workgroup
+// at a time, so the result has the host flush's bits. A non-finite entry leaves a non-finite
+// total (a non-finite partial sum stays non-finite under these additions).
+__global__ __launch_bounds__(NHF_BLOCK) void k_flush_terms(Dev d, double *out) {
+    const int t = blockIdx.x;
+    __shared__ double2 s_pair[NHF_BLOCK];
+    double hi = 0.0, lo = 0.0;
+    for (int base = 0; base < d.acc_owners; base += NHF_BLOCK) {
+        const int b = base + (int)threadIdx.x;
+        s_pair[threadIdx.x] = b < d.acc_owners ? d.acc[(size_t)b * d.terms + t] : make_double2(0.0, 0.0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int n = d.acc_owners - base < NHF_BLOCK ? d.acc_owners - base : NHF_BLOCK;
+            for (int j = 0; j < n; ++j) {
+                const double pair[2] = {s_pair[j].x, s_pair[j].y};
+                for (double x : pair) {
+                    const double s = hi + x;
+                    const double bp = s - hi;
+                    lo += (hi - (s - bp)) + (x - bp);
+                    hi = s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * t] = hi;
+        out[2 * t + 1] = lo;
+    }
 }
 
 __global__ void k_clear_acc(Dev d, int part) {
@@ -1337,41 +1371,39 @@ size_t nhf_workspace_bytes(const nhf_mesh *mesh, const nhf_params *params) {
     return carve(mesh, params).total;
 }
 
-static int32_t create_ctx(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, const PartPlan *plan,
-                          void *workspace, size_t workspace_bytes, void *stream) {
-    if (!out || !mesh || !params || !workspace) return set_error_text("nhf_create: null argument");
-    if (params->batch_capacity < 1 || params->batch_capacity > NHF_MAX_BATCH)
-        return set_error_text("nhf_create: batch_capacity out of range");
-    if (mesh->e_num < 2 || mesh->s_num < 1) return set_error_text("nhf_create: empty mesh");
-    Carve cv = carve(mesh, params, plan);
-    if (workspace_bytes < cv.total) return set_error_text("nhf_create: workspace too small");
-    if ((params->theta != 1.0f) && !mesh->open_face)
-        return set_error_text("nhf_create: open_face is required when sita != 1");
-    if (mesh->n_hang > 0 && (!mesh->edge_index || !mesh->x || !mesh->y || !mesh->grad_cell))
-        return set_error_text("nhf_create: hanging reconstruction arrays missing");
-    if ((mesh->pipe_volume != nullptr) != (mesh->pipe_correction != nullptr) || (mesh->pipe_volume && !mesh->pipe_q))
-        return set_error_text("nhf_create: pipe volume and correction go together, with pipe_q");
-    if (plan && mesh->pipe_q) return set_error_text("nhu_create: pipe exchange is not supported with parts");
-    if (const char *name = missing_mesh_array(mesh, true, true)) {
-        snprintf(nhf_error_text, sizeof nhf_error_text, "nhf_create: mesh array %s is NULL", name);
+// Argument checks shared by nhf_create, nhu_create and nhf_run_static.
+static int32_t check_mesh(const char *who, const nhf_mesh *mesh, const nhf_params *params, bool parts) {
+    if (params->batch_capacity < 1 || params->batch_capacity > NHF_MAX_BATCH) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "%s: batch_capacity out of range", who);
         return -2;
     }
-    char *ws = (char *)workspace;
-    if (plan) {
-        // the part tables, copied once into the workspace
-        HIP_TRY(hipMemcpy(ws + cv.cell_chunk, plan->cell_chunk.data(), sizeof(int32_t) * plan->cell_chunk.size(),
-                          hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(ws + cv.side_chunk, plan->side_chunk.data(), sizeof(int32_t) * plan->side_chunk.size(),
-                          hipMemcpyHostToDevice));
-        std::vector<int4> table(plan->range);
-        table.insert(table.end(), plan->extra.begin(), plan->extra.end());
-        HIP_TRY(hipMemcpy(ws + cv.range, table.data(), sizeof(int4) * table.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(ws + cv.cfl_part, plan->cfl_part.data(), sizeof(int32_t) * plan->cfl_part.size(),
-                          hipMemcpyHostToDevice));
-        HIP_TRY(hipMemset(ws + cv.ctl, 0, sizeof(Ctl) * plan->range.size()));
+    const char *why = nullptr;
+    if (mesh->e_num < 2 || mesh->s_num < 1)
+        why = "empty mesh";
+    else if ((params->theta != 1.0f) && !mesh->open_face)
+        why = "open_face is required when sita != 1";
+    else if (mesh->n_hang > 0 && (!mesh->edge_index || !mesh->x || !mesh->y || !mesh->grad_cell))
+        why = "hanging reconstruction arrays missing";
+    else if ((mesh->pipe_volume != nullptr) != (mesh->pipe_correction != nullptr) ||
+             (mesh->pipe_volume && !mesh->pipe_q))
+        why = "pipe volume and correction go together, with pipe_q";
+    else if (parts && mesh->pipe_q)
+        why = "pipe exchange is not supported with parts";
+    if (why) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "%s: %s", who, why);
+        return -2;
     }
-    nhf_ctx *ctx = new nhf_ctx();
-    Dev &d = ctx->d;
+    if (const char *name = missing_mesh_array(mesh, true, true)) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "%s: mesh array %s is NULL", who, name);
+        return -2;
+    }
+    return 0;
+}
+
+// The device view of one mesh whose scratch is carved from `ws` (default launch shapes).
+static void bind_dev(Dev &d, const nhf_mesh *mesh, const nhf_params *params, const PartPlan *plan, char *ws,
+                     const Carve &cv) {
+    d = Dev{};
     d.m = *mesh;
     if (mesh->n_hang == 0) d.m.edge_index = nullptr;
     d.p = *params;
@@ -1395,8 +1427,6 @@ static int32_t create_ctx(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params 
     d.acc_owners = acc_owner_count(mesh->e_num);
     d.theta_one = params->theta == 1.0f;
     d.parts = PartDev{};
-    ctx->parts = 0;
-    ctx->host_ctl = nullptr;
     if (plan) {
         const int n = (int)plan->range.size();
         d.parts.count = n;
@@ -1405,7 +1435,35 @@ static int32_t create_ctx(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params 
         d.parts.range = (const int4 *)(ws + cv.range);
         d.parts.extra = d.parts.range + n;
         d.parts.cfl_part = (const int32_t *)(ws + cv.cfl_part);
-        ctx->parts = n;
+    }
+}
+
+static int32_t create_ctx(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params *params, const PartPlan *plan,
+                          void *workspace, size_t workspace_bytes, void *stream) {
+    if (!out || !mesh || !params || !workspace) return set_error_text("nhf_create: null argument");
+    if (int32_t rc = check_mesh(plan ? "nhu_create" : "nhf_create", mesh, params, plan != nullptr)) return rc;
+    Carve cv = carve(mesh, params, plan);
+    if (workspace_bytes < cv.total) return set_error_text("nhf_create: workspace too small");
+    char *ws = (char *)workspace;
+    if (plan) {
+        // the part tables, copied once into the workspace
+        HIP_TRY(hipMemcpy(ws + cv.cell_chunk, plan->cell_chunk.data(), sizeof(int32_t) * plan->cell_chunk.size(),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ws + cv.side_chunk, plan->side_chunk.data(), sizeof(int32_t) * plan->side_chunk.size(),
+                          hipMemcpyHostToDevice));
+        std::vector<int4> table(plan->range);
+        table.insert(table.end(), plan->extra.begin(), plan->extra.end());
+        HIP_TRY(hipMemcpy(ws + cv.range, table.data(), sizeof(int4) * table.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ws + cv.cfl_part, plan->cfl_part.data(), sizeof(int32_t) * plan->cfl_part.size(),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(ws + cv.ctl, 0, sizeof(Ctl) * plan->range.size()));
+    }
+    nhf_ctx *ctx = new nhf_ctx();
+    bind_dev(ctx->d, mesh, params, plan, ws, cv);
+    ctx->parts = 0;
+    ctx->host_ctl = nullptr;
+    if (plan) {
+        ctx->parts = (int)plan->range.size();
         ctx->part_range = plan->range;
         ctx->part_extra = plan->extra;
         ctx->part_pixels = plan->pixels;
@@ -2007,6 +2065,90 @@ int32_t nhf_clock_probe(const uint64_t *now, const uint64_t *target, const uint6
     }
     (void)hipFree(buf);
     return rc;
+}
+
+// --------------------------------------------------------------------------- no allocation
+
+}  // extern "C"
+
+// Scratch of nhf_run_static: a file-scope device array (zero-filled when the code object loads),
+// so that entry point needs no workspace and allocates nothing. Each GPU the library runs on holds
+// its own copy, at its own address, until the process ends. In host code `scratch` names a host
+// stand-in, not device memory: only the address hipGetSymbolAddress returns for the current
+// device may reach a kernel or a copy.
+static __device__ __align__(256) unsigned char scratch[NHF_STATIC_SCRATCH_BYTES];
+
+#define NHF_STATIC_DEVICES 64
+static std::mutex g_static_lock;
+static char *g_static_ptr[NHF_STATIC_DEVICES];
+
+// The current device's address of `scratch`, looked up once per device.
+static int32_t static_scratch(char **out) {
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    if (device < 0 || device >= NHF_STATIC_DEVICES) return set_error_text("nhf_run_static: device index out of range");
+    std::lock_guard<std::mutex> hold(g_static_lock);
+    if (!g_static_ptr[device]) {
+        void *ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return set_error_text("nhf_run_static: hipGetSymbolAddress returned NULL");
+        g_static_ptr[device] = (char *)ptr;
+    }
+    *out = g_static_ptr[device];
+    return 0;
+}
+
+// Scratch bytes nhf_run_static needs for a mesh: nhf_create's workspace plus the budget totals.
+static size_t static_bytes(const nhf_mesh *mesh, const nhf_params *params) {
+    return align_up(carve(mesh, params).total) + sizeof(double) * 2 * (size_t)(4 + 6 * mesh->b_num);
+}
+
+extern "C" {
+
+int32_t nhf_run_static(const nhf_mesh *mesh, const nhf_params *params, uint64_t now_ticks, uint64_t target_ticks,
+                       uint64_t origin_ticks, const float *coefficients, const float *rain_rates, int32_t raining,
+                       int32_t steps, uint32_t *trace, double *budget, void *stream) {
+    if (!mesh || !params || !trace) return set_error_text("nhf_run_static: null argument");
+    if (int32_t rc = check_mesh("nhf_run_static", mesh, params, false)) return rc;
+    if (steps < 1 || steps > params->batch_capacity) return set_error_text("nhf_run_static: steps out of range");
+    // the scratch keeps the previous call's contents (maybe another mesh's): no forcing carries over
+    if ((mesh->b_num > 0 && !coefficients) || (raining && mesh->n_pixels > 0 && !rain_rates))
+        return set_error_text("nhf_run_static: coefficients and, when raining, rain_rates are required");
+    const size_t need = static_bytes(mesh, params);
+    if (need > NHF_STATIC_SCRATCH_BYTES) {
+        snprintf(nhf_error_text, sizeof nhf_error_text,
+                 "nhf_run_static: this mesh needs %zu bytes of scratch, more than the static maximum of %zu "
+                 "(use nhf_create with a workspace)", need, (size_t)NHF_STATIC_SCRATCH_BYTES);
+        return -2;
+    }
+    char *ws = nullptr;
+    if (int32_t rc = static_scratch(&ws)) return rc;
+    const Carve cv = carve(mesh, params);
+    // A context on the stack over the static scratch: the launches are nhf_create's, the vectors
+    // stay empty (eager launches without timing touch none of them).
+    nhf_ctx ctx{};
+    bind_dev(ctx.d, mesh, params, nullptr, ws, cv);
+    ctx.stream = (hipStream_t)stream;
+    double *totals = (double *)(ws + align_up(cv.total));
+    const int terms = ctx.d.terms;
+    int32_t rc = 0;
+    if ((rc = reset_part(&ctx, 0, now_ticks)) || (rc = forcing_part(&ctx, 0, coefficients, rain_rates, raining)) ||
+        (rc = compute_cfl_any(&ctx)) || (rc = begin_part(&ctx, 0, target_ticks, origin_ticks)) ||
+        (rc = launch_batch(&ctx, steps, false)))
+        return rc;
+    const Dev &d = ctx.d;
+    hipStream_t s = ctx.stream;
+    hipLaunchKernelGGL(k_settle_extrema, dim3(blocks_for(d.m.e_num) < 2048 ? blocks_for(d.m.e_num) : 2048),
+                       dim3(NHF_BLOCK), 0, s, d);
+    if (budget) hipLaunchKernelGGL(k_flush_terms, dim3(terms), dim3(NHF_BLOCK), 0, s, d, totals);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(trace, d.ctl->trace, sizeof(uint32_t) * NHF_TRACE_COLS * steps, hipMemcpyDeviceToHost, s));
+    if (budget) HIP_TRY(hipMemcpyAsync(budget, totals, sizeof(double) * 2 * terms, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (budget)
+        for (int i = 0; i < 2 * terms; ++i)
+            if (!isfinite(budget[i])) return 1;
+    return 0;
 }
 
 // --------------------------------------------------------------------------- parts (nhu_*)

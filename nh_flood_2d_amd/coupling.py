@@ -174,84 +174,111 @@ def prepare_pipe(pipe_cfg: PipeConfig, mesh) -> PipeTopology:
 
 # ------------------------------------------------------------------------------ one window
 
+@dataclass
+class Links:
+    """The (cell, junction) pairs of one mesh that exchange water, sorted by cell then junction:
+    a junction drains its primary cell and its weak neighbourhood, except cells of land class 8
+    and the exterior; outfalls exchange nothing (only their stage is read). `count` is the number
+    of junctions of each cell."""
+    cell: np.ndarray
+    node: np.ndarray
+    count: np.ndarray
+
+    @classmethod
+    def of(cls, topology: PipeTopology, land_types: np.ndarray, n_cells: int) -> 'Links':
+        n_nodes = len(topology.node_names)
+        sizes = np.diff(np.asarray(topology.topo_ptr, dtype=np.int64))
+        node = np.concatenate([np.arange(n_nodes), np.repeat(np.arange(n_nodes), sizes)]).astype(np.int64)
+        cell = np.concatenate([np.asarray(topology.primary_ei, dtype=np.int64),
+                               np.asarray(topology.topo_ei, dtype=np.int64)])
+        keep = ~np.asarray(topology.is_outfall, dtype=bool)[node] & (cell != 0)
+        keep[keep] = np.asarray(land_types)[cell[keep]] != IMPERVIOUS_BUILDING_CLASS
+        pair = np.unique(cell[keep] * n_nodes + node[keep])
+        cell, node = pair // n_nodes, pair % n_nodes
+        return cls(cell, node, np.bincount(cell, minlength=n_cells))
+
+    def nodes_of(self, c: int) -> tuple:
+        lo, hi = np.searchsorted(self.cell, [c, c + 1])
+        return tuple(int(n) for n in self.node[lo:hi])
+
+    __getitem__ = nodes_of
+
+
 def node_cells(topology: PipeTopology, land_types: np.ndarray, n_cells: int) -> list:
-    """For each cell, the sorted unique junctions it drains to (primary or weak link, land class
-    not 8); outfalls take no drainage."""
-    rel = [set() for _ in range(n_cells)]
-    for node in range(len(topology.node_names)):
-        if topology.is_outfall[node]:
-            continue
-        cells = {int(topology.primary_ei[node])}
-        cells.update(int(c) for c in topology.topo_ei[topology.topo_ptr[node]:topology.topo_ptr[node + 1]])
-        for c in cells:
-            if c and land_types[c] != IMPERVIOUS_BUILDING_CLASS:
-                rel[c].add(node)
-    return [tuple(sorted(s)) for s in rel]
+    """For each cell, the sorted junctions it exchanges with (see Links)."""
+    links = Links.of(topology, land_types, n_cells)
+    out = [()] * n_cells
+    for c in np.unique(links.cell):
+        out[int(c)] = links.nodes_of(int(c))
+    return out
+
+
+def orifice(head: float) -> float:
+    """Orifice rate C h^1.5 (m^3/s) for a head h (m), evaluated by the C library's pow."""
+    return ORIFICE_COEFFICIENT * head ** 1.5
 
 
 def compute_exchange(window_dt: float, depth, bed, area, land_types, topology: PipeTopology, heads: dict):
     """Source rates (F32, m^3/s, + into the surface), junction boundaries, diagnostics and the
     cell -> junction links for one window (core/coupled/exchange.py:18-122).
 
-    Drainage of a linked cell: min(C h^1.5, h A / window), shared equally by its junctions.
-    Overflow of a surcharged junction (head above rim + h): min(C dh^1.5, dh A / window / k) x 0.5
-    with dh capped at 3 m and k the junctions whose primary cell it is. The F32 rounding of each
-    cell's source is charged back to its junctions, so the pair balances to rounding."""
+    * Drainage: each linked cell loses min(C h^1.5, h A / window), split equally over its
+      junctions (the water present at the window start bounds it).
+    * Overflow: a surcharged junction (head above rim + h of its primary cell) returns
+      min(C dh^1.5, dh A / window / k) x 0.5 to its primary cell, dh capped at 3 m, k the
+      number of junctions sharing that primary cell.
+    * The surface receives the F32 rounding of each cell's source; that rounding is charged back
+      to the cell's junctions, so surface and pipe see the same volume.
+    Every per-junction sum adds its cells in increasing cell order.
+    """
     depth = np.asarray(depth, dtype=np.float64)
     bed = np.asarray(bed, dtype=np.float64)
     area = np.asarray(area, dtype=np.float64)
     n_cells = len(depth)
     names = topology.node_names
-    rel = node_cells(topology, land_types, n_cells)
+    links = Links.of(topology, land_types, n_cells)
+    linked = np.flatnonzero(links.count)
     source = np.zeros(n_cells, dtype=np.float64)
     node_flow = np.zeros(len(names), dtype=np.float64)
-    drainage = 0.0
-    for ei in range(1, n_cells):
-        if not rel[ei]:
-            continue
-        h = float(depth[ei])
-        q = min(ORIFICE_COEFFICIENT * h ** 1.5, h * float(area[ei]) / window_dt)
-        source[ei] -= q
-        drainage += q
-        for node in rel[ei]:
-            node_flow[node] += q / len(rel[ei])
-    primaries = np.zeros(n_cells, dtype=np.int64)
-    for node in range(len(names)):
-        if not topology.is_outfall[node] and topology.primary_ei[node] > 0:
-            primaries[topology.primary_ei[node]] += 1
+    # drainage of every linked cell, then its equal shares to the junctions
+    drain = np.array([min(orifice(float(depth[c])), float(depth[c]) * float(area[c]) / window_dt) for c in linked])
+    source[linked] -= drain
+    drainage = float(np.add.accumulate(np.concatenate([[0.], drain]))[-1])
+    rate = np.zeros(n_cells)
+    rate[linked] = drain
+    np.add.at(node_flow, links.node, rate[links.cell] / links.count[links.cell])
+    # overflow of surcharged junctions into their primary cell
+    outfall = np.asarray(topology.is_outfall, dtype=bool)
+    primary = np.asarray(topology.primary_ei, dtype=np.int64)
+    sharing = np.bincount(primary[~outfall & (primary > 0)], minlength=n_cells)
     overflow, active = 0.0, 0
     for node, name in enumerate(names):
-        ei = int(topology.primary_ei[node])
-        if topology.is_outfall[node] or ei == 0 or land_types[ei] == IMPERVIOUS_BUILDING_CLASS:
+        c = int(primary[node])
+        if outfall[node] or c == 0 or land_types[c] == IMPERVIOUS_BUILDING_CLASS:
             continue
-        dh = float(heads[name]) - (float(topology.rims[node]) + float(depth[ei]))
-        if dh <= 0:
-            continue
-        dh = min(dh, MAX_OVERFLOW_HEAD_M)
-        q = min(ORIFICE_COEFFICIENT * dh ** 1.5, dh * float(area[ei]) / window_dt / int(primaries[ei]))
-        q *= OVERFLOW_RELAXATION
-        source[ei] += q
-        node_flow[node] -= q
-        overflow += q
-        active += 1
+        rise = float(heads[name]) - (float(topology.rims[node]) + float(depth[c]))
+        if rise > 0:
+            rise = min(rise, MAX_OVERFLOW_HEAD_M)
+            q = min(orifice(rise), rise * float(area[c]) / window_dt / int(sharing[c])) * OVERFLOW_RELAXATION
+            source[c] += q
+            node_flow[node] -= q
+            overflow += q
+            active += 1
+    # the F32 rounding of each linked cell's source, charged back in equal shares
     source_f32 = source.astype(np.float32)
-    for ei in range(1, n_cells):
-        if rel[ei]:
-            correction = (float(source_f32[ei]) - source[ei]) / len(rel[ei])
-            for node in rel[ei]:
-                node_flow[node] -= correction
-    boundaries = {}
-    for node, name in enumerate(names):
-        ei = int(topology.primary_ei[node])
-        stage = float(depth[ei] + bed[ei]) if ei else 0.0
-        outfall = bool(topology.is_outfall[node])
-        boundaries[name] = {'level': stage if outfall else 0.0, 'flow': 0.0 if outfall else float(node_flow[node])}
+    rounding = np.zeros(n_cells)
+    rounding[linked] = (source_f32[linked].astype(np.float64) - source[linked]) / links.count[linked]
+    np.subtract.at(node_flow, links.node, rounding[links.cell])
+    has_cell = primary > 0
+    stage = np.where(has_cell, depth[primary] + bed[primary], 0.)
+    boundaries = {name: ({'level': float(stage[k]), 'flow': 0.0} if outfall[k] else
+                         {'level': 0.0, 'flow': float(node_flow[k])}) for k, name in enumerate(names)}
     source_f32[0] = 0.
-    diagnostics = {'drainage_m3_s': drainage, 'overflow_m3_s': overflow,
-                   'net_pipe_source_m3_s': float(source_f32.sum(dtype=np.float64)),
+    surface = float(source_f32.sum(dtype=np.float64))
+    diagnostics = {'drainage_m3_s': drainage, 'overflow_m3_s': overflow, 'net_pipe_source_m3_s': surface,
                    'net_swmm_inflow_m3_s': float(node_flow.sum()), 'overflow_nodes': active,
-                   'surface_pipe_balance_m3_s': float(source_f32.sum(dtype=np.float64) + node_flow.sum())}
-    return source_f32, boundaries, diagnostics, rel
+                   'surface_pipe_balance_m3_s': float(surface + node_flow.sum())}
+    return source_f32, boundaries, diagnostics, links
 
 
 # ------------------------------------------------------------------------------ pipe models
@@ -369,32 +396,40 @@ class PipeCoupling:
             raise ValueError('Pipe window extends past simulation end')
         if elapsed < 0 < elapsed + seconds:
             raise ValueError('Pipe window must end at zero before official simulation')
-        source, boundaries, diag, rel = compute_exchange(seconds, depth, bed, area, land_types, self.t, self._heads)
-        self._plan = (source, boundaries, rel)
+        source, boundaries, diag, links = compute_exchange(seconds, depth, bed, area, land_types, self.t, self._heads)
+        self._plan = (source, boundaries, links)
         self._pending = dict(elapsed_start_s=elapsed, elapsed_end_s=elapsed + seconds, window_seconds=seconds,
                              swmm_before_s=self._seconds, **diag)
         return source
 
     def end_window(self, actual_volumes) -> None:
+        """Pair the volume each cell actually exchanged with its junctions, then advance the pipe
+        model (core/coupled/runtime.py:152-202): a cell whose volume differs from its request
+        (the limiter held back drainage, or nothing was there to drain) moves its junctions'
+        inflow by the difference, in equal shares; drainage given back that way is recorded."""
         if self._pending is None:
             raise RuntimeError('No open pipe exchange window')
-        source, planned, rel = self._plan
+        source, planned, links = self._plan
         actual = np.asarray(actual_volumes, dtype=np.float64)
         if actual.shape != source.shape or not np.all(np.isfinite(actual)):
             raise ValueError('actual surface volumes must be finite and match the surface mesh')
         seconds = self._pending['window_seconds']
         requested = source.astype(np.float64) * seconds
-        boundaries = {name: dict(v) for name, v in planned.items()}
-        returned = 0.0
-        for ei in np.flatnonzero(actual - requested):
-            change = actual[ei] - requested[ei]
-            if not rel[ei]:
-                raise ValueError(f'Unpaired surface/pipe exchange at cell {ei}')
-            per_node = change / (seconds * len(rel[ei]))
-            for node in rel[ei]:
-                boundaries[self.t.node_names[node]]['flow'] -= per_node
-            if requested[ei] < 0. and change > 0.:
-                returned += change
+        change = actual - requested
+        links_of_node = np.bincount(links.node, minlength=len(self.t.node_names)) > 0
+        moved = np.flatnonzero(change)
+        orphans = moved[links.count[moved] == 0]
+        if orphans.size:
+            raise ValueError(f'Unpaired surface/pipe exchange at cell {int(orphans[0])}')
+        per_cell = np.zeros(len(change))
+        per_cell[moved] = change[moved] / (seconds * links.count[moved])
+        names = self.t.node_names
+        flows = np.array([planned[name]['flow'] for name in names], dtype=np.float64)
+        np.subtract.at(flows, links.node, per_cell[links.cell])     # junction by junction, cell order
+        boundaries = {name: dict(planned[name], flow=float(flows[k]) if links_of_node[k] else planned[name]['flow'])
+                      for k, name in enumerate(names)}
+        given_back = moved[(requested[moved] < 0.) & (change[moved] > 0.)]
+        returned = float(np.add.accumulate(np.concatenate([[0.], change[given_back]]))[-1])
         requested_pipe = sum(v['flow'] for v in planned.values()) * seconds
         applied_pipe = sum(v['flow'] for v in boundaries.values()) * seconds
         balance = float(actual.sum(dtype=np.float64) + applied_pipe)
@@ -404,7 +439,7 @@ class PipeCoupling:
                              applied_surface_exchange_m3=float(actual.sum(dtype=np.float64)),
                              requested_swmm_inflow_m3=float(requested_pipe),
                              applied_swmm_inflow_m3=float(applied_pipe),
-                             limited_drainage_return_m3=float(returned), surface_swmm_balance_error_m3=balance)
+                             limited_drainage_return_m3=returned, surface_swmm_balance_error_m3=balance)
         self._heads, total = self.actor.advance(boundaries, seconds)
         if abs(total - (self._seconds + seconds)) > 0.05:
             raise RuntimeError(f'pipe model clock {total}s differs from {self._seconds + seconds}s')

@@ -23,20 +23,30 @@ from .config import absolute_time
 from .mesh import Mesh
 
 
+def read_series_csv(path, origin) -> tuple[np.ndarray, np.ndarray]:
+    """(seconds from `origin`, values) of a ``time,value`` CSV, in file order."""
+    with open(path, newline='') as f:
+        pairs = [((absolute_time(r['time']) - origin).total_seconds(), float(r['value'])) for r in csv.DictReader(f)]
+    table = np.array(pairs, dtype=np.float64).reshape(-1, 2)
+    return table[:, 0].copy(), table[:, 1].copy()
+
+
 @dataclass
 class TimeSeries:
-    times: np.ndarray   # seconds from the simulation start (F64)
+    """A boundary's prescribed level or discharge: knots in seconds from the simulation start
+    (F64), linear between knots (preprocess/force.py:84-101)."""
+    times: np.ndarray
     values: np.ndarray
 
     @classmethod
     def load(cls, path, origin, begin, end):
-        with open(path, newline='') as f:
-            rows = list(csv.DictReader(f))
-        times = np.array([(absolute_time(r['time']) - origin).total_seconds() for r in rows])
-        values = np.array([float(r['value']) for r in rows])
-        if len(times) < 2 or not np.all(np.isfinite(values)) or np.any(np.diff(times) <= 0):
+        """Read and check a series: at least two knots, finite values, strictly increasing
+        times, and knots at or beyond both ends of [begin, end]."""
+        times, values = read_series_csv(path, origin)
+        usable = len(times) >= 2 and bool(np.isfinite(values).all()) and bool((np.diff(times) > 0).all())
+        if not usable:
             raise ValueError(f'{path}: need finite values and strictly increasing times (at least two)')
-        if times[0] > begin or times[-1] < end:
+        if not (times[0] <= begin and times[-1] >= end):
             raise ValueError(f'{path}: series does not cover [{begin}, {end}] seconds from start')
         return cls(times, values)
 
@@ -91,15 +101,18 @@ class RainField:
 
 @dataclass
 class Boundary:
+    """A prescribed boundary line: the interior cells of the exterior sides it covers, each with
+    its share of the selected length (F64, summing to 1), and its series."""
     name: str
     kind: str            # 'level' | 'discharge'
     cells: np.ndarray
-    weights: np.ndarray  # F64 length shares of the selected exterior edges
+    weights: np.ndarray
     side_ids: np.ndarray
     series: TimeSeries
 
     def distribute(self, q):
-        return self.weights * q
+        """A total discharge split over the boundary's cells by length share."""
+        return q * self.weights
 
 
 @dataclass

@@ -15,6 +15,7 @@ only, so a byte-compatible fastdb4py backend can replace them without touching c
 from __future__ import annotations
 
 import csv
+import re
 import json
 import os
 from dataclasses import dataclass
@@ -112,15 +113,20 @@ def read_depth_frame(path, expected_count=None):
     return values
 
 
+FRAME_NAME = re.compile(r'depth_(.*)')
+
+
 def list_depth_frames(output_root):
-    frames = []
-    for path in (Path(output_root) / 'frames').glob('depth_*.fdb'):
+    """(elapsed seconds, path) of every ``frames/depth_<s>.fdb`` under output_root, by time; a
+    frame whose name is not a whole number of seconds is an error (core/output.py:118-127)."""
+    def seconds_of(path):
         try:
-            seconds = int(path.stem.removeprefix('depth_'))
+            return int(FRAME_NAME.fullmatch(path.stem).group(1))
         except ValueError as exc:
             raise ValueError(f'Invalid depth frame name: {path}') from exc
-        frames.append((seconds, path))
-    return sorted(frames)
+    paths = list((Path(output_root) / 'frames').glob('depth_*.fdb'))
+    keyed = [(seconds_of(p), p) for p in paths]
+    return sorted(keyed)
 
 
 @dataclass
@@ -241,17 +247,18 @@ def seed_checkpoint(depth, bed, q, *, meta: dict | None = None, rain_time=None, 
 
 
 def append_station_samples(path, *, elapsed_seconds, time, samples):
+    """Append one row per station, stations in name order, to ``stations.csv`` (header on first
+    write; core/output.py:271-284). `samples` maps station name -> cell_id, depth, level, u, v."""
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
-    needs_header = not path.exists()
+    rows = [dict(elapsed_seconds=elapsed_seconds, time=time, station=name,
+                 **{key: samples[name][key] for key in STATION_FIELDS[3:]}) for name in sorted(samples)]
+    fresh = not path.exists()
     with path.open('a', newline='') as stream:
-        writer = csv.DictWriter(stream, fieldnames=STATION_FIELDS)
-        if needs_header:
-            writer.writeheader()
-        for name in sorted(samples):
-            sample = samples[name]
-            writer.writerow({'elapsed_seconds': elapsed_seconds, 'time': time, 'station': name,
-                             **{k: sample[k] for k in ('cell_id', 'depth', 'level', 'u', 'v')}})
+        out = csv.DictWriter(stream, fieldnames=STATION_FIELDS)
+        if fresh:
+            out.writeheader()
+        out.writerows(rows)
 
 
 def resolve_station_cells(points: dict, mesh) -> dict:

@@ -30,7 +30,19 @@ FRAME_INTERVAL_S = 900
 
 
 class WaterBudget:
-    """Host F64 water ledger (core/run_record.py:107-161)."""
+    """The host's F64 water ledger of one run, closed against the stored volume at each output
+    (the ``mass_balance.csv`` rows of core/run_record.py:107-161).
+
+    The host integrates the nominal rain and each discharge boundary's requested inflow from the
+    forcing; the device supplies its cumulative source terms, in the step's order: 0 infiltration,
+    1 the positivity floor at non-level cells, 2 the floor at level cells (diagnostic: the level
+    overwrites it), 3 the pipe exchange, then five components per boundary (its level cells'
+    storage change, face flux, pipe, rain and infiltration) and last the discharge each boundary
+    applied after the limiter.
+    """
+
+    TAIL = ('bed_clamp_nonlevel_m3', 'bed_clamp_level_m3', 'gate_bed_m3', 'pipe_exchange_m3',
+            'initial_storage_m3', 'storage_m3', 'residual_m3', 'relative_residual')
 
     def __init__(self, boundaries, initial_storage: float):
         self.boundaries = boundaries
@@ -42,41 +54,48 @@ class WaterBudget:
         self.rows = []
 
     def fields(self):
-        f = ['time', 'elapsed_seconds', 'phase', 'rainfall_m3', 'infiltration_m3']
-        f += [f'{b.kind}:{b.name}_m3' for b in self.boundaries]
-        f += [f'requested_discharge:{b.name}_m3' for b in self.boundaries if b.kind == 'discharge']
-        f += ['bed_clamp_nonlevel_m3', 'bed_clamp_level_m3', 'gate_bed_m3', 'pipe_exchange_m3',
-              'initial_storage_m3', 'storage_m3', 'residual_m3', 'relative_residual']
-        return f
+        inflows = [b for b in self.boundaries if b.kind == 'discharge']
+        return (['time', 'elapsed_seconds', 'phase', 'rainfall_m3', 'infiltration_m3'] +
+                [f'{b.kind}:{b.name}_m3' for b in self.boundaries] +
+                [f'requested_discharge:{b.name}_m3' for b in inflows] + list(self.TAIL))
 
     def add_forcing(self, dt, rain_rate_volume, boundary_values):
-        self.rain += np.float64(dt) * rain_rate_volume
+        step = np.float64(dt)
+        self.rain += step * rain_rate_volume
         for i, b in enumerate(self.boundaries):
             if b.kind == 'discharge':
-                self.discharge[i] += np.float64(dt) * boundary_values[i]
+                self.discharge[i] += step * boundary_values[i]
+
+    def boundary_exchange(self):
+        """Volume each boundary put into the domain so far (negative: taken out): the net of its
+        level cells' five components, and the discharge it applied (0 for a level boundary)."""
+        nb = len(self.boundaries)
+        t = self.device_terms
+        level_net = t[4:4 + 5 * nb].reshape(nb, 5).sum(axis=1, dtype=np.float64)
+        return level_net, t[4 + 5 * nb:4 + 6 * nb]
 
     def output(self, time_text, seconds, storage, phase):
-        nb = len(self.boundaries)
-        terms = self.device_terms
-        levels = terms[4:4 + 5 * nb].reshape(nb, 5).sum(axis=1, dtype=np.float64)
-        floor, level_floor, pipe = terms[1], terms[2], terms[3]
-        inputs = self.rain - terms[0] + floor + pipe
-        actual_discharge = terms[4 + 5 * nb:]
-        inputs += actual_discharge.sum(dtype=np.float64) + levels.sum(dtype=np.float64)
-        residual = float(storage - self.initial - inputs)
-        total_in = self.rain + np.maximum(actual_discharge, 0).sum() + np.maximum(levels, 0).sum() + max(pipe, 0.)
-        scale = max(self.initial, float(total_in),
-                    float(np.abs(actual_discharge).sum() + np.abs(terms[[0, 1, 3]]).sum() + np.abs(levels).sum()))
-        row = dict(time=time_text, elapsed_seconds=seconds, phase=phase,
-                   rainfall_m3=float(self.rain), infiltration_m3=float(terms[0]),
-                   bed_clamp_nonlevel_m3=float(floor), bed_clamp_level_m3=float(level_floor), gate_bed_m3=0.,
-                   pipe_exchange_m3=float(pipe), initial_storage_m3=self.initial,
-                   storage_m3=float(storage), residual_m3=residual,
-                   relative_residual=residual / scale if scale else None)
+        t = self.device_terms
+        infiltration, floor, level_floor, pipe = (float(v) for v in t[:4])
+        level_net, applied = self.boundary_exchange()
+        boundaries_in = applied.sum(dtype=np.float64) + level_net.sum(dtype=np.float64)
+        net_input = self.rain - infiltration + floor + pipe + boundaries_in
+        residual = float(storage - self.initial - net_input)
+        gross_in = (self.rain + np.maximum(applied, 0.).sum() + np.maximum(level_net, 0.).sum() +
+                    max(pipe, 0.))
+        turnover = (np.abs(applied).sum() + abs(infiltration) + abs(floor) + abs(pipe) +
+                    np.abs(level_net).sum())
+        scale = max(self.initial, float(gross_in), float(turnover))
+        row = dict(time=time_text, elapsed_seconds=seconds, phase=phase, rainfall_m3=float(self.rain),
+                   infiltration_m3=infiltration)
         for i, b in enumerate(self.boundaries):
-            row[f'{b.kind}:{b.name}_m3'] = float(actual_discharge[i] if b.kind == 'discharge' else levels[i])
+            row[f'{b.kind}:{b.name}_m3'] = float(applied[i] if b.kind == 'discharge' else level_net[i])
+        for i, b in enumerate(self.boundaries):
             if b.kind == 'discharge':
                 row[f'requested_discharge:{b.name}_m3'] = float(self.discharge[i])
+        row.update(bed_clamp_nonlevel_m3=floor, bed_clamp_level_m3=level_floor, gate_bed_m3=0.,
+                   pipe_exchange_m3=pipe, initial_storage_m3=self.initial, storage_m3=float(storage),
+                   residual_m3=residual, relative_residual=residual / scale if scale else None)
         self.last_row = row
         self.rows.append(row)
         return row

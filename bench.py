@@ -600,20 +600,12 @@ def served_bench(args) -> dict:
     from nh_flood_2d_amd.forcing import cell_forcing_arrays
     from nh_flood_2d_amd.layout import build_layout
     from nh_flood_2d_amd.serve import FloodServer, FrameStream, Request, footprint_bytes
-    rng = np.random.default_rng(args.seed + 11)
     t0 = time.time()
     specs = []
-    arrival = 0.
-    for i in range(args.serve_requests):
-        cells = float(np.exp(rng.uniform(np.log(args.serve_min_cells), np.log(args.serve_max_cells))))
-        blocks = max(8, int(round(np.sqrt(cells / 1.6))))
-        duration = float(np.round(rng.uniform(args.serve_min_h, args.serve_max_h) * 3600. / 300.) * 300.)
-        base = synth.two_level_fractal(args.seed + 100 + i, blocks=blocks, duration=duration,
-                                       tide=1.0 if rng.uniform() < 0.5 else 0.)
-        sc = synth.scenario_variants(base, 1, seed=args.seed + 200 + i)[0]
+    for i, sc, arrival in synth.serving_load(args.seed, args.serve_requests, args.serve_rate, args.serve_min_cells,
+                                             args.serve_max_cells, args.serve_min_h, args.serve_max_h):
         layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, synth.scenario_forcing(sc)),
                               sc.sita)
-        arrival += float(rng.exponential(1. / args.serve_rate)) if i else 0.
         specs.append((i, sc, layout, arrival))
     log(f'serving workload: {len(specs)} solves, cells {[s[1].mesh.e_num - 1 for s in specs]}, '
         f'hours {[round(s[1].duration / 3600, 2) for s in specs]} ({time.time() - t0:.1f} s)')

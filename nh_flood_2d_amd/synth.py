@@ -603,6 +603,26 @@ def scenario_variants(base: Scenario, count: int, seed: int = 0, peak_range=(20.
     return out
 
 
+def serving_load(seed: int, count: int, rate: float = 2.0, min_cells: float = 100_000,
+                 max_cells: float = 8_000_000, min_hours: float = 1.0, max_hours: float = 6.0) -> list:
+    """The serving load: `count` solves arriving as a Poisson process of `rate` per second, each a
+    seeded two-level mesh with a cell count log-uniform in [min_cells, max_cells], a simulated
+    duration uniform in [min_hours, max_hours] (whole 300 s), a triangular hyetograph with peak
+    U[20, 150] mm/h and, for about half of them, a sinusoidal tide boundary. Returns
+    (index, scenario, arrival seconds) in arrival order."""
+    rng = np.random.default_rng(seed + 11)
+    out, arrival = [], 0.
+    for i in range(count):
+        cells = float(np.exp(rng.uniform(np.log(min_cells), np.log(max_cells))))
+        blocks = max(8, int(round(np.sqrt(cells / 1.6))))
+        duration = float(np.round(rng.uniform(min_hours, max_hours) * 3600. / 300.) * 300.)
+        base = two_level_fractal(seed + 100 + i, blocks=blocks, duration=duration,
+                                 tide=1.0 if rng.uniform() < 0.5 else 0.)
+        arrival += float(rng.exponential(1. / rate)) if i else 0.
+        out.append((i, scenario_variants(base, 1, seed=seed + 200 + i)[0], arrival))
+    return out
+
+
 def simulation(sc: Scenario, engine_factory=None, end_time: float | None = None, yield_step: float = 300.,
                batch_size: int = 32, base_layout=None, outputs=None, coupling=None):
     """A Simulation of a synthetic scenario (no files); `base_layout` reuses another scenario's

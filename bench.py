@@ -326,6 +326,21 @@ def _set_forcing(eng, forcing, t: float):
                     rain.rates(rain.index_at(t)).astype(np.float32), True)
 
 
+def _chunks(steps: int, batch: int) -> list:
+    """Batch sizes the protocol below uses for `steps` steps."""
+    return [min(batch, steps - done) for done in range(0, steps, batch)]
+
+
+def _prime(engines, run_batch, steps: int, now: list, batch: int = 32) -> None:
+    """Capture, outside any timed region, the graph of every batch size a run of `steps` steps will
+    launch: each is launched once towards a target equal to the clock, so every step is inactive
+    and the state does not change (the kernels return at once)."""
+    for n in sorted(set(_chunks(steps, batch))):
+        for i, eng in enumerate(engines):
+            eng.begin(now[i], now[i])
+        run_batch(n)
+
+
 def _run_protocol(engines, run_batch, steps: int, horizon: int, now: list, batch: int = 32) -> int:
     """Advance every engine `steps` steps with the solver's batch protocol (begin, run, read the
     trace); returns the completed steps summed over the engines."""
@@ -393,10 +408,42 @@ def small_mesh_bench(args, device) -> dict:
             'ratio_to_reference': REFERENCE_SMALL_MS / ms}
 
 
-def batched_bench(args, device, base=None, label: str = 'batch', t_start: float = 1800.) -> dict:
+def develop_batch(variants, layout0, params, device, until: float, per_graph: int = 64) -> list:
+    """Every scenario's own storm run on the same-mesh batch (BatchRunner: one slot each, own dt)
+    from its initial state to `until` seconds; returns each scenario's state."""
+    from nh_flood_2d_amd import synth
+    from nh_flood_2d_amd.batch import BatchEngine, BatchRunner
+    batch = BatchEngine(len(variants), device=device)
+    batch.set_graph(True)
+    runner = BatchRunner(batch, batch_size=per_graph)
+    sims, states = {}, {}
+
+    class Final:   # the state at the last output, before the runner frees the slot
+        def __init__(self, key):
+            self.key = key
+
+        def emit(self, sim, seconds, phase, frame, final, depth, row, stats):
+            if final:
+                states[self.key] = sim.state()
+
+        def finish(self, sim):
+            pass
+    for i, v in enumerate(variants):
+        def make(factory, i=i, v=v):
+            sims[i] = synth.simulation(v, engine_factory=factory, end_time=until, yield_step=until,
+                                       batch_size=per_graph, base_layout=layout0, outputs=Final(i))
+            return sims[i]
+        runner.submit(i, make)
+    runner.run()
+    batch.close()
+    return [states[i] for i in range(len(variants))], max(sim.steps for sim in sims.values())
+
+
+def batched_bench(args, device, base=None, label: str = 'batch') -> dict:
     """Same-mesh scenario batch: `--batch-slots` scenarios (rain peak, Manning factor and start
-    state drawn per scenario) on one shared mesh (by default the two-level 1 M-cell mesh), against
-    one of them on the plain path."""
+    state drawn per scenario) on one shared mesh (by default the two-level 1 M-cell mesh), each
+    developed by its own storm to the rain peak (a third of its duration) in the batch, then
+    timed from there; against the first of them alone on the plain path from the same state."""
     import torch
     from nh_flood_2d_amd import synth
     from nh_flood_2d_amd.batch import BatchEngine
@@ -413,16 +460,21 @@ def batched_bench(args, device, base=None, label: str = 'batch', t_start: float 
     cf = cell_forcing_arrays(mesh, synth.scenario_forcing(variants[0]))
     layout0 = build_layout(mesh, variants[0].land_parameters, cf, base.sita)
     params = make_params(base.afa, base.sita, base.min_h, 64)
+    t_start = base.duration / 3
     log(f'{label} workload: {len(variants)} scenarios x {cells} cells, {mesh.s_num - 1} sides ({time.time() - t0:.1f} s)')
+    w0 = time.perf_counter()
+    developed, develop_steps = develop_batch(variants, layout0, params, device, t_start)
+    wet = float(np.mean([np.mean(st['depth'][1:] >= base.min_h) for st in developed]))
+    log(f'{label} developed: every scenario to {t_start:.0f} s (the rain peak), at most {develop_steps} steps, '
+        f'{time.perf_counter() - w0:.1f} s wall; wet cells {wet:.3f} on average')
     start = seconds_to_ticks(t_start)
     horizon = seconds_to_ticks(t_start + 1e5)
 
     def states():
-        for v in variants:
+        for v, dev_state in zip(variants, developed):
             lay = relayout(layout0, mesh, v.land_parameters, cf)
             st = initial_state(mesh, lay)
-            if v.initial_depth is not None:
-                st['depth'] = v.initial_depth.copy()
+            st.update({k: x.copy() for k, x in dev_state.items()})
             yield v, lay, st
 
     # plain single-scenario path on the first scenario
@@ -435,6 +487,7 @@ def batched_bench(args, device, base=None, label: str = 'batch', t_start: float 
     now = [start]
     one = lambda n: plain.run_batch(n)[None]
     _run_protocol([plain], one, args.warmup, horizon, now)
+    _prime([plain], one, args.steps, now)
     plain.synchronize()
     w0 = time.perf_counter()
     steps_plain = _run_protocol([plain], one, args.steps, horizon, now)
@@ -462,6 +515,7 @@ def batched_bench(args, device, base=None, label: str = 'batch', t_start: float 
             f"{per_graph} steps per graph, {btune['us_per_step']:.0f} us/step")
     now = [start] * len(views)   # forcing origins; the slots' clocks continue from the tuning steps
     _run_protocol(views, batch.run_batch, args.warmup, horizon, now, per_graph)
+    _prime(views, batch.run_batch, args.steps, now, per_graph)
     batch.synchronize()
     w0 = time.perf_counter()
     steps_batch = _run_protocol(views, batch.run_batch, args.steps, horizon, now, per_graph)
@@ -485,20 +539,23 @@ def batched_bench(args, device, base=None, label: str = 'batch', t_start: float 
     return {
         'workload': f'{len(variants)} scenarios on one shared mesh ({base.name}), {cells} cells, '
                     f'{mesh.s_num - 1} sides; rain peak U[20,150] mm/h, Manning x lognormal(0.2), dry or 0.3 m '
-                    'pond start, own dt per scenario',
+                    f'pond start, own dt per scenario; each developed by its own storm to the rain peak '
+                    f'({t_start:.0f} s) in the batch, then timed',
         'scenarios': len(variants), 'cells': cells, 'value': batch_value, 'unit': 'cell-updates/s',
         'plain_value': plain_value, 'batched_vs_plain': batch_value / plain_value,
         'scenario_steps': steps_batch, 'wall_s': batch_wall, 'plain_steps': steps_plain,
         'simulated_s_min': min(sim_seconds), 'simulated_s_max': max(sim_seconds), 'kernels': kernels,
-        'tuning': btune, 'batch_steps': per_graph,
+        'tuning': btune, 'batch_steps': per_graph, 'wet_cell_frac': wet,
+        'developed': dict(until_s=t_start, max_steps=int(develop_steps)),
     }
 
 
 def uneven_bench(args, device) -> dict:
     """Uneven-size batch: --uneven-meshes meshes with cell counts log-uniform in
     [--uneven-min-cells, --uneven-max-cells], 1-3 refinement levels (finest cells 5 m), seeded
-    fractal DEMs, 2-4 land classes with Horton infiltration, rain at its peak, a tide on half of
-    them and a 0.3 m pond on half. All advance together in one union (one launch sequence per step
+    fractal DEMs, 2-4 land classes with Horton infiltration, a tide on half of them and a 0.3 m
+    pond on half, each developed by its own storm to the rain peak in one union. From there all
+    advance together in one union (one launch sequence per step
     for every mesh, each with its own dt; autotuned cell-pass shape and steps per graph, one
     hipGraph per batch); the same steps are then timed one mesh at a time on single-mesh engines."""
     import torch
@@ -517,19 +574,34 @@ def uneven_bench(args, device) -> dict:
     sc0 = items[0][0]
     params = make_params(sc0.afa, sc0.sita, sc0.min_h, 64)
 
-    def state_of(sc, layout):
+    t_peak = sc0.duration / 3
+    # every mesh's own storm to its rain peak, all in one union (UnionRunner: own dt per mesh)
+    from nh_flood_2d_amd.union import UnionRunner
+    w0 = time.perf_counter()
+    developer = UnionEngine([lay for _, lay in items], params, device=device)
+    developer.set_graph(True)
+    sims = {p: synth.simulation(sc, engine_factory=developer.view(p), end_time=t_peak, yield_step=t_peak,
+                                base_layout=lay) for p, (sc, lay) in enumerate(items)}
+    UnionRunner(developer, sims).run()
+    developed = [sims[p].state() for p in range(len(items))]
+    develop_steps = max(sim.steps for sim in sims.values())
+    developer.close()
+    del developer, sims
+    wet = float(np.sum([np.sum(st['depth'][1:] >= sc0.min_h) for st in developed]) / cells.sum())
+    log(f'uneven developed: every mesh to {t_peak:.0f} s (the rain peak), at most {develop_steps} steps, '
+        f'{time.perf_counter() - w0:.1f} s wall; wet cells {wet:.3f}')
+
+    def state_of(p, sc, layout):
         st = initial_state(sc.mesh, layout)
-        if sc.initial_depth is not None:
-            st['depth'] = sc.initial_depth.copy()
+        st.update({k: x.copy() for k, x in developed[p].items()})
         return st
 
-    t_peak = sc0.duration / 3
     start = seconds_to_ticks(t_peak)
     horizon = seconds_to_ticks(t_peak + 1e6)
     union = UnionEngine([lay for _, lay in items], params, device=device)
     views = []
     for p, (sc, lay) in enumerate(items):
-        v = union.view(p)(lay, params, state_of(sc, lay))
+        v = union.view(p)(lay, params, state_of(p, sc, lay))
         _set_forcing(v, synth.scenario_forcing(sc), t_peak)
         v.reset(start)
         views.append(v)
@@ -544,6 +616,7 @@ def uneven_bench(args, device) -> dict:
             f"{per_graph} steps per graph, {utune['us_per_step']:.0f} us/step")
     now = [start] * len(views)    # forcing origins; the parts' clocks continue from the tuning steps
     _run_protocol(views, union.run_batch, args.uneven_warmup, horizon, now, per_graph)
+    _prime(views, union.run_batch, args.uneven_steps, now, per_graph)
     union.synchronize()
     w0 = time.perf_counter()
     _run_protocol(views, union.run_batch, args.uneven_steps, horizon, now, per_graph)
@@ -556,8 +629,8 @@ def uneven_bench(args, device) -> dict:
     torch.cuda.synchronize()
     # the same steps one mesh at a time (tuned like the union's cell passes)
     plain_wall = 0.0
-    for sc, lay in items:
-        eng = Engine(lay, params, state_of(sc, lay), device=device)
+    for p, (sc, lay) in enumerate(items):
+        eng = Engine(lay, params, state_of(p, sc, lay), device=device)
         if utune:
             eng.set_tuning(eng.tuning_range()['cfl_blocks'], utune['cell_block'], utune['cells_per_thread'])
         _set_forcing(eng, synth.scenario_forcing(sc), t_peak)
@@ -567,6 +640,7 @@ def uneven_bench(args, device) -> dict:
         one = lambda n, eng=eng: eng.run_batch(n)[None]
         mine = [start]
         _run_protocol([eng], one, args.uneven_warmup, horizon, mine, per_graph)
+        _prime([eng], one, args.uneven_steps, mine, per_graph)
         eng.synchronize()
         w0 = time.perf_counter()
         _run_protocol([eng], one, args.uneven_steps, horizon, mine, per_graph)
@@ -581,8 +655,9 @@ def uneven_bench(args, device) -> dict:
     return {
         'workload': f'{len(items)} meshes, {int(cells.sum())} cells in all ({int(cells.min())} to {int(cells.max())}, '
                     f'log-uniform {args.uneven_min_cells}-{args.uneven_max_cells}), 1-3 levels (finest 5 m), '
-                    'fractal DEMs, 2-4 land classes with Horton, rain at peak, tide on half, pond on half, '
-                    'own dt per mesh',
+                    'fractal DEMs, 2-4 land classes with Horton, tide on half, pond on half, own dt per mesh; '
+                    f'each developed by its own storm to the rain peak ({t_peak:.0f} s) in the union, then timed',
+        'wet_cell_frac': wet, 'developed': dict(until_s=t_peak, max_steps=int(develop_steps)),
         'meshes': len(items), 'cells': int(cells.sum()), 'value': union_value, 'unit': 'cell-updates/s',
         'plain_value': plain_value, 'uneven_vs_plain': union_value / plain_value, 'steps': args.uneven_steps,
         'wall_s': union_wall, 'plain_wall_s': plain_wall, 'tuning': utune, 'batch_steps': per_graph,
@@ -758,6 +833,7 @@ def main():
         return done
 
     run(args.warmup)
+    _prime([eng], lambda n: eng.run_batch(n)[None], args.steps, [now], per_graph)
     eng.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -870,7 +946,7 @@ def main():
             # the plain path is bound by launches rather than bytes
             from nh_flood_2d_amd import synth
             result['batched_small'] = batched_bench(args, device, base=synth.plain_uniform(args.seed, n=256, dx=10.),
-                                                    label='small batch', t_start=900.)
+                                                    label='small batch')
     if args.uneven_meshes:
         result['uneven'] = uneven_bench(args, device)
     if args.serve_requests:

@@ -146,9 +146,11 @@ def kernel_base_name(name: str) -> str:
     return m.group(1) if m else name
 
 
-def pmc_summary(fetch_csv: str, write_csv: str) -> str:
+def pmc_summary(fetch_csv: str, write_csv: str, launches: dict | None = None) -> str:
     """Per-kernel HBM bytes per dispatch from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE in
     KiB), scaled by what each counter reads for the 256 MiB device copy of --calibrate-copy.
+    `launches` (summary name -> count) keeps only each kernel's last that many dispatches of its
+    section: the per-kernel timing pass, after the storm that develops the state.
     Lines: 'calibration <counter> ... factor F' and 'kernel <name> ... hbm_bytes B'."""
     import csv
     from collections import defaultdict
@@ -172,6 +174,11 @@ def pmc_summary(fetch_csv: str, write_csv: str) -> str:
                 per_kernel[PMC_KERNELS[name]].append(value)
         if not copies:
             raise SystemExit(f'{path}: no calibration copy found (run bench.py with --calibrate-copy)')
+        if launches:
+            for k in per_kernel:
+                n = launches.get(k)
+                if n:
+                    per_kernel[k] = per_kernel[k][-n:]
         factor = copy_bytes / (sum(copies) / len(copies))
         lines.append(f'calibration {counter} copy_bytes {copy_bytes:.0f} counter_bytes '
                      f'{sum(copies) / len(copies):.0f} factor {factor:.4f}')
@@ -230,10 +237,17 @@ def trace_summary(trace_csv: str, bench_json: str) -> str:
         lines.append(f'| {name} | {len(every[name])} | {np.mean(every[name]) / 1e3:.1f} | {len(d)} | '
                      f'{np.mean(d) / 1e3:.1f} | {ev:.1f} |' if d else f'| {name} | {len(every[name])} | - | 0 | - | - |')
     batch = sorted((k for k in every if k.startswith('kb_')), key=lambda k: -sum(every[k]))
-    lines += ['', '| Batch kernel | Dispatches | Average (us) | Min (us) | Max (us) |', '| --- | ---: | ---: | ---: | ---: |']
+    bk = res.get('batched', {}).get('kernels', {})
+    lines += ['', '| Batch kernel | Dispatches (all) | Average all (us) | Dispatches (timing pass) | '
+              'Average timing pass (us) | HIP-event average (us) |', '| --- | ---: | ---: | ---: | ---: | ---: |']
     for name in batch:
         d = np.array(every[name]) / 1e3
-        lines.append(f'| {name} | {len(d)} | {d.mean():.1f} | {d.min():.1f} | {d.max():.1f} |')
+        short = name[3:]
+        n = bk.get(short, {}).get('launches', 0)
+        tail = d[-n:] if n else d[:0]
+        ev = bk.get(short, {}).get('avg_us')
+        lines.append(f'| {name} | {len(d)} | {d.mean():.1f} | {len(tail)} | '
+                     + (f'{tail.mean():.1f}' if len(tail) else '-') + ' | ' + (f'{ev:.1f}' if ev else '-') + ' |')
     # the parts instantiations of the single-mesh kernels (meshes of different sizes in one union)
     parts = defaultdict(list)
     for r in rows:
@@ -242,11 +256,17 @@ def trace_summary(trace_csv: str, bench_json: str) -> str:
         if name.startswith('k_') and re.search(r'<[^>]*\btrue\b', full):
             parts[name].append(int(r['End_Timestamp']) - int(r['Start_Timestamp']))
     if parts:
-        lines += ['', '| Union kernel (parts) | Dispatches | Average (us) | Min (us) | Max (us) |',
-                  '| --- | ---: | ---: | ---: | ---: |']
+        # the timed union run is the union's last activity: its steps are each kernel's last dispatches
+        timed = res.get('uneven', {}).get('steps', 0)
+        lines += ['', '| Union kernel (parts) | Dispatches (all) | Average all (us) | Dispatches (timed run) | '
+                  'Average timed run (us) | Min timed (us) | Max timed (us) |',
+                  '| --- | ---: | ---: | ---: | ---: | ---: | ---: |']
         for name in sorted(parts, key=lambda k: -sum(parts[k])):
             d = np.array(parts[name]) / 1e3
-            lines.append(f'| {name} | {len(d)} | {d.mean():.1f} | {d.min():.1f} | {d.max():.1f} |')
+            tail = d[-timed:] if timed and name not in ('k_gradients',) else d[:0]
+            stats = (f'{len(tail)} | {tail.mean():.1f} | {tail.min():.1f} | {tail.max():.1f}' if len(tail)
+                     else '0 | - | - | -')
+            lines.append(f'| {name} | {len(d)} | {d.mean():.1f} | {stats} |')
     return '\n'.join(lines) + '\n'
 
 
@@ -763,8 +783,11 @@ def main():
         with open(bench_json) as f:
             line = [ln for ln in f if ln.startswith('{')][-1]
         cfg = json.loads(line)['config']
+        res = json.loads(line)
+        launches = {k: v['launches'] for k, v in res.get('kernels', {}).items()}
+        launches.update({f'batch_{k}': v['launches'] for k, v in res.get('batched', {}).get('kernels', {}).items()})
         print(f"workload cells {cfg['cells']} sides {cfg['sides']} side_list {cfg['side_list']}")
-        print(pmc_summary(fetch_csv, write_csv), end='')
+        print(pmc_summary(fetch_csv, write_csv, launches), end='')
         return
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '

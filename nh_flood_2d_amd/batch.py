@@ -178,6 +178,13 @@ class BatchEngine:
         """Enqueue one batch for every slot with a begun target and return at once."""
         abi.check(self.lib, self.lib.nhb_launch_batch(self.ctx, int(steps)), 'nhb_launch_batch')
         self._pending = int(steps)
+        marker = getattr(self.device, 'marker', None)
+        self._done = marker() if marker is not None else None
+
+    def ready(self) -> bool:
+        """True when the launched batch and its trace copy have finished."""
+        done = getattr(self, '_done', None)
+        return done is None or done.query()
 
     def wait_batch(self) -> np.ndarray:
         """Wait for the launched batch; returns the [slots, steps, 9] trace."""

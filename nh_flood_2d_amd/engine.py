@@ -104,6 +104,13 @@ class TorchDevice:
     def synchronize(self) -> None:
         self.stream.synchronize()
 
+    def marker(self):
+        """An event recorded now on this device's stream: its query() turns true once everything
+        enqueued before it has run."""
+        event = self.torch.cuda.Event()
+        event.record(self.stream)
+        return event
+
 
 class Engine:
     """A mesh, its state and the step library (``nhf_*`` entry points)."""
@@ -269,6 +276,13 @@ class Engine:
         """Enqueue a batch on this engine's stream and return at once (see wait_batch)."""
         abi.check(self.lib, self.lib.nhf_launch_batch(self.ctx, int(steps)), 'nhf_launch_batch')
         self._pending = int(steps)
+        marker = getattr(self.device, 'marker', None)
+        self._done = marker() if marker is not None else None
+
+    def ready(self) -> bool:
+        """True when the launched batch and its trace copy have finished (wait_batch will not block)."""
+        done = getattr(self, '_done', None)
+        return done is None or done.query()
 
     def wait_batch(self) -> np.ndarray:
         """Wait for the launched batch and return its trace rows."""

@@ -119,7 +119,7 @@ class FloodServer:
 
     def __init__(self, budget_bytes: int | None = None, batch_size: int = 32, device: str = 'cuda:0',
                  max_live: int = 64, group_slots: int = 8, device_factory: Callable | None = None,
-                 graph: bool = True):
+                 graph: bool = True, poll: bool = True):
         if budget_bytes is None:
             import torch
             total = torch.cuda.get_device_properties(device).total_memory
@@ -130,6 +130,7 @@ class FloodServer:
         self.max_live = max_live
         self.group_slots = group_slots
         self.graph = graph
+        self.poll = poll   # consume batches in the order they finish (else oldest first)
 
     def _engine_factory(self, stream_device):
         def make(layout, params, state, pipe_q=None):
@@ -215,9 +216,20 @@ class FloodServer:
                 if not live and not waiting and pending and realtime:
                     sleep(max(0., min(0.05, pending[0].arrival - (perf_counter() - t0))))
                 continue
-            # consume the oldest batch in flight while the others run; its engine is planned and
-            # relaunched at the top of the next pass
-            owner, members = inflight.pop(0)
+            # consume a finished batch while the others run, the first to finish rather than the
+            # oldest (a small mesh's batch must not wait behind a large one's); its engine is planned
+            # and relaunched at the top of the next pass. Until one finishes, keep polling, and go
+            # back to admission as soon as a request arrives.
+            ready = 0
+            if self.poll:
+                ready = None
+                while ready is None:
+                    ready = next((i for i, (o, _) in enumerate(inflight) if o.ready()), None)
+                    if ready is None and pending and realtime and pending[0].arrival <= perf_counter() - t0:
+                        break       # a request has arrived: admit it first
+                if ready is None:
+                    continue
+            owner, members = inflight.pop(ready)
             trace = owner.wait_batch()
             busy.discard(id(owner))
             for key, sim in members:

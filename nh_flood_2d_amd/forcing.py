@@ -88,11 +88,9 @@ class RainField:
         if np.any((rr < 0) | (rr >= height) | (cc < 0) | (cc >= width)):
             raise ValueError('Cell centre is outside the rainfall raster')
         pixels, inverse = np.unique(rr * width + cc, return_inverse=True)
-        rain = raster.bands[:, pixels // width, pixels % width].astype(float)
-        bad = ~np.isfinite(rain) | (rain < 0)
-        if raster.nodata is not None:
-            bad |= rain == raster.nodata
-        if np.any(bad):
+        samples = raster.bands[:, pixels // width, pixels % width]
+        rain = samples.astype(float)
+        if np.any(raster.is_nodata(samples) | ~np.isfinite(rain) | (rain < 0)):
             raise ValueError('Rainfall at a cell centre is NoData, nonfinite or negative')
         mapping = np.zeros(mesh.e_num, dtype=np.int32)
         mapping[1:] = inverse

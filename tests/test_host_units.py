@@ -262,6 +262,69 @@ def test_rain_nodata_or_outside_is_rejected(tmp_path, square4, outside):
                        square4, 2326, ORIGIN, 0, 600)
 
 
+def _rain_csv(tmp_path, bands=2):
+    times = ['2023-09-07T08:00:00Z', '2023-09-07T08:05:00Z', '2023-09-07T08:10:00Z', '2023-09-07T08:15:00Z']
+    rows = ''.join(f'{times[k]},{times[k + 1]}\n' for k in range(bands))
+    (tmp_path / 'rain.csv').write_text('interval_start,interval_end\n' + rows)
+    return NS(raster=str(tmp_path / 'rain.tif'), intervals=str(tmp_path / 'rain.csv'))
+
+
+def test_rain_pixel_is_point_is_shifted_half_a_pixel(tmp_path, square4):
+    """A 3 x 3 grid whose pixel corners sit at x = -0.75 + k, y = 2.75 - k, written once by its
+    corner and once (PixelIsPoint) by the centre of its first pixel: both map the unit cells'
+    centres to the same pixels. Read without the shift, every centre would move one pixel up
+    and left (22 -> 11 ...)."""
+    from nh_flood_2d_amd.geotiff import read_geotiff, write_geotiff
+    band = np.array([[10 * r + c + 1 for c in range(3)] for r in range(3)], np.float32)
+    bands = np.stack([band, np.zeros_like(band)])
+    got = {}
+    for point in (False, True):
+        west, north = (-.25, 2.25) if point else (-.75, 2.75)
+        write_geotiff(tmp_path / 'rain.tif', bands, west, north, 1., 1., 2326, pixel_is_point=point)
+        raster = read_geotiff(tmp_path / 'rain.tif')
+        assert (raster.west, raster.north) == (-.75, 2.75)
+        r = RainField.load(_rain_csv(tmp_path), square4, 2326, ORIGIN, 0, 600)
+        got[point] = r.rates(0)[r.pixel_index][1:] * 300
+    np.testing.assert_allclose(got[True], [.022, .023, .012, .013])
+    np.testing.assert_array_equal(got[True], got[False])
+
+
+@pytest.mark.parametrize('case,message', [('matrix', 'ModelTransformationTag'), ('bare', 'required')])
+def test_rain_raster_without_scale_and_tiepoint_is_refused(tmp_path, square4, case, message):
+    from nh_flood_2d_amd.geotiff import write_geotiff
+    bands = np.zeros((2, 2, 2), np.float32)
+    if case == 'matrix':
+        write_geotiff(tmp_path / 'rain.tif', bands, 0., 2., 1., 1., 2326,
+                      transformation=[1., 0., 0., 0., 0., -1., 0., 2., 0., 0., 1., 0., 0., 0., 0., 1.])
+    else:
+        write_geotiff(tmp_path / 'rain.tif', bands, 0., 2., 1., 1., 2326, georeference=False)
+    with pytest.raises(ValueError, match=message):
+        RainField.load(_rain_csv(tmp_path), square4, 2326, ORIGIN, 0, 600)
+
+
+@pytest.mark.parametrize('dtype,nodata,pixel,rejected', [
+    ('<f4', 0.1, 0.1, True),          # 0.1 is no float32: compared as float64 it would pass as rain
+    ('<f4', 0.1, 0.2, False),
+    ('<i2', -9999, -9999, True),
+    ('<u1', -1, 255, False),          # NODATA outside the byte range masks no pixel
+    ('<u1', 7.5, 7, False),           # nor does a fractional one on an integer band
+])
+def test_rain_nodata_is_compared_in_the_band_type(tmp_path, square4, dtype, nodata, pixel, rejected):
+    from nh_flood_2d_amd.geotiff import read_geotiff, write_geotiff
+    bands = np.zeros((2, 2, 2), dtype=dtype)
+    bands[0, 1, 1] = pixel
+    write_geotiff(tmp_path / 'rain.tif', bands, 0., 2., 1., 1., 2326, nodata=nodata, dtype=dtype)
+    assert read_geotiff(tmp_path / 'rain.tif').bands.dtype == np.dtype(dtype).newbyteorder('=')
+    cfg = _rain_csv(tmp_path)
+    if rejected:
+        with pytest.raises(ValueError, match='NoData'):
+            RainField.load(cfg, square4, 2326, ORIGIN, 0, 600)
+    else:
+        r = RainField.load(cfg, square4, 2326, ORIGIN, 0, 600)
+        # pixel (1, 1) holds the cell centred at (1.5, 0.5), the second cell in mesh order
+        assert r.rates(0)[r.pixel_index][2] * 300 == pytest.approx(float(np.dtype(dtype).type(pixel)) / 1000)
+
+
 @pytest.mark.parametrize('start', [1694054400, '2023-09-07T16:00:00'])
 def test_start_requires_timezone_iso8601(tmp_path, start):
     (tmp_path / 'ne').write_text('mesh')

@@ -315,6 +315,12 @@ typedef struct nhb_state {
     float *q;
     const float *manning;        /* [s_num * slots] per-scenario side Manning n, or NULL to share
                                     mesh->manning */
+    const float *class_manning;  /* [256 * slots] per-scenario Manning n by land class code, slot
+                                    innermost, or NULL. When set, `manning` is not read: a side's n
+                                    is rebuilt per scenario from its two cells' classes (the class
+                                    bits of mesh->cell_meta) as map_land_parameters forms it
+                                    (preprocess/hydraulics.py:63-93): the F32 mean of the two cells'
+                                    n, or the interior cell's n on an exterior side. */
     /* boundary cells (static): bslot[cell] = index into the list or -1; the list is sorted by
        boundary, and bcell_start[b] .. bcell_start[b + 1] are boundary b's cells */
     const int32_t *bslot;        /* [e_num] */

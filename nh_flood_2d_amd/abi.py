@@ -49,7 +49,7 @@ class MeshStruct(ctypes.Structure):
 
 # nhb_state: per-scenario state of a batch ([entity][slot] device arrays) and the boundary cells
 BATCH_STATE_FIELDS = ['depth', 'rain_time', 'capacity', 'max_depth', 'max_speed', 'peak_tick', 'q',
-                      'manning', 'bslot', 'bcell_start']
+                      'manning', 'class_manning', 'bslot', 'bcell_start']
 
 
 class BatchState(ctypes.Structure):

@@ -86,8 +86,9 @@ class BatchEngine:
                 dtype = torch.int64 if name == 'peak_tick' else torch.float32
                 self.state[name] = torch.zeros(e_num * S, dtype=dtype, device=dev.device)
             self.state['q'] = torch.zeros(s_num * S, dtype=torch.float32, device=dev.device)
-            self.manning = None if self.shared_manning else torch.zeros(s_num * S, dtype=torch.float32,
-                                                                        device=dev.device)
+            # per scenario: Manning n by land class, from which the kernels rebuild each side's n
+            self.class_manning = None if self.shared_manning else torch.zeros(256 * S, dtype=torch.float32,
+                                                                              device=dev.device)
         bslot, bstart, n_bcells = boundary_cells(layout)
         self.static['_bslot'] = dev.array(bslot)
         self.static['_bcell_start'] = dev.array(bstart)
@@ -104,7 +105,8 @@ class BatchEngine:
         st = abi.BatchState()
         for name in ('depth', 'rain_time', 'capacity', 'max_depth', 'max_speed', 'peak_tick', 'q'):
             setattr(st, name, dev.ptr(self.state[name]))
-        st.manning = dev.ptr(self.manning) if self.manning is not None else None
+        st.manning = None
+        st.class_manning = dev.ptr(self.class_manning) if self.class_manning is not None else None
         st.bslot = dev.ptr(self.static['_bslot'])
         st.bcell_start = dev.ptr(self.static['_bcell_start'])
         st.n_bcells = n_bcells
@@ -125,7 +127,7 @@ class BatchEngine:
         if (layout.e_num, layout.s_num) != (self.layout.e_num, self.layout.s_num):
             raise ValueError('scenario mesh differs from the batch mesh')
         for name, values in self.layout.arrays.items():
-            if name == 'manning' and not self.shared_manning:
+            if name in ('manning', '_class_manning') and not self.shared_manning:
                 continue
             other = layout.arrays.get(name)
             if other is None or not np.array_equal(other, values):
@@ -157,8 +159,9 @@ class BatchEngine:
             self.owner[s] = v
             for name in SLOT_CELLS + STATE_SIDES:
                 v.write_state(name, state[name])
-            if self.manning is not None:
-                v._write_column(self.manning, np.ascontiguousarray(layout.arrays['manning'], dtype=np.float32))
+            if self.class_manning is not None:
+                v._write_column(self.class_manning,
+                                np.ascontiguousarray(layout.arrays['_class_manning'], dtype=np.float32))
             return v
         return factory
 

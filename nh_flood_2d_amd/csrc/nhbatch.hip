@@ -23,6 +23,11 @@
 
 #include "nhf_device.h"
 
+// kb_cfl loads the next round's side records while the current round's rows are in flight.
+#ifndef NHB_CFL_PREFETCH
+#define NHB_CFL_PREFETCH 1
+#endif
+
 struct BDev {
     nhf_mesh m;  // static mesh arrays (the per-scenario state pointers of nhf_mesh are unused)
     nhf_params p;
@@ -32,6 +37,9 @@ struct BDev {
     float *depth, *rain_time, *capacity, *max_depth, *max_speed, *q;
     uint64_t *peak_tick;
     const float *manning;  // [s_num][slot] or NULL: mesh manning shared by every slot
+    // [256 classes][slot] or NULL: Manning n per land class, side n rebuilt from side_class
+    const float *class_manning;
+    uint32_t *side_class;  // [s_num] class of the lower | higher cell << 16, NHB_NO_CLASS outside
     // boundary-cell list (static)
     const int32_t *bslot, *bcell_start;
     // scratch
@@ -60,6 +68,9 @@ struct Stage {
     float *rain;                // [slot][n_pixels]
 };
 enum { STAGE_BEGIN = 1, STAGE_COEFF = 2, STAGE_RAIN = 4, STAGE_RAINING = 8, STAGE_RAINING_ON = 16 };
+
+// Class half-word of the exterior cell in side_class: its n is zero.
+#define NHB_NO_CLASS 0xffffu
 
 // Index of entity e, slot s in a per-scenario array; `sh` is log2 of the slot count in scope.
 #define IX(e) ((((size_t)(e)) << sh) | (size_t)s)
@@ -263,9 +274,28 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_face_avg(BDev d) {
 // A side's loaded values for one slot.
 struct BFlux {
     int2 lh;
-    uint32_t meta;
-    float q, zl, zh, dl, dh, sdc, n;
+    uint32_t meta, cls;
+    float q, zl, zh, dl, dh, sdc, n, nl, nh;
 };
+
+// The side classes of the per-scenario Manning table: one word per side, built once.
+__global__ void kb_side_classes(BDev d) {
+    const nhf_mesh &m = d.m;
+    for (int si = blockIdx.x * blockDim.x + threadIdx.x; si < m.s_num; si += gridDim.x * blockDim.x) {
+        const int l = m.side_cells[2 * si], h = m.side_cells[2 * si + 1];
+        const uint32_t cl = l ? (uint32_t)meta_class(m.cell_meta[l]) : NHB_NO_CLASS;
+        const uint32_t ch = h ? (uint32_t)meta_class(m.cell_meta[h]) : NHB_NO_CLASS;
+        d.side_class[si] = cl | (ch << 16);
+    }
+}
+
+// Side n of one slot from its cells' class n (map_land_parameters, preprocess/hydraulics.py:63-93,
+// with the exterior's n zero): the F32 mean, or n + 0 on an exterior side.
+static __device__ __forceinline__ float class_side_n(uint32_t cls, float nl, float nh) {
+    const float both = nl + nh;
+    const bool one_sided = (cls & 0xffffu) == NHB_NO_CLASS || (cls >> 16) == NHB_NO_CLASS;
+    return one_sided ? both : both * 0.5f;
+}
 
 // The new discharge of side si for slot s (k_flux's arithmetic, solver_compact.py:439-486).
 static __device__ __forceinline__ float bflux_value(const BDev &d, int s, int sh, int si, float dt, const BFlux &f) {
@@ -306,8 +336,10 @@ static __device__ __forceinline__ float bflux_value(const BDev &d, int s, int sh
 
 // K consecutive sides per thread (per wavefront with 64 slots): every side's loads are issued
 // before any arithmetic, since a lone side is one short chain of dependent loads. The centre
-// distance and Manning n are loaded for every side (shared by the 64 slots, they cost little),
-// so a wet face waits for no third round (501 against 527 us).
+// distance and Manning n (or the cells' class n) are loaded for every side (shared by the 64
+// slots, they cost little), so a wet face waits for no third round (501 against 527 us). The
+// hanging-edge index stays in the wet branch: loaded with the records it measured slower (482
+// against 455 us on the 64-slot benchmark batch).
 template <bool W, int K>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
     const int sh = W ? 6 : d.shift;
@@ -326,6 +358,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
             f[k].lh = reinterpret_cast<const int2 *>(m.side_cells)[si];
             f[k].meta = m.side_meta[si];
             f[k].q = d.q[IX(si)];
+            if (d.class_manning) f[k].cls = d.side_class[si];
         }
     }
 #pragma unroll
@@ -336,9 +369,18 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
             f[k].dl = d.depth[IX(f[k].lh.x)];
             f[k].dh = d.depth[IX(f[k].lh.y)];
             f[k].sdc = m.sdc[first + k];
-            f[k].n = d.manning ? d.manning[IX(first + k)] : m.manning[first + k];
+            if (d.class_manning) {
+                const uint32_t cl = f[k].cls & 0xffffu, ch = f[k].cls >> 16;
+                f[k].nl = cl != NHB_NO_CLASS ? d.class_manning[IX(cl)] : 0.0f;
+                f[k].nh = ch != NHB_NO_CLASS ? d.class_manning[IX(ch)] : 0.0f;
+            } else {
+                f[k].n = d.manning ? d.manning[IX(first + k)] : m.manning[first + k];
+            }
         }
     }
+    if (d.class_manning)
+#pragma unroll
+        for (int k = 0; k < K; ++k) f[k].n = class_side_n(f[k].cls, f[k].nl, f[k].nh);
     float nq[K];
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -739,15 +781,22 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
         const float afa = d.p.afa, g = d.p.g;
         const size_t total = (size_t)(m.s_num - 1) << sh;
         const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
-        for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += U * stride) {
-            int si[U];
-            int2 lh[U];
+        // the cell pairs of a round are loaded during the round before (NHB_CFL_PREFETCH), so
+        // each round waits for one load latency instead of two
+        int si[U];
+        int2 lh[U];
+        auto records = [&](size_t t0, int *ids, int2 *cells) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const bool live = t + u * stride < total;
-                si[u] = live ? entity_of<W>(t + u * stride, sh) + 1 : 0;
-                lh[u] = live ? reinterpret_cast<const int2 *>(m.side_cells)[si[u]] : make_int2(0, 0);
+                const bool live = t0 + u * stride < total;
+                ids[u] = live ? entity_of<W>(t0 + u * stride, sh) + 1 : 0;
+                cells[u] = live ? reinterpret_cast<const int2 *>(m.side_cells)[ids[u]] : make_int2(0, 0);
             }
+        };
+        size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
+        if (NHB_CFL_PREFETCH) records(t, si, lh);
+        for (; t < total; t += U * stride) {
+            if (!NHB_CFL_PREFETCH) records(t, si, lh);
             float zl[U], zh[U], dl[U], dh[U], len[U], qs[U];
             uint32_t ref[U];
 #pragma unroll
@@ -762,6 +811,9 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
                     ref[u] = m.side_key[si[u]];
                 }
             }
+            int si_next[U];
+            int2 lh_next[U];
+            if (NHB_CFL_PREFETCH) records(t + U * stride, si_next, lh_next);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (lh[u].x != 0 && lh[u].y != 0) {
@@ -775,6 +827,12 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
                     }
                 }
             }
+            if (NHB_CFL_PREFETCH)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    si[u] = si_next[u];
+                    lh[u] = lh_next[u];
+                }
         }
     }
     for (int o = 32; o >= S; o >>= 1) {
@@ -992,7 +1050,7 @@ static int slot_shift(int slots) {
 
 struct BCarve {
     size_t ctl, dt, raining, coeff, values, level, rain, bdonor, face_qn, grad, key, idx, acc, bvals, bacc,
-        flush, total;
+        flush, side_class, total;
 };
 
 static BCarve bcarve(const nhf_mesh *m, const nhb_state *st, const nhf_params *p, int slots) {
@@ -1020,6 +1078,7 @@ static BCarve bcarve(const nhf_mesh *m, const nhb_state *st, const nhf_params *p
     c.bvals = off; off = align_b(off + sizeof(float) * 6 * (size_t)nb * S);
     c.bacc = off; off = align_b(off + sizeof(double2) * terms * S);
     c.flush = off; off = align_b(off + sizeof(double2) * terms);
+    c.side_class = off; off = align_b(off + (st->class_manning ? sizeof(uint32_t) * (size_t)m->s_num : 0));
     c.total = off;
     return c;
 }
@@ -1118,7 +1177,7 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
         return set_error_text("nhb_create: open_face is required when sita != 1");
     if (mesh->n_hang > 0 && (!mesh->edge_index || !mesh->x || !mesh->y || !mesh->grad_cell))
         return set_error_text("nhb_create: hanging reconstruction arrays missing");
-    if (const char *name = missing_mesh_array(mesh, false, state->manning == nullptr)) {
+    if (const char *name = missing_mesh_array(mesh, false, !state->manning && !state->class_manning)) {
         snprintf(nhf_error_text, sizeof nhf_error_text, "nhb_create: mesh array %s is NULL", name);
         return -2;
     }
@@ -1148,6 +1207,7 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     d.peak_tick = state->peak_tick;
     d.q = state->q;
     d.manning = state->manning;
+    d.class_manning = state->class_manning;
     d.bslot = state->bslot;
     d.bcell_start = state->bcell_start;
     d.ctl = (Ctl *)(ws + cv.ctl);
@@ -1166,6 +1226,7 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     d.bvals = (float *)(ws + cv.bvals);
     d.bacc = (double2 *)(ws + cv.bacc);
     d.flush = (double2 *)(ws + cv.flush);
+    d.side_class = d.class_manning ? (uint32_t *)(ws + cv.side_class) : nullptr;
     ctx->slots = slots;
     ctx->stream = (hipStream_t)stream;
     ctx->graph_enabled = 0;
@@ -1206,6 +1267,8 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     // every slot starts idle with a clean clock
     for (int s = 0; s < slots; ++s)
         hipLaunchKernelGGL(kb_reset, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, s, (uint64_t)0);
+    if (d.class_manning)
+        hipLaunchKernelGGL(kb_side_classes, dim3(b_blocks((size_t)mesh->s_num)), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d);
     if ((err = hipGetLastError()) != hipSuccess) {
         (void)hipHostFree(ctx->stage_host);
         (void)hipFree(ctx->stage_dev);

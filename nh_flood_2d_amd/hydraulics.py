@@ -57,14 +57,22 @@ def map_land_parameters(table: dict, land_types: np.ndarray, topo: np.ndarray):
     its two cells' n; on an exterior side the exterior's zero drops out and the interior cell's
     n is taken as it is.
     """
-    per_cell = class_tables(table, land_types)[:, np.asarray(land_types, dtype=np.intp)]
+    return land_parameters_of(class_tables(table, land_types), land_types, topo)
+
+
+def land_parameters_of(rows: np.ndarray, land_types: np.ndarray, topo: np.ndarray):
+    """map_land_parameters from the class rows of class_tables."""
+    per_cell = rows[:, np.asarray(land_types, dtype=np.intp)]
     per_cell[:, 0] = 0.
     cell_n, f0, fc, k = (np.ascontiguousarray(row) for row in per_cell)
-    ends = np.asarray(topo)[:, 1:]
+    return side_manning(cell_n, np.asarray(topo)[:, 1:]), f0, fc, k
+
+
+def side_manning(cell_n: np.ndarray, ends: np.ndarray) -> np.ndarray:
+    """F32 side n from its two cells' n (`ends`: [sides, 2] cell numbers, 0 the exterior, whose n
+    is zero): the mean, or on an exterior side the interior cell's n (n + 0)."""
     both = cell_n[ends[:, 0]] + cell_n[ends[:, 1]]
-    one_sided = np.any(ends == 0, axis=1)
-    side_n = np.where(one_sided, both, both * np.float32(.5)).astype(np.float32)
-    return side_n, f0, fc, k
+    return np.where(np.any(ends == 0, axis=1), both, both * np.float32(.5)).astype(np.float32)
 
 
 # ------------------------------------------------------------------------------ open faces

@@ -23,7 +23,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .hydraulics import hanging_geometry, map_land_parameters, open_face_fractions, side_centre_distance
+from .hydraulics import (class_tables, hanging_geometry, land_parameters_of, open_face_fractions,
+                         side_centre_distance)
 from .mesh import Mesh
 
 
@@ -122,9 +123,11 @@ def relayout(base: Layout, mesh: Mesh, land_parameters: dict, forcing: Forcing) 
             and base.n_pixels == forcing.n_pixels and base.b_num == len(forcing.kinds))
     if not same:
         raise ValueError('relayout needs the forcing geometry of the base layout')
-    side_n, f0, fc, hk = map_land_parameters(land_parameters, mesh.land, mesh.topo)
+    rows = class_tables(land_parameters, mesh.land)
+    side_n, f0, fc, hk = land_parameters_of(rows, mesh.land, mesh.topo)
     arrays = dict(base.arrays)
     arrays['manning'] = side_n[base.side_perm]
+    arrays['_class_manning'] = rows[0].copy()
     arrays['class_horton'] = horton_table(mesh.land.astype(np.int64), f0, fc, hk).ravel()
     layout = copy.copy(base)
     layout.arrays = arrays
@@ -190,7 +193,8 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
     e_num, s_num = mesh.e_num, mesh.s_num
     topo = mesh.topo
     _, widths, heights, area = mesh.geometry()
-    side_n, f0, fc, hk = map_land_parameters(land_parameters, mesh.land, topo)
+    rows = class_tables(land_parameters, mesh.land)
+    side_n, f0, fc, hk = land_parameters_of(rows, mesh.land, topo)
     sdc = side_centre_distance(mesh.x, mesh.y, topo)
 
     cperm = cell_order(mesh, order)
@@ -251,7 +255,7 @@ def build_layout(mesh: Mesh, land_parameters: dict, forcing: Forcing, theta: flo
         csr_start=start.astype(np.int32), csr_count=packed.astype(np.uint32), isl=isl, isl_cell=isl_cell,
         side_cells=side_cells.ravel(), side_meta=meta,
         length=mesh.length[sperm].astype(np.float32), sdc=sdc[sperm],
-        manning=side_n[sperm], side_key=sperm.astype(np.uint32),
+        manning=side_n[sperm], side_key=sperm.astype(np.uint32), _class_manning=rows[0].copy(),
         kind=(forcing.kinds.astype(np.int32) if len(forcing.kinds) else np.zeros(1, np.int32)),
     )
     arrays['cell_quad'] = cell_records(counts, start, packed, isl, arrays['length'], arrays['size'],

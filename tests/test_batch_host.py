@@ -82,3 +82,31 @@ def test_lane_count(count, lanes):
 def test_lane_count_limit():
     with pytest.raises(ValueError):
         lanes_for(65)
+
+
+def test_class_manning_rebuilds_every_side_n(variants):
+    """A batch keeps one Manning n per land class and scenario; its flux kernel rebuilds each
+    side's n from the classes of the side's two cells (the class bits of cell_meta), with the
+    exterior's n zero. That must be map_land_parameters' side n in layout order, bit for bit, on
+    every side: here for scenario land tables on a one-class mesh and on a four-class 2:1 mesh
+    whose class codes include 0 (so the exterior is told apart by its cell, not its class)."""
+    from nh_flood_2d_amd.hydraulics import side_manning
+    multi = synth.small_multires(2, discharge=1.0)
+    cases = list(variants[:2]) + synth.scenario_variants(multi, 3, seed=6)
+    for sc in cases:
+        layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, synth.scenario_forcing(sc)),
+                              sc.sita)
+        table = layout.arrays['_class_manning']
+        assert table.dtype == np.float32 and table.shape == (256,)
+        cell_n = table[layout.arrays['cell_meta'] & 255]
+        cell_n[0] = 0.
+        ends = layout.arrays['side_cells'].reshape(-1, 2)
+        assert np.any(ends[1:] == 0, axis=1).any()
+        assert side_manning(cell_n, ends).tobytes() == layout.arrays['manning'].tobytes(), sc.name
+    classes = np.unique(multi.mesh.land[1:])
+    assert 0 in classes and len(classes) == 4
+    # scenarios on one mesh differ only in the table
+    layouts = [build_layout(v.mesh, v.land_parameters, cell_forcing_arrays(v.mesh, synth.scenario_forcing(v)), v.sita)
+               for v in cases[2:]]
+    assert not np.array_equal(layouts[0].arrays['_class_manning'], layouts[1].arrays['_class_manning'])
+    assert np.array_equal(layouts[0].arrays['cell_meta'], layouts[1].arrays['cell_meta'])

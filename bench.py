@@ -335,6 +335,71 @@ def cpu_baseline(sc, forcing, developed: dict, t_start: float, seconds_budget: f
                 single_thread_sample=f'{steps1} steps, {wall1:.1f} s wall, 1 thread')
 
 
+def fp64_error(sc, forcing, developed: dict, t_start: float, device, window: float = 10.0) -> dict:
+    """Error of the F32 HIP step against an FP64 run of the same passes (the C oracle built with
+    double arithmetic, as the checker), over `window` simulated seconds from the developed state,
+    adaptive dt on each side and the forcing set at each batch's start: max |dh|, max |dq|, every
+    water-budget term and the stored volume, relative to the volume scale (storage at the start
+    plus the window's rain). Beside it, the F32 step against itself started from depths one ulp
+    deeper: the flow's own sensitivity over the same window, which bounds what any arithmetic can
+    be held to from this state."""
+    import oracle
+    from nh_flood_2d_amd import abi
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state
+    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, forcing), sc.sita)
+    params = make_params(sc.afa, sc.sita, sc.min_h, 32)
+    now, target = seconds_to_ticks(t_start), seconds_to_ticks(t_start + window)
+    bumped = {k: v.copy() for k, v in developed.items()}
+    wet = bumped['depth'] > 0
+    bumped['depth'][wet] = np.nextafter(bumped['depth'][wet], np.float32(1))
+
+    def run(dev, start):
+        st = initial_state(sc.mesh, layout)
+        st.update({k: v.copy() for k, v in start.items()})
+        e = Engine(layout, params, st, device=dev)
+        e.reset(now)
+        _set_forcing(e, forcing, t_start)
+        e.compute_cfl()
+        t, steps = now, 0
+        while t < target:
+            _set_forcing(e, forcing, t / 2 ** 40)
+            e.begin(target, t)
+            rows = e.run_batch(32)
+            done = rows[rows[:, 3] == abi.COMPLETED]
+            if not len(done):
+                raise RuntimeError(f'no step completed: status {rows[0, 3]}')
+            t = (int(done[-1, 1]) << 32) | int(done[-1, 2])
+            steps += len(done)
+        out = dict(depth=e.read_state('depth').astype(np.float64), q=e.read_state('q').astype(np.float64),
+                   budget=e.flush_budget().sum(axis=1), steps=steps)
+        e.close()
+        return out
+    f32, f64, ulp = run(device, developed), run(oracle.HostDevice('f64'), developed), run(device, bumped)
+    _, _, _, area = sc.mesh.geometry()
+    rain = forcing.rain
+    scale = float(np.dot(developed['depth'].astype(np.float64), area)) + float(
+        np.dot(rain.rates(rain.index_at(t_start))[rain.pixel_index][1:].astype(np.float64), area[1:])) * window
+    terms = np.abs(f32['budget'] - f64['budget']) / scale
+    names = ['infiltration', 'floor_nonlevel', 'floor_level', 'pipe']
+    for b in range(layout.b_num):
+        names += [f'level{b}_{c}' for c in ('storage', 'faces', 'pipe', 'rain', 'infiltration')]
+    names += [f'discharge{b}' for b in range(layout.b_num)]
+    worst = int(np.argmax(terms))
+    volume = abs(float(np.dot(f32['depth'] - f64['depth'], area))) / scale
+    return dict(window_s=window, start_s=t_start, steps_f32=f32['steps'], steps_fp64=f64['steps'],
+                max_abs_dh_m=float(np.max(np.abs(f32['depth'] - f64['depth']))),
+                max_abs_dq_m2_s=float(np.max(np.abs(f32['q'] - f64['q']))),
+                budget_terms_rel=dict(zip(names, (float(x) for x in terms))), worst_budget_term=names[worst],
+                worst_budget_rel=float(terms[worst]), stored_volume_rel=volume, volume_scale_m3=scale,
+                one_ulp_max_abs_dh_m=float(np.max(np.abs(f32['depth'] - ulp['depth']))),
+                one_ulp_max_abs_dq_m2_s=float(np.max(np.abs(f32['q'] - ulp['q']))),
+                note='F32 HIP against the FP64 C oracle from the developed state; one_ulp_* is the F32 step '
+                     'against itself from depths one ulp deeper (the flow\'s sensitivity over the window)')
+
+
 def _set_forcing(eng, forcing, t: float):
     c = np.zeros((max(1, len(forcing.boundaries)), 2), dtype=np.float32)
     for bi, b in enumerate(forcing.boundaries):
@@ -385,6 +450,38 @@ def _run_protocol(engines, run_batch, steps: int, horizon: int, now: list, batch
 # step of synchronized device-graph wall time excluding the first two batches, on an Apple M1 Max
 # with Taichi 1.7.4 Metal (docs/gate-validation/metal-performance-v2.json:92-95).
 REFERENCE_SMALL_MS = 0.212545
+
+
+def flood_map_bench(sc, developed: dict, device, repeats: int = 20) -> dict:
+    """The flood-map rasteriser on the benchmark mesh's developed state: every cell's masked
+    depth written to the pixels of the mesh's 2 m grid (one 2400 x 2400 tile), timed with HIP
+    events on the rasteriser's stream over `repeats` back-to-back passes. Algorithmic bytes per
+    pass: 20 B per cell (centre, half sizes, value) read and 4 B per pixel written."""
+    from nh_flood_2d_amd import maps
+    meta = maps.area_meta(sc.mesh)
+    grid = meta.grid()
+    raster = maps.Rasterizer(meta, device)
+    image = raster.raster(maps.masked_depth(developed['depth'][1:], sc.min_h, maps.NODATA), grid)
+    painted = int(np.sum((2 * meta.half_w / meta.hr) * (2 * meta.half_h / meta.vr)))
+    torch = device.torch
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    raster.enqueue(grid)
+    start.record(device.stream)
+    for _ in range(repeats):
+        raster.enqueue(grid)
+    end.record(device.stream)
+    end.synchronize()
+    us = start.elapsed_time(end) * 1e3 / repeats
+    cells = len(meta.x)
+    nbytes = 20 * cells + 4 * painted
+    gbs = nbytes / us / 1e3
+    log(f'flood map: {cells} cells on {grid.width} x {grid.height} px ({painted} painted), {us:.1f} us per map, '
+        f'{gbs:.0f} GB/s of its bytes')
+    return dict(workload=f'masked depth of the developed benchmark state on its {float(meta.hr):g} m grid',
+                cells=cells, width=grid.width, height=grid.height, painted_pixels=painted,
+                wet_pixels=int(np.count_nonzero(image != maps.NODATA)), us_per_map=us,
+                cells_per_s=cells / us * 1e6, pixels_per_s=painted / us * 1e6, algorithmic_bytes=nbytes,
+                achieved_gbs=gbs, frac_of_hbm_peak=gbs / 8000., launches=repeats)
 
 
 def small_mesh_bench(args, device) -> dict:
@@ -756,6 +853,8 @@ def main():
     ap.add_argument('--batch-blocks', type=int, default=790, help='10 m blocks per side of the batch mesh')
     ap.add_argument('--no-small-batch', dest='small_batch', action='store_false',
                     help='skip the batch on the 256 x 256 grid of the CPU-runnable configuration')
+    ap.add_argument('--no-flood-map', dest='flood_map', action='store_false',
+                    help='skip the flood-map rasteriser leg')
     ap.add_argument('--no-small-mesh', dest='small_mesh', action='store_false',
                     help='skip the time per step on a mesh of the size of the reference\'s published timing')
     ap.add_argument('--uneven-meshes', type=int, default=32, help='meshes of the uneven-size batch (0: skip)')
@@ -960,6 +1059,8 @@ def main():
     if args.calibrate_copy:
         result['copy_calibration_bytes'] = 2 * src.numel() * 4
     del src, dst
+    if args.flood_map:
+        result['flood_map'] = flood_map_bench(sc, developed, device)
     if args.small_mesh:
         result['small_mesh'] = small_mesh_bench(args, device)
     if args.batch_slots:
@@ -978,6 +1079,11 @@ def main():
         del eng
         torch.cuda.synchronize()
         result['cpu_baseline'] = cpu_baseline(sc, forcing, developed, t_start, args.cpu_seconds)
+        result['precision'] = fp64_error(sc, forcing, developed, t_start, device)
+        p = result['precision']
+        log(f"precision over {p['window_s']:.0f} s from the developed state: max|dh| {p['max_abs_dh_m']:.3g} m, "
+            f"max|dq| {p['max_abs_dq_m2_s']:.3g} m2/s, worst budget term {p['worst_budget_rel']:.3g}, stored volume "
+            f"{p['stored_volume_rel']:.3g} against FP64; one ulp deeper: max|dh| {p['one_ulp_max_abs_dh_m']:.3g} m")
     print(json.dumps(result), flush=True)
 
 

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 10
+#define NHF_ABI_VERSION 11
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -259,6 +259,18 @@ int32_t nhf_kernel_times(nhf_ctx *ctx, double *ms, int64_t *count, int32_t n, in
  * on the same inputs except the depth pass, which updates the state in place, so the context's
  * state is not meaningful afterwards. */
 int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us);
+
+/* Flood-map rasteriser (replaces rasterize_tile_kernel, output/flood_map.py:180-222, and
+ * rasterize_wse, :688-721, whose masking the caller applies to `value` first). Cell i is the
+ * rectangle x[i] +- half_w[i], y[i] +- half_h[i]; it writes value[i] to every pixel of the tile
+ * whose centre it holds ([min, max) along x, (min, max] along y), where the tile is rows x cols
+ * pixels of hr x vr with its top-left corner at (tile_min_x, tile_max_y) and row r starts at
+ * tile + r * ld. F32 arithmetic in the reference's order; pixels no cell holds are left as they
+ * are (the caller fills the tile with its nodata value first). All arrays are DEVICE pointers;
+ * the launch is asynchronous on `stream`. */
+int32_t nhf_rasterize(const float *x, const float *y, const float *half_w, const float *half_h, const float *value,
+                      int32_t n, float tile_min_x, float tile_max_y, float hr, float vr, int32_t rows, int32_t cols,
+                      float *tile, int64_t ld, void *stream);
 
 /* Test hook with no reference entry point: evaluate StepClock.choose (clock.py:94-167) for n
  * independent cases on the device. Inputs and outputs are host arrays. out_words[i*4 + 0..3]

@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -106,6 +106,8 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_tuning_range': (c_int32, [vp, vp]),
         'nhf_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
         'nhf_clock_probe': (c_int32, [vp, vp, vp, vp, c_float, c_int32, vp, vp, vp]),
+        'nhf_rasterize': (c_int32, [vp, vp, vp, vp, vp, c_int32, c_float, c_float, c_float, c_float, c_int32,
+                                    c_int32, vp, ctypes.c_int64, vp]),
     }
     if hasattr(lib, 'nhf_run_static'):
         sig['nhf_run_static'] = (c_int32, [POINTER(MeshStruct), POINTER(Params), c_uint64, c_uint64, c_uint64,
@@ -171,7 +173,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhb_tuning_range',
             'nhu_workspace_bytes', 'nhu_create', 'nhu_reset', 'nhu_set_forcing', 'nhu_begin', 'nhu_run_batch',
             'nhu_launch_batch', 'nhu_wait_batch', 'nhu_compute_cfl', 'nhu_flush_budget', 'nhu_limit_capture',
-            'nhu_cfl_state']
+            'nhu_cfl_state', 'nhf_rasterize']
 
 _hip_lib = None
 

@@ -115,6 +115,14 @@ class DomainConfig(InputModel):
     def frame_dir(self) -> str:
         return self._ensure(Path(self.output_dir) / 'frames')
 
+    @property
+    def flood_map_dir(self) -> str:
+        return self._ensure(Path(self.domain_dir) / 'flood_maps')
+
+    @property
+    def max_inundation_dir(self) -> str:
+        return self._ensure(Path(self.domain_dir) / 'max_inundation')
+
 
 class BoundaryConfig(InputModel):
     name: str = Field(min_length=1)

@@ -282,3 +282,56 @@ def write_manifest(path, manifest):
     temporary = path.with_name(f'.{path.name}.{os.getpid()}.tmp')
     temporary.write_text(json.dumps(manifest, ensure_ascii=False, indent=2) + '\n')
     os.replace(temporary, path)
+
+
+def read_manifest(path) -> dict:
+    return json.loads(Path(path).read_text())
+
+
+class BedTimeline:
+    """The bed a run had at each output time, for water levels from depth frames (the reference's
+    BedTimeline, core/output.py:299-343): ``bed.fdb`` under the output root, then the manifest's
+    ``bed_events`` ({elapsed_seconds, cell_ids, new_bed}) applied in time order up to the asked
+    time, then every gate of its ``gate_control`` report set between its closed bed and the open
+    elevation by the fraction of its opening time elapsed since its command (F32, as the device
+    moves it). Times must not decrease from one call to the next."""
+
+    def __init__(self, output_root, manifest: dict | None = None, expected_count=None):
+        root = Path(output_root)
+        if manifest is None:
+            manifest = read_manifest(root.parent / 'manifest.json')
+        self.bed = read_bed(root / 'bed.fdb', expected_count).copy()
+        self.events = sorted(manifest.get('bed_events', []), key=lambda e: e['elapsed_seconds'])
+        self.gates = manifest.get('gate_control') or {}
+        self.applied = 0
+        self.last = float('-inf')
+
+    def _gate_fraction(self, gate: dict, seconds: float) -> np.float32:
+        command = gate.get('command_seconds')
+        if command is None or seconds < command:
+            return np.float32(0.)
+        hi, lo = gate['command_clock_ticks']
+        tick = round((seconds + self.gates['clock_origin_before_start_s']) * 2 ** 40)
+        age = max(tick - ((int(hi) << 32) | int(lo)), 0)
+        age_s = np.float32(age >> 32) / np.float32(256.) + np.float32(age & 0xffffffff) * np.float32(2. ** -40)
+        return np.minimum(age_s / np.float32(self.gates['opening_seconds']), np.float32(1.))
+
+    def at(self, seconds: float) -> np.ndarray:
+        if seconds < self.last:
+            raise ValueError('BedTimeline needs frame times in nondecreasing order')
+        self.last = seconds
+        while self.applied < len(self.events) and self.events[self.applied]['elapsed_seconds'] <= seconds:
+            event = self.events[self.applied]
+            cells = _uint_array(event['cell_ids'], 'bed event cell_ids')
+            values = _float_array(event['new_bed'], 'bed event new_bed')
+            if len(cells) != len(values) or np.any(cells >= len(self.bed)):
+                raise ValueError('bed event has invalid cell ids or values')
+            self.bed[cells] = values
+            self.applied += 1
+        open_bed = np.float32(self.gates.get('open_elevation', 0.))
+        for gate in self.gates.get('gates', []):
+            closed = np.asarray(gate['closed_elevation'], dtype=np.float32)
+            fraction = self._gate_fraction(gate, seconds)
+            cells = np.asarray(gate['cell_ids'], dtype=np.int64)
+            self.bed[cells] = open_bed if fraction >= 1. else closed + (open_bed - closed) * fraction
+        return self.bed

@@ -1187,3 +1187,39 @@ int32_t nhf_oracle_math(const float *x, const float *y, int32_t n, float *pow_ou
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ flood-map raster */
+
+/* rasterize_tile_kernel (output/flood_map.py:180-222) over host arrays, one cell after another in
+ * cell order: the F32 bounding-box test against the tile, then the pixel range
+ *   columns ceil((min_x - left) / hr - 0.5) .. ceil((max_x - left) / hr - 0.5) - 1,
+ *   rows    ceil((top - max_y) / vr - 0.5) .. ceil((top - min_y) / vr - 0.5) - 1,
+ * each operation rounded to F32 as written, clamped to the tile. Always F32, whatever REAL is:
+ * the maps are F32 in the reference. */
+int32_t nhf_rasterize(const float *x, const float *y, const float *half_w, const float *half_h, const float *value,
+                      int32_t n, float tile_min_x, float tile_max_y, float hr, float vr, int32_t rows, int32_t cols,
+                      float *tile, int64_t ld, void *stream) {
+    (void)stream;
+    if (n < 0 || rows < 0 || cols < 0 || ld < cols) return fail("nhf_rasterize: negative size or ld < cols");
+    if (!(hr > 0.0f) || !(vr > 0.0f)) return fail("nhf_rasterize: pixel sizes must be positive");
+    if (n == 0 || rows == 0 || cols == 0) return 0;
+    const float tile_max_x = tile_min_x + (float)cols * hr;
+    const float tile_min_y = tile_max_y - (float)rows * vr;
+    for (int32_t i = 0; i < n; ++i) {
+        const float e_min_x = x[i] - half_w[i], e_max_x = x[i] + half_w[i];
+        const float e_min_y = y[i] - half_h[i], e_max_y = y[i] + half_h[i];
+        if (!(e_max_x >= tile_min_x && e_min_x < tile_max_x && e_max_y >= tile_min_y && e_min_y < tile_max_y))
+            continue;
+        const float cl = (e_min_x - tile_min_x) / hr, ch = (e_max_x - tile_min_x) / hr;
+        const float rl = (tile_max_y - e_max_y) / vr, rh = (tile_max_y - e_min_y) / vr;
+        int32_t c0 = (int32_t)ceilf(cl - 0.5f), c1 = (int32_t)(ceilf(ch - 0.5f) - 1.0f);
+        int32_t r0 = (int32_t)ceilf(rl - 0.5f), r1 = (int32_t)(ceilf(rh - 0.5f) - 1.0f);
+        if (c0 < 0) c0 = 0;
+        if (c1 > cols - 1) c1 = cols - 1;
+        if (r0 < 0) r0 = 0;
+        if (r1 > rows - 1) r1 = rows - 1;
+        for (int32_t r = r0; r <= r1; ++r)
+            for (int32_t c = c0; c <= c1; ++c) tile[(int64_t)r * ld + c] = value[i];
+    }
+    return 0;
+}

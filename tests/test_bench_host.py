@@ -39,3 +39,23 @@ def test_pmc_summary_scales_counters_by_the_copy_calibration(tmp_path):
     assert f'fetch_bytes {2 * 2000 * 1024} write_bytes {100 * 1024} ' in flux      # mean of 1000 and 3000 KiB
     depth = next(ln for ln in lines if ln.startswith('kernel batch_depth '))
     assert f'hbm_bytes {2 * 500 * 1024 + 50 * 1024}' in depth
+
+
+def test_fp64_error_reports_both_runs_and_the_one_ulp_sensitivity():
+    """The precision object of the bench line, with the F32 oracle standing in for the HIP engine
+    (the same passes): over 10 s of a developed 2:1 mesh with tide and river, F32 against FP64
+    differs by rounding only, and every budget term is named."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    import oracle
+    from nh_flood_2d_amd import synth
+    sc = synth.small_multires(2, discharge=1.0)
+    forcing = synth.scenario_forcing(sc)
+    sim = synth.simulation(sc, engine_factory=oracle.engine_factory('f32'), end_time=600., yield_step=600.)
+    sim.run()
+    p = bench.fp64_error(sc, forcing, sim.state(), 600., oracle.HostDevice('f32'), window=10.)
+    assert p['steps_f32'] > 10 and abs(p['steps_f32'] - p['steps_fp64']) <= 1
+    assert 0 < p['max_abs_dh_m'] < 1e-5 and p['max_abs_dq_m2_s'] < 1e-5
+    assert set(p['budget_terms_rel']) >= {'infiltration', 'floor_nonlevel', 'level0_storage', 'discharge0'}
+    assert p['worst_budget_rel'] < 1e-7 and p['stored_volume_rel'] < 1e-6
+    assert p['one_ulp_max_abs_dh_m'] > 0

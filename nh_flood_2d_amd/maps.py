@@ -17,6 +17,7 @@ pixels and georeferencing.
 * :func:`generate_f1_score_map` scores two binary extents (:464-578).
 * :func:`spatial_mae` and :func:`plot_spatial_mae_curve` give the water-level error between two
   runs on their shared grid (:580-832).
+* :func:`flood_video_frames` and :func:`generate_flood_video` animate the maps (:834-955).
 
 GeoTIFFs go through this package's own writer (uncompressed strips): the pixel values and
 georeferencing are the reference's, the LZW tiling and the F1 map's embedded colour table are not.
@@ -400,3 +401,87 @@ def plot_spatial_mae_curve(cfg_ref, cfg_cmp, force_cfg=None, min_depth: float = 
         plt.show()
     plt.close(fig)
     return errors
+
+
+SHALLOW = np.array([173, 216, 230], dtype=_F32)     # light blue
+DEEP = np.array([0, 0, 139], dtype=_F32)            # dark blue
+
+
+def colour_depths(data: np.ndarray, nodata, low: float, high: float) -> np.ndarray:
+    """[h, w, 3] uint8: finite pixels other than `nodata` on the light-to-dark blue ramp over
+    [low, high] (F32, truncated to 8 bits), the rest black."""
+    frame = np.zeros(data.shape + (3,), dtype=np.uint8)
+    mask = ((data != nodata) & np.isfinite(data)) if nodata is not None else np.isfinite(data)
+    if mask.any():
+        t = np.clip((data[mask] - low) / (high - low), 0.0, 1.0)
+        frame[mask] = ((1.0 - t[:, None]) * SHALLOW + t[:, None] * DEEP).astype(np.uint8)
+    return frame
+
+
+def flood_video_frames(cfg):
+    """(label, RGB frame) of every flood map in index order (generate_flood_video,
+    flood_map.py:834-955): one depth scale over all maps (their minimum to maximum, 0-1 m when
+    flat), each wet pixel coloured linearly from light blue (shallow) to dark blue (deep) and
+    truncated to 8 bits, dry and nodata pixels black, the map's wall time drawn top right."""
+    from datetime import datetime as wall
+    from PIL import Image, ImageDraw, ImageFont
+    paths = sorted(Path(cfg.flood_map_dir).glob('flood_map_*.tif'), key=lambda p: int(p.stem.split('_')[2]))
+    if not paths:
+        return
+    first = read_geotiff(paths[0])
+    nodata = first.nodata
+    height, width = first.bands.shape[1:]
+
+    def valid(data):
+        return (data != nodata) & np.isfinite(data) if nodata is not None else np.isfinite(data)
+    low, high = np.inf, -np.inf
+    for path in paths:
+        data = read_geotiff(path).bands[0]
+        mask = valid(data)
+        if mask.any():
+            low, high = min(low, float(data[mask].min())), max(high, float(data[mask].max()))
+    if low >= high:
+        low, high = 0.0, 1.0
+    size = max(14, int(width * 0.015))
+    try:
+        font = ImageFont.truetype('arial.ttf', size)
+    except OSError:
+        font = ImageFont.load_default()
+    pad = max(6, size // 3)
+    for path in paths:
+        frame = colour_depths(read_geotiff(path).bands[0], nodata, low, high)
+        label = wall.strptime(path.stem.split('_')[-1], '%Y%m%d-%H%M%S').strftime('%Y-%m-%d  %H:%M:%S')
+        image = Image.fromarray(frame)
+        draw = ImageDraw.Draw(image)
+        box = draw.textbbox((0, 0), label, font=font)
+        x, y = width - (box[2] - box[0]) - 2 * pad, pad
+        draw.rectangle([x - pad, y - pad, x + (box[2] - box[0]) + pad, y + (box[3] - box[1]) + pad], fill=(0, 0, 0))
+        draw.text((x, y), label, font=font, fill=(255, 255, 255))
+        yield label, np.array(image)
+
+
+def generate_flood_video(cfg, output_path, interval_second: float = 0.2) -> Path | None:
+    """The flood maps as an animation, one map per `interval_second`: an animated GIF (Pillow)
+    for a .gif path, an H.264 MP4 for .mp4 when imageio with ffmpeg is installed. Returns the
+    path, or None when there are no maps."""
+    from PIL import Image
+    frames = flood_video_frames(cfg)
+    first = next(frames, None)
+    if first is None:
+        return None
+    out = Path(output_path)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    if out.suffix.lower() == '.mp4':
+        try:
+            import imageio
+        except ImportError as exc:
+            raise RuntimeError('an MP4 needs imageio with ffmpeg; give a .gif path instead') from exc
+        with imageio.get_writer(str(out), fps=1.0 / interval_second, codec='libx264', quality=8,
+                                macro_block_size=1) as writer:
+            writer.append_data(first[1])
+            for _, frame in frames:
+                writer.append_data(frame)
+        return out
+    images = [Image.fromarray(first[1])] + [Image.fromarray(frame) for _, frame in frames]
+    images[0].save(out, save_all=True, append_images=images[1:], duration=int(round(interval_second * 1000)), loop=0)
+    return out

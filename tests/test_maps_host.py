@@ -202,6 +202,26 @@ def test_flood_maps_of_a_run(tmp_path, oracle_solver):
         expected = _located(mesh, meta.grid(), maps.masked_depth(depth, dc.min_h, maps.NODATA))
         np.testing.assert_array_equal(image.bands[0], expected)
     assert np.all(read_geotiff(paths[-1]).bands[0] > 0)       # 30 mm on every cell by the end
+    frames = list(maps.flood_video_frames(dc))
+    assert [label for label, _ in frames] == [(cases.START + timedelta(seconds=s)).strftime('%Y-%m-%d  %H:%M:%S')
+                                             for s in (0, 900, 1800)]
+    assert all(frame.shape == (4, 4, 3) and frame.dtype == np.uint8 for _, frame in frames)
+    depths = [read_geotiff(p).bands[0] for p in paths]
+    wet = np.concatenate([d[d != maps.NODATA] for d in depths])
+    low, high = float(wet.min()), float(wet.max())
+    colours = [maps.colour_depths(d, np.float32(maps.NODATA), low, high) for d in depths]
+    assert colours[-1][depths[-1] == wet.max()].tolist()[0] == [0, 0, 139]        # the deepest: dark blue
+    assert colours[1][depths[1] == wet.min()].tolist()[0] == [173, 216, 230]      # the shallowest: light blue
+    assert not colours[0].any()                                                   # all dry at 0 s: black
+    middle = (low + high) / 2
+    assert maps.colour_depths(np.array([[middle]], np.float32), None, low, high)[0, 0].tolist() == [86, 108, 184]
+    from PIL import Image
+    gif = maps.generate_flood_video(dc, tmp_path / 'video' / 'flood.gif')
+    assert Image.open(gif).n_frames == 3
+    import importlib.util
+    if importlib.util.find_spec('imageio') is None:
+        with pytest.raises(RuntimeError, match='imageio'):
+            maps.generate_flood_video(dc, tmp_path / 'video' / 'flood.mp4')
     extent = maps.generate_max_inundation_extent_map(dc, min_depth=.02, is_absolute=True, device=dev)
     binary = read_geotiff(extent)
     maximum = read_extrema(f'{dc.domain_dir}/output/extrema.fdb').max_depth[1:]

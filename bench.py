@@ -808,15 +808,31 @@ def served_bench(args) -> dict:
             def build(factory, sc=sc, layout=layout, i=i):
                 return synth.simulation(sc, engine_factory=factory, yield_step=300., base_layout=layout,
                                         batch_size=64, outputs=stream.sink(i))
-            out.append(Request(i, build, arr, sc.mesh.e_num - 1, footprint_bytes(layout)))
+            out.append(Request(i, build, arr, sc.mesh.e_num - 1, footprint_bytes(layout), layout=layout,
+                               params=(sc.afa, sc.sita, sc.min_h)))
         return out
 
     stream = FrameStream(None, every_output=True)
-    served = FloodServer(batch_size=64).serve(requests(stream), realtime=True)
+    server = FloodServer(batch_size=64)
+    served = server.serve(requests(stream), realtime=True)
     stream.close()
     s = served.summary()
+    import torch
+    device_bytes = torch.cuda.get_device_properties(0).total_memory
     log(f"served: {s['requests']} solves, p50 {s['p50_latency_s']:.2f} s, p99 {s['p99_latency_s']:.2f} s, "
-        f"{s['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s, peak live {s['peak_live']}")
+        f"{s['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s, peak live {s['peak_live']}, peak footprint "
+        f"{s['peak_device_bytes'] / 1e9:.2f} GB of a {server.budget / 1e9:.0f} GB admission budget "
+        f"({device_bytes / 1e9:.0f} GB device)")
+    u = None
+    if args.serve_union_window > 0:
+        stream = FrameStream(None, every_output=True)
+        grouped = FloodServer(batch_size=64, union_window=args.serve_union_window,
+                              union_parts=args.serve_union_parts).serve(requests(stream), realtime=True)
+        stream.close()
+        u = grouped.summary()
+        log(f"served through unions (window {args.serve_union_window} s, up to {args.serve_union_parts} solves): "
+            f"{u['unions']} unions, p50 {u['p50_latency_s']:.2f} s, p99 {u['p99_latency_s']:.2f} s, "
+            f"{u['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s")
     stream = FrameStream(None, every_output=True)
     alone = FloodServer(batch_size=64, max_live=1).serve(requests(stream), realtime=False)
     stream.close()
@@ -830,6 +846,10 @@ def served_bench(args) -> dict:
         'value': s['cell_updates_per_s'], 'unit': 'cell-updates/s', 'p50_latency_s': s['p50_latency_s'],
         'p99_latency_s': s['p99_latency_s'], 'plain_value': a['cell_updates_per_s'],
         'served_vs_plain': s['cell_updates_per_s'] / a['cell_updates_per_s'], 'served': s, 'one_at_a_time': a,
+        'admission_budget_bytes': server.budget, 'device_bytes': device_bytes,
+        'through_unions': None if u is None else dict(
+            value=u['cell_updates_per_s'], vs_plain=u['cell_updates_per_s'] / a['cell_updates_per_s'],
+            window_s=args.serve_union_window, max_parts=args.serve_union_parts, summary=u),
     }
 
 
@@ -867,6 +887,9 @@ def main():
     ap.add_argument('--serve-rate', type=float, default=2.0)
     ap.add_argument('--serve-min-cells', type=int, default=100_000)
     ap.add_argument('--serve-max-cells', type=int, default=8_000_000)
+    ap.add_argument('--serve-union-window', type=float, default=0.0,
+                    help='also serve the load through unions: seconds an arrival may wait for others (0: skip)')
+    ap.add_argument('--serve-union-parts', type=int, default=8)
     ap.add_argument('--serve-min-h', type=float, default=1.0)
     ap.add_argument('--serve-max-h', type=float, default=6.0)
     ap.add_argument('--trace-summary', nargs=2, metavar=('KERNEL_TRACE_CSV', 'BENCH_JSON'),

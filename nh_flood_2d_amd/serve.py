@@ -45,6 +45,8 @@ class Request:
     cells: int
     footprint: int
     group: object = None       # requests with the same group share a BatchEngine (same mesh)
+    layout: object = None      # the solve's layout and step parameters (afa, sita, min_h): with
+    params: tuple | None = None  # both, the request may be served as a part of a union
 
 
 @dataclass
@@ -58,6 +60,7 @@ class ServeReport:
     peak_live: int = 0
     peak_bytes: int = 0
     batches: int = 0
+    unions: int = 0            # union engines formed (each holding two or more solves)
 
     def percentile(self, p: float) -> float:
         values = np.array(sorted(self.latency.values()))
@@ -69,7 +72,8 @@ class ServeReport:
                     mean_queue_s=float(np.mean(list(self.queued.values()))) if self.queued else 0.,
                     cell_updates=self.cell_updates, wall_s=self.wall,
                     cell_updates_per_s=self.cell_updates / self.wall if self.wall else 0.,
-                    peak_live=self.peak_live, peak_device_bytes=self.peak_bytes, loop_iterations=self.batches)
+                    peak_live=self.peak_live, peak_device_bytes=self.peak_bytes, loop_iterations=self.batches,
+                    unions=self.unions)
 
 
 class FrameStream:
@@ -115,11 +119,18 @@ class FrameStream:
 
 
 class FloodServer:
-    """Continuous batching of flood solves on one GPU, admitted by device footprint."""
+    """Continuous batching of flood solves on one GPU, admitted by device footprint.
+
+    With `union_window` > 0, requests that carry their layout and step parameters are admitted
+    in groups: arrivals are held until the oldest has waited `union_window` seconds, `union_parts`
+    are waiting, or no more arrivals are due, and the group (equal parameters, within the budget)
+    becomes one UnionEngine on its own stream: one launch sequence steps every solve of the group,
+    each with its own clock (nhu_*), instead of one per solve. A solve computes the same bits
+    either way. The union's memory is released when its last solve finishes."""
 
     def __init__(self, budget_bytes: int | None = None, batch_size: int = 32, device: str = 'cuda:0',
                  max_live: int = 64, group_slots: int = 8, device_factory: Callable | None = None,
-                 graph: bool = True, poll: bool = True):
+                 graph: bool = True, poll: bool = True, union_window: float = 0.0, union_parts: int = 8):
         if budget_bytes is None:
             import torch
             total = torch.cuda.get_device_properties(device).total_memory
@@ -131,21 +142,39 @@ class FloodServer:
         self.group_slots = group_slots
         self.graph = graph
         self.poll = poll   # consume batches in the order they finish (else oldest first)
+        self.union_window = union_window
+        self.union_parts = union_parts
 
     def _engine_factory(self, stream_device):
         def make(layout, params, state, pipe_q=None):
             return Engine(layout, params, state, device=stream_device, pipe_q=pipe_q)
         return make
 
+    def _unionable(self, r: Request) -> bool:
+        return self.union_window > 0 and r.group is None and r.layout is not None and r.params is not None
+
+    def _union_due(self, waiting, pending, now: float, realtime: bool) -> list[Request]:
+        """The requests to admit as one union now (empty: keep holding)."""
+        held = [r for r in waiting if self._unionable(r)]
+        if not held:
+            return []
+        key = held[0].params
+        group = [r for r in held if r.params == key][:self.union_parts]
+        due = (len(group) >= self.union_parts or not pending or not realtime
+               or now - min(r.arrival for r in group) >= self.union_window)
+        return group if due else []
+
     def serve(self, requests: list[Request], realtime: bool = True) -> ServeReport:
         """Run every request; arrivals are honoured on the wall clock when `realtime`."""
         from .batch import BatchEngine
+        from .engine import make_params
+        from .union import UnionEngine
         report = ServeReport()
         pending = sorted(requests, key=lambda r: r.arrival)
         waiting: list[Request] = []
         live = {}          # key -> (sim, request, owner): owner is the Engine or the BatchEngine
         groups = {}        # group -> BatchEngine
-        group_use = {}     # group -> live members
+        group_use = {}     # id(shared engine: BatchEngine or UnionEngine) -> live members
         used = 0
         t0 = perf_counter()
         inflight = []      # (owner, members) in launch order
@@ -154,9 +183,35 @@ class FloodServer:
             now = perf_counter() - t0
             while pending and (pending[0].arrival <= now or not realtime):
                 waiting.append(pending.pop(0))
-            # admission, first come first served, by footprint
+            # unions: a due group of held requests, within the budget, in one UnionEngine
+            group = self._union_due(waiting, pending, now, realtime)
+            while group and len(live) + len(group) > self.max_live:
+                group = group[:-1]
+            while group and used + sum(r.footprint for r in group) > self.budget and live:
+                group = group[:-1]
+            if len(group) >= 2:
+                sb = group[0].params
+                union = UnionEngine([r.layout for r in group], make_params(sb[0], sb[1], sb[2], self.batch_size),
+                                    device=self.device_factory())
+                if self.graph:
+                    union.set_graph(True)
+                group_use[id(union)] = len(group)
+                report.unions += 1
+                for p, r in enumerate(group):
+                    waiting.remove(r)
+                    sim = r.build(union.view(p))
+                    if sim.settings.batch_size != self.batch_size:
+                        raise ValueError('every served solve uses the server batch size')
+                    sim.start()
+                    live[r.key] = (sim, r, union)
+                    used += r.footprint
+                    report.queued[r.key] = perf_counter() - t0 - r.arrival
+            # admission, first come first served, by footprint (a held union request is admitted
+            # alone once it is due and no other solve of its parameters waits)
             while waiting and len(live) < self.max_live:
                 r = waiting[0]
+                if self._unionable(r) and r not in group:
+                    break
                 if used + r.footprint > self.budget and live:
                     break
                 if r.group is not None and groups.get(r.group) is not None and id(groups[r.group]) in busy \

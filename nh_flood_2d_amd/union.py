@@ -262,6 +262,13 @@ class UnionEngine:
     def launch_batch(self, steps: int) -> None:
         abi.check(self.lib, self.lib.nhu_launch_batch(self.ctx, int(steps)), 'nhu_launch_batch')
         self._pending = int(steps)
+        marker = getattr(self.device, 'marker', None)
+        self._done = marker() if marker is not None else None
+
+    def ready(self) -> bool:
+        """True when the launched batch and its trace copy have finished."""
+        done = getattr(self, '_done', None)
+        return done is None or done.query()
 
     def wait_batch(self) -> np.ndarray:
         buf = np.zeros((len(self.layouts), self._pending, abi.TRACE_COLS), dtype=np.uint32)
@@ -314,6 +321,11 @@ class PartView:
         self.c0, self.nc = int(plan.cell_begin[part]), self.layout.e_num - 1
         self.s0, self.ns = int(plan.side_begin[part]), self.layout.s_num - 1
         self.sentinel = {}   # the part's own values at cell 0 / side 0 (kept on the host)
+
+    @property
+    def slot(self) -> int:
+        """The part's row in the union's trace (as a batch slot's in a batch trace)."""
+        return self.part
 
     def _slice(self, name):
         buf = self.union.state[name]

@@ -1,8 +1,9 @@
-"""Serving loop on the GPU: uneven solves overlapped on per-solve streams, each the same as alone.
+"""Serving loop on the GPU: uneven solves overlapped on per-solve streams, or grouped into
+unions, each the same as alone.
 
 Plain-engine solves must match their run alone bit for bit, mass balance included; solves that
-share a mesh group run in one BatchEngine and match in state and dt sequence, with budget rows
-equal to 1e-9 relative (FP64 sums grouped differently).
+share a mesh group run in one BatchEngine, and with a union window the others run as parts of one
+UnionEngine: both match in state and dt sequence, with budget rows equal to 1e-9 relative.
 """
 import math
 
@@ -22,7 +23,8 @@ def _say(text):
     print(text, flush=True)
 
 
-def test_served_mix_matches_each_solve_alone(gpu_device):
+@pytest.mark.parametrize('union_window', [0.0, 10.0])
+def test_served_mix_matches_each_solve_alone(gpu_device, union_window):
     scenarios, groups = [], []
     for i, (blocks, duration) in enumerate([(40, 600.), (16, 300.), (64, 420.), (24, 540.)]):
         base = synth.two_level_fractal(40 + i, blocks=blocks, duration=duration)
@@ -65,18 +67,20 @@ def test_served_mix_matches_each_solve_alone(gpu_device):
         def build(engine_factory, sc=sc, layout=layout, i=i):
             return synth.simulation(sc, engine_factory=engine_factory, yield_step=60., base_layout=layout,
                                     outputs=Capture(i))
-        requests.append(Request(i, build, 0.02 * i, sc.mesh.e_num - 1, footprint_bytes(layout), group=g))
-    server = FloodServer(batch_size=32, group_slots=4)
+        requests.append(Request(i, build, 0.02 * i, sc.mesh.e_num - 1, footprint_bytes(layout), group=g,
+                                layout=layout, params=(sc.afa, sc.sita, sc.min_h)))
+    server = FloodServer(batch_size=32, group_slots=4, union_window=union_window)
     report = server.serve(requests, realtime=True)
     stream.close()
     _say(f'[serve] {report.summary()}')
     assert report.peak_live >= 2
+    assert report.unions == (1 if union_window else 0)
     for i, (row, state, sha) in enumerate(alone):
         got_state, record = finals[i]
         assert record['dt_sequence_sha256'] == sha, f'solve {i}'
         for name, values in state.items():
             np.testing.assert_array_equal(got_state[name], values, err_msg=f'solve {i} {name}')
-        if groups[i] is None:
+        if groups[i] is None and not union_window:
             assert report.results[i] == row
         else:
             for k, x in row.items():

@@ -84,3 +84,23 @@ def test_percentiles(p):
     from nh_flood_2d_amd.serve import ServeReport
     r = ServeReport(latency={i: float(i) for i in range(1, 101)})
     assert r.percentile(p) == pytest.approx(np.percentile(np.arange(1, 101), p))
+
+
+def test_union_admission_rule():
+    """Held requests (layout and parameters given, no mesh group) are admitted together once the
+    oldest has waited the window, the group is full, or no more arrivals are due; only requests of
+    equal parameters share a union."""
+    from nh_flood_2d_amd.serve import FloodServer, Request
+
+    def req(key, arrival, params=(.5, 1., .02), group=None, layout=object()):
+        return Request(key, None, arrival, 10, 1, group=group, layout=layout, params=params)
+    server = FloodServer(budget_bytes=1 << 30, union_window=2.0, union_parts=3)
+    waiting = [req(0, 0.), req(1, .5), req(2, .6, params=(.5, .7, .02)), req(3, .7, group='g')]
+    later = [req(9, 5.)]
+    assert server._union_due(waiting, later, 1.0, True) == []                 # holding
+    assert [r.key for r in server._union_due(waiting, later, 2.0, True)] == [0, 1]
+    assert [r.key for r in server._union_due(waiting, [], 1.0, True)] == [0, 1]  # nothing more to come
+    full = [req(k, .1 * k) for k in range(5)]
+    assert [r.key for r in server._union_due(full, later, .5, True)] == [0, 1, 2]
+    assert FloodServer(budget_bytes=1 << 30)._union_due(waiting, later, 9., True) == []   # no window: off
+    assert server._union_due([req(5, 0., layout=None)], later, 9., True) == []

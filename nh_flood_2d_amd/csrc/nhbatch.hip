@@ -553,16 +553,18 @@ struct BDepth {
     float ds, bvalue, bweight, level_value, bed, rainq, f0, fc, hk, area_stored;
 };
 
-// First round: what needs the cell index only.
+// First round: what needs the cell index only. A cell with rain recomputes its capacity from its
+// rain time, so the stored capacity is read only where the cell may have none: `rained` is true
+// when the slot's one rain pixel has a positive rate (every cell of the slot has rain).
 static __device__ __forceinline__ void bdepth_first(const BDev &d, int s, int sh, int cell, bool raining,
-                                                    BDepth &v) {
+                                                    bool rained, BDepth &v) {
     const nhf_mesh &m = d.m;
     v.meta = m.cell_meta[cell];
     v.rec = cell_record(m, cell);
     v.pix = d.n_pixels > 1 ? m.rain_index[cell] : 0;
     v.j = d.bslot[cell];
     v.h = d.depth[IX(cell)];
-    v.cap = d.capacity[IX(cell)];
+    v.cap = rained ? 0.0f : d.capacity[IX(cell)];
     v.rain_time = raining ? d.rain_time[IX(cell)] : 0.0f;
     v.max_s = d.max_speed[IX(cell)];
     v.max_d = d.max_depth[IX(cell)];
@@ -721,12 +723,14 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
         const size_t total = (size_t)(m.e_num - 1) << sh;
         const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
         const bool raining = d.raining[s] != 0;
+        const bool rained = raining && d.n_pixels == 1 && d.rain_rates[s] > 0.0f;
         const uint64_t step_end = c->step_end;
         for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += U * stride) {
             BDepth v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (t + u * stride < total) bdepth_first(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, v[u]);
+                if (t + u * stride < total)
+                    bdepth_first(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, rained, v[u]);
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (t + u * stride < total) bdepth_second(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, v[u]);

@@ -377,6 +377,21 @@ int32_t nhb_set_tuning(nhb_ctx *ctx, int32_t depth_blocks, int32_t cfl_blocks);
 int32_t nhb_tuning_range(nhb_ctx *ctx, int32_t *out4);
 int32_t nhb_set_timing(nhb_ctx *ctx, int32_t enable);
 int32_t nhb_kernel_times(nhb_ctx *ctx, double *ms, int64_t *count, int32_t n, int32_t reset);
+/* nhf_settle_extrema of one slot (slot -1: every slot), which also writes out the rain time and
+ * capacity of a slot in the uniform rain state (below). Call it before reading a slot's
+ * max_depth, peak_tick, rain_time or capacity, and before writing its depth, max_depth or
+ * peak_tick. No reference counterpart. */
+int32_t nhb_settle_state(nhb_ctx *ctx, int32_t slot);
+/* The rain state of one slot (no reference counterpart). With one rain pixel every cell rains
+ * alike (the depth update of solver_compact.py:543-548), so once every cell has the same rain time
+ * and each land class one capacity, that stays so: in the uniform state (uniform != 0; rain_time
+ * of every cell, class_capacity [256] the capacity of each class code, a host array) the slot
+ * keeps one rain time and a capacity per class, advanced once per step, and the depth pass does
+ * not read or write the per-cell arrays; nhb_settle_state writes them out. uniform == 0 returns
+ * the slot to its per-cell arrays, which must be current (settled) then. The caller sets the
+ * state after writing a slot's rain_time and capacity; every slot starts per-cell. Results are
+ * the same bits in both states. Refused with more than one rain pixel. */
+int32_t nhb_set_rain_state(nhb_ctx *ctx, int32_t slot, int32_t uniform, float rain_time, const float *class_capacity);
 
 /* ------------------------------------------------------------------------------------------
  * Meshes of different sizes advanced by the same launches (nhu_*).

@@ -134,6 +134,8 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
             'nhb_set_tuning': (c_int32, [vp, c_int32, c_int32]),
             'nhb_tuning_range': (c_int32, [vp, vp]),
             'nhb_kernel_times': (c_int32, [vp, vp, vp, c_int32, c_int32]),
+            'nhb_settle_state': (c_int32, [vp, c_int32]),
+            'nhb_set_rain_state': (c_int32, [vp, c_int32, c_int32, c_float, vp]),
         })
     if hasattr(lib, 'nhu_create'):
         sig.update({
@@ -170,7 +172,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
             'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
             'nhb_sync', 'nhb_set_graph', 'nhb_set_timing', 'nhb_kernel_times', 'nhb_set_tuning',
-            'nhb_tuning_range',
+            'nhb_tuning_range', 'nhb_settle_state', 'nhb_set_rain_state',
             'nhu_workspace_bytes', 'nhu_create', 'nhu_reset', 'nhu_set_forcing', 'nhu_begin', 'nhu_run_batch',
             'nhu_launch_batch', 'nhu_wait_batch', 'nhu_compute_cfl', 'nhu_flush_budget', 'nhu_limit_capture',
             'nhu_cfl_state', 'nhf_rasterize']

@@ -26,6 +26,8 @@ from .layout import Layout, meta_boundary
 
 # per-slot cell state of a batch; the bed is shared
 SLOT_CELLS = tuple(n for n in STATE_CELLS if n != 'bed')
+# arrays the device may hold deferred (extrema of rising cells, the uniform rain state)
+SETTLED = ('max_depth', 'peak_tick', 'rain_time', 'capacity')
 
 
 def lanes_for(count: int) -> int:
@@ -121,6 +123,7 @@ class BatchEngine:
         abi.check(self.lib, rc, 'nhb_create')
         self.ctx = handle
         self.terms = 4 + 6 * layout.b_num
+        self.cell_class = (layout.arrays['cell_meta'] & 255).astype(np.intp)
         self._pending = 0
 
     def _check_compatible(self, layout: Layout, params: abi.Params) -> None:
@@ -223,6 +226,10 @@ class BatchEngine:
     def synchronize(self) -> None:
         abi.check(self.lib, self.lib.nhb_sync(self.ctx), 'nhb_sync')
 
+    def settle(self, slot: int = -1) -> None:
+        """Write out a slot's deferred extrema and uniform rain state (every slot for -1)."""
+        abi.check(self.lib, self.lib.nhb_settle_state(self.ctx, int(slot)), 'nhb_settle_state')
+
     def close(self) -> None:
         if self.ctx:
             self.lib.nhb_destroy(self.ctx)
@@ -261,6 +268,8 @@ class ScenarioView:
     def read_state(self, name: str) -> np.ndarray:
         if name == 'bed':
             return self.batch.bed.copy()
+        if name in SETTLED:
+            self.batch.settle(self.slot)
         dev = self.batch.device
         dev.stream.synchronize()
         with dev.torch.cuda.stream(dev.stream):
@@ -278,8 +287,29 @@ class ScenarioView:
             if not np.array_equal(values, self.batch.bed):
                 raise ValueError('the bed is shared by every scenario of a batch')
             return
+        if name in SETTLED or name == 'depth':
+            self.batch.settle(self.slot)   # a marked cell's maximum is its depth: keep the pair final
         values = self.layout.sides_to_device(values) if name in STATE_SIDES else self.layout.cells_to_device(values)
         self._write_column(self.batch.state[name], values)
+        if name in ('rain_time', 'capacity'):
+            self._rain_state()
+
+    def _rain_state(self) -> None:
+        """The slot's rain state after its rain time or capacity was written: uniform (one rain
+        time, one capacity per land class, one rain pixel) when the arrays allow it, else per cell
+        (nhb_set_rain_state). Values are compared bit for bit."""
+        b = self.batch
+        rt = self._column(b.state['rain_time']).contiguous().cpu().numpy()[1:]
+        cap = self._column(b.state['capacity']).contiguous().cpu().numpy()[1:]
+        cls = b.cell_class[1:]
+        table = np.zeros(256, dtype=np.float32)
+        table[cls] = cap
+        uniform = (b.layout.n_pixels <= 1 and rt.size > 0
+                   and bool(np.all(rt.view(np.uint32) == rt.view(np.uint32)[0]))
+                   and np.array_equal(table[cls].view(np.uint32), cap.view(np.uint32)))
+        rc = self.lib.nhb_set_rain_state(b.ctx, self.slot, int(uniform), ctypes.c_float(rt[0] if uniform else 0.),
+                                         table.ctypes.data if uniform else None)
+        abi.check(self.lib, rc, 'nhb_set_rain_state')
 
     # ------------------------------------------------------------------ per-slot protocol
     def reset(self, now_ticks: int) -> None:

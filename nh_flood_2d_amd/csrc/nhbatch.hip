@@ -42,6 +42,15 @@ struct BDev {
     uint32_t *side_class;  // [s_num] class of the lower | higher cell << 16, NHB_NO_CLASS outside
     // boundary-cell list (static)
     const int32_t *bslot, *bcell_start;
+    // Uniform rain state of a slot with one rain pixel (rain_uniform[slot] = 1): every cell's rain
+    // time is rt_u[slot] and its capacity cap_u[class][slot], so the depth pass neither reads nor
+    // writes the per-cell arrays (kb_settle_state writes them out); the prepare step advances
+    // rt_u and the capacity of each used class (class_used) once per step.
+    int32_t *rain_uniform;        // [slot]
+    float *rt_u;                  // [slot]
+    float *cap_u;                 // [256 classes][slot]
+    int32_t *class_used;          // [256]: 1 for a class some real cell has
+    float *cap_stage;             // [256]: nhb_set_rain_state's capacities on their way in
     // scratch
     Ctl *ctl;                     // [slot]
     float *dt;                    // [slot]; 0 = the slot does not step
@@ -160,6 +169,25 @@ __global__ void kb_apply(BDev d, Stage st) {
             d.rain_rates[(size_t)s * d.n_pixels + i] = st.rain[(size_t)s * d.n_pixels + i];
 }
 
+// The rain state a prepared step of a uniform slot advances to, by the whole workgroup (thread
+// t takes classes t, t + blockDim, ...), after prepare_slot_b: where the slot's one pixel rains,
+// every cell's rain time becomes rt_u + dt and its capacity its class's Horton value there, the
+// arithmetic of the depth pass (solver_compact.py:543-548) once per class instead of per cell.
+__device__ void prepare_rain_b(const BDev &d, int s) {
+    const int sh = d.shift;
+    const Ctl *c = &d.ctl[s];
+    if (!d.rain_uniform[s] || !c->active || !d.raining[s] || !(d.rain_rates[s] > 0.0f)) return;
+    const float rt = d.rt_u[s] + c->dt;
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+        if (!d.class_used[k]) continue;
+        const float *hp = d.m.class_horton + 3 * k;
+        const float f0 = hp[0], fc = hp[1], hk = hp[2];
+        d.cap_u[((size_t)k << sh) | (size_t)s] = fc + (f0 - fc) * det_expf(-hk * rt / 3600.0f);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) d.rt_u[s] = rt;
+}
+
 // First step of a batch: clear the trace rows of every slot (one workgroup per slot), then
 // prepare step 0.
 __global__ void kb_prepare(BDev d, int steps) {
@@ -167,6 +195,48 @@ __global__ void kb_prepare(BDev d, int steps) {
     for (int i = threadIdx.x; i < steps * NHF_TRACE_COLS; i += blockDim.x) d.ctl[s].trace[i] = 0u;
     __syncthreads();
     if (threadIdx.x == 0) prepare_slot_b(d, s, 0);
+    __syncthreads();
+    prepare_rain_b(d, s);
+}
+
+// The classes real cells have (for the uniform rain state), once per context.
+__global__ void kb_class_used(BDev d) {
+    for (int cell = blockIdx.x * blockDim.x + threadIdx.x + 1; cell < d.m.e_num; cell += gridDim.x * blockDim.x)
+        d.class_used[meta_class(d.m.cell_meta[cell])] = 1;
+}
+
+// Make a slot's state arrays current (slot < 0: every slot): the deferred extrema of rising cells
+// (a negative max_depth: the maximum is the depth, reached at the end of the slot's last step)
+// and, in the uniform rain state, every cell's rain time and capacity.
+__global__ __launch_bounds__(NHF_BLOCK) void kb_settle_state(BDev d, int slot) {
+    const int sh = d.shift;
+    const int S = 1 << sh;
+    const int s0 = slot < 0 ? 0 : slot, s1 = slot < 0 ? S : slot + 1;
+    for (int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1; cell < d.m.e_num; cell += gridDim.x * NHF_BLOCK) {
+        const int cls = meta_class(d.m.cell_meta[cell]);
+        for (int s = s0; s < s1; ++s) {
+            if (d.max_depth[IX(cell)] < 0.0f) {
+                d.max_depth[IX(cell)] = d.depth[IX(cell)];
+                d.peak_tick[IX(cell)] = d.ctl[s].now;
+            }
+            if (d.rain_uniform[s]) {
+                d.rain_time[IX(cell)] = d.rt_u[s];
+                d.capacity[IX(cell)] = d.cap_u[((size_t)cls << sh) | (size_t)s];
+            }
+        }
+    }
+}
+
+// Enter (uniform != 0) or leave the uniform rain state of one slot; `cap` = [256] capacity by
+// class, `rt` the rain time of every cell (one workgroup).
+__global__ void kb_set_rain_state(BDev d, int s, int uniform, float rt, const float *cap) {
+    const int sh = d.shift;
+    if (uniform)
+        for (int k = threadIdx.x; k < 256; k += blockDim.x) d.cap_u[((size_t)k << sh) | (size_t)s] = cap[k];
+    if (threadIdx.x == 0) {
+        d.rain_uniform[s] = uniform ? 1 : 0;
+        d.rt_u[s] = rt;
+    }
 }
 
 __global__ void kb_reset_limit(BDev d, int s) {
@@ -556,24 +626,27 @@ struct BDepth {
 // First round: what needs the cell index only. A cell with rain recomputes its capacity from its
 // rain time, so the stored capacity is read only where the cell may have none: `rained` is true
 // when the slot's one rain pixel has a positive rate (every cell of the slot has rain).
+// In the uniform rain state (`uniform`) the cell's rain time and capacity are the slot's: neither
+// is loaded (the capacity comes from the class table in the second round).
 static __device__ __forceinline__ void bdepth_first(const BDev &d, int s, int sh, int cell, bool raining,
-                                                    bool rained, BDepth &v) {
+                                                    bool rained, bool uniform, BDepth &v) {
     const nhf_mesh &m = d.m;
     v.meta = m.cell_meta[cell];
     v.rec = cell_record(m, cell);
     v.pix = d.n_pixels > 1 ? m.rain_index[cell] : 0;
     v.j = d.bslot[cell];
     v.h = d.depth[IX(cell)];
-    v.cap = rained ? 0.0f : d.capacity[IX(cell)];
-    v.rain_time = raining ? d.rain_time[IX(cell)] : 0.0f;
+    v.cap = (rained || uniform) ? 0.0f : d.capacity[IX(cell)];
+    v.rain_time = (raining && !uniform) ? d.rain_time[IX(cell)] : 0.0f;
     v.max_s = d.max_speed[IX(cell)];
     v.max_d = d.max_depth[IX(cell)];
 }
 
 // Second round: side discharges of a short list, boundary values, rain rate, Horton parameters.
 static __device__ __forceinline__ void bdepth_second(const BDev &d, int s, int sh, int cell, bool raining,
-                                                     BDepth &v) {
+                                                     bool uniform, BDepth &v) {
     const nhf_mesh &m = d.m;
+    if (uniform) v.cap = d.cap_u[((size_t)meta_class(v.meta) << sh) | (size_t)s];
     cell_list(m, v.rec, v.l);
     if (v.l.short_list) gather_list(v.l, [&](int side) { return d.q[IX(side)]; }, v.qv);
     v.bi = meta_boundary(v.meta);
@@ -600,7 +673,7 @@ static __device__ __forceinline__ void bdepth_second(const BDev &d, int s, int s
 // The update of one (cell, slot): sources, Horton, face totals, continuity, infiltration,
 // floor, level cells and extrema (k_depth's arithmetic, solver_compact.py:525-613), stores last.
 static __device__ __forceinline__ void bdepth_finish(const BDev &d, int s, int sh, int cell, float dt,
-                                                     uint64_t step_end, BDepth &v, double &a_infil,
+                                                     uint64_t step_start, bool uniform, BDepth &v, double &a_infil,
                                                      double &a_floor) {
     const nhf_mesh &m = d.m;
     const int bi = v.bi, kind = v.kind, j = v.j;
@@ -613,8 +686,9 @@ static __device__ __forceinline__ void bdepth_finish(const BDev &d, int s, int s
     }
     float ssq = pipe + discharge;
     const float rainq = v.rainq;
+    // the uniform state's capacity is already this step's (prepare_rain_b)
     float cap = v.cap, rt = 0.0f;
-    if (rainq > 0.0f) {
+    if (rainq > 0.0f && !uniform) {
         rt = v.rain_time + dt;
         cap = v.fc + (v.f0 - v.fc) * det_expf(-v.hk * rt / 3600.0f);
     }
@@ -685,7 +759,7 @@ static __device__ __forceinline__ void bdepth_finish(const BDev &d, int s, int s
         if (kind == 1) d.bvals[IX(6 * j + 0)] = discharge * dt;
         h_new = fmx(raw, 0.0f);
     }
-    if (rainq > 0.0f) {
+    if (rainq > 0.0f && !uniform) {
         d.rain_time[IX(cell)] = rt;
         d.capacity[IX(cell)] = cap;
     }
@@ -699,9 +773,16 @@ static __device__ __forceinline__ void bdepth_finish(const BDev &d, int s, int s
     }
     float speed = sqrtf(u * u + w * w);
     if (speed > v.max_s) d.max_speed[IX(cell)] = speed;
-    if (h_new > v.max_d) {
-        d.max_depth[IX(cell)] = h_new;
-        d.peak_tick[IX(cell)] = step_end;
+    // Extrema as the single-mesh step defers them (solver_compact.py:600-606): a cell whose depth
+    // rises marks max_depth negative once (its maximum is its depth, reached at the end of its
+    // last step); the pair is written out when the rise stops, or by kb_settle_state.
+    if (v.max_d < 0.0f) {
+        if (!(h_new > v.h)) {
+            d.max_depth[IX(cell)] = v.h;
+            d.peak_tick[IX(cell)] = step_start;
+        }
+    } else if (h_new > v.max_d) {
+        d.max_depth[IX(cell)] = -1.0f;
     }
 }
 
@@ -710,8 +791,12 @@ static __device__ __forceinline__ void bdepth_finish(const BDev &d, int s, int s
 // order of its cells; the block then adds them to its own compensated accumulators. U strides
 // at a time: both load rounds of all U cells come before their arithmetic and stores. Boundary
 // cells leave their terms in bvals for kb_finish.
+// With 64 slots the pass is held to 64 registers (8 wavefronts per SIMD): the compiler's own
+// choice of 78 leaves 6, and the pass is bound by how many loads are in flight. It then keeps a
+// few loop-invariant 64-bit values in scratch (52 bytes per lane, reloaded from cache), and still
+// runs faster: 432 against 467 us on the 1 M-cell batch mesh.
 template <bool W, int U>
-__global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
+__global__ __launch_bounds__(NHF_BLOCK) __attribute__((amdgpu_waves_per_eu(W ? 8 : 1))) void kb_depth(BDev d) {
     const int sh = W ? 6 : d.shift;
     const int S = 1 << sh;
     const int s = threadIdx.x & (S - 1);
@@ -724,20 +809,23 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_depth(BDev d) {
         const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
         const bool raining = d.raining[s] != 0;
         const bool rained = raining && d.n_pixels == 1 && d.rain_rates[s] > 0.0f;
-        const uint64_t step_end = c->step_end;
+        const bool uniform = d.rain_uniform[s] != 0;
+        const uint64_t step_start = c->now;  // kb_finish commits this step's end
         for (size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x; t < total; t += U * stride) {
             BDepth v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (t + u * stride < total)
-                    bdepth_first(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, rained, v[u]);
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (t + u * stride < total) bdepth_second(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, v[u]);
+                    bdepth_first(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, rained, uniform, v[u]);
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (t + u * stride < total)
-                    bdepth_finish(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, dt, step_end, v[u], a_infil, a_floor);
+                    bdepth_second(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, raining, uniform, v[u]);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (t + u * stride < total)
+                    bdepth_finish(d, s, sh, entity_of<W>(t + u * stride, sh) + 1, dt, step_start, uniform, v[u], a_infil,
+                                  a_floor);
         }
     }
     // per-slot block totals: lanes of one slot within a wavefront, then the wavefronts
@@ -940,6 +1028,10 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_finish(BDev d, int mode, int ste
             if (step + 1 < steps) prepare_slot_b(d, s, step + 1);
         }
     }
+    if (mode == 1 && step + 1 < steps) {
+        __syncthreads();
+        prepare_rain_b(d, s);
+    }
 }
 
 // Interval budget of slot s: the depth blocks' (sum, compensation) pairs of terms 0 and 1 are
@@ -1054,7 +1146,7 @@ static int slot_shift(int slots) {
 
 struct BCarve {
     size_t ctl, dt, raining, coeff, values, level, rain, bdonor, face_qn, grad, key, idx, acc, bvals, bacc,
-        flush, side_class, total;
+        flush, side_class, rain_uniform, rt_u, cap_u, class_used, cap_stage, total;
 };
 
 static BCarve bcarve(const nhf_mesh *m, const nhb_state *st, const nhf_params *p, int slots) {
@@ -1083,6 +1175,11 @@ static BCarve bcarve(const nhf_mesh *m, const nhb_state *st, const nhf_params *p
     c.bacc = off; off = align_b(off + sizeof(double2) * terms * S);
     c.flush = off; off = align_b(off + sizeof(double2) * terms);
     c.side_class = off; off = align_b(off + (st->class_manning ? sizeof(uint32_t) * (size_t)m->s_num : 0));
+    c.rain_uniform = off; off = align_b(off + sizeof(int32_t) * S);
+    c.rt_u = off; off = align_b(off + sizeof(float) * S);
+    c.cap_u = off; off = align_b(off + sizeof(float) * 256 * S);
+    c.class_used = off; off = align_b(off + sizeof(int32_t) * 256);
+    c.cap_stage = off; off = align_b(off + sizeof(float) * 256);
     c.total = off;
     return c;
 }
@@ -1231,6 +1328,11 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     d.bacc = (double2 *)(ws + cv.bacc);
     d.flush = (double2 *)(ws + cv.flush);
     d.side_class = d.class_manning ? (uint32_t *)(ws + cv.side_class) : nullptr;
+    d.rain_uniform = (int32_t *)(ws + cv.rain_uniform);
+    d.rt_u = (float *)(ws + cv.rt_u);
+    d.cap_u = (float *)(ws + cv.cap_u);
+    d.class_used = (int32_t *)(ws + cv.class_used);
+    d.cap_stage = (float *)(ws + cv.cap_stage);
     ctx->slots = slots;
     ctx->stream = (hipStream_t)stream;
     ctx->graph_enabled = 0;
@@ -1268,9 +1370,15 @@ int32_t nhb_create(nhb_ctx **out, const nhf_mesh *mesh, const nhb_state *state, 
     }
     stage_layout(d, slots, ctx->st_host, ctx->stage_host);
     stage_layout(d, slots, ctx->st_dev, ctx->stage_dev);
-    // every slot starts idle with a clean clock
-    for (int s = 0; s < slots; ++s)
+    // every slot starts idle with a clean clock, in the per-cell rain state
+    for (int s = 0; s < slots; ++s) {
         hipLaunchKernelGGL(kb_reset, dim3(64), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, s, (uint64_t)0);
+        hipLaunchKernelGGL(kb_set_rain_state, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, s, 0, 0.0f,
+                           (const float *)nullptr);
+    }
+    (void)hipMemsetAsync(d.class_used, 0, sizeof(int32_t) * 256, ctx->stream);
+    hipLaunchKernelGGL(kb_class_used, dim3(b_blocks((size_t)mesh->e_num) < 1024 ? b_blocks((size_t)mesh->e_num) : 1024),
+                       dim3(NHF_BLOCK), 0, ctx->stream, ctx->d);
     if (d.class_manning)
         hipLaunchKernelGGL(kb_side_classes, dim3(b_blocks((size_t)mesh->s_num)), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d);
     if ((err = hipGetLastError()) != hipSuccess) {
@@ -1480,6 +1588,32 @@ int32_t nhb_cfl_state(nhb_ctx *ctx, int32_t slot, float *ndt, int32_t *side) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ndt) HIP_TRY(hipMemcpy(ndt, &ctl->ndt, sizeof(float), hipMemcpyDeviceToHost));
     if (side) HIP_TRY(hipMemcpy(side, &ctl->cfl_arg, sizeof(int32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int32_t nhb_settle_state(nhb_ctx *ctx, int32_t slot) {
+    if (!ctx) return set_error_text("nhb_settle_state: null context");
+    if (slot < -1 || slot >= ctx->slots) return set_error_text("nhb_settle_state: slot out of range");
+    if (ctx->pending_steps) return set_error_text("nhb_settle_state: a batch is in flight");
+    const int blocks = b_blocks((size_t)ctx->d.m.e_num);
+    hipLaunchKernelGGL(kb_settle_state, dim3(blocks < 2048 ? blocks : 2048), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d,
+                       (int)slot);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int32_t nhb_set_rain_state(nhb_ctx *ctx, int32_t slot, int32_t uniform, float rain_time, const float *class_capacity) {
+    if (check_slot(ctx, slot)) return -2;
+    if (ctx->pending_steps) return set_error_text("nhb_set_rain_state: a batch is in flight");
+    if (uniform && (ctx->d.m.n_pixels > 1 || !class_capacity))
+        return set_error_text("nhb_set_rain_state: the uniform state needs one rain pixel and the class capacities");
+    if (uniform)
+        HIP_TRY(hipMemcpyAsync(ctx->d.cap_stage, class_capacity, sizeof(float) * 256, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(kb_set_rain_state, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, (int)slot, (int)(uniform != 0),
+                       rain_time, (const float *)ctx->d.cap_stage);
+    HIP_TRY(hipGetLastError());
+    // the host's capacities are read by the copy before this returns
+    if (uniform) HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

@@ -5,6 +5,7 @@ mesh (more scenarios than slots, so a slot is reused). The dt sequence, final st
 must be identical bit for bit; the mass-balance rows may differ only by the grouping of the FP64
 budget sums (relative 1e-9, far inside the 1e-7 volume-budget tolerance).
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -43,6 +44,19 @@ def _run_alone(v, device, yield_step):
     return out
 
 
+def _two_pixels(variants):
+    """The variants with a two-pixel rain raster: cells west of the median centre get the
+    hyetograph, the others 0.6 of it, so no slot can keep one rain time (the per-cell rain state)."""
+    out = []
+    for v in variants:
+        x = v.mesh.geometry()[0][:, 0]
+        pixel = (x > np.median(x[1:])).astype(np.int32)
+        pixel[0] = 0
+        mm = np.asarray(v.rain_mm, float)[:, :1]
+        out.append(dataclasses.replace(v, rain_mm=np.concatenate([mm, 0.6 * mm], axis=1), rain_pixel=pixel))
+    return out
+
+
 def _compare(variants, alone, results, sims):
     for i, (row, sha, state, steps) in enumerate(alone):
         assert sims[i].record['dt_sequence_sha256'] == sha, f'scenario {i}: dt sequence differs'
@@ -53,8 +67,10 @@ def _compare(variants, alone, results, sims):
 # 64 slots take the wide kernels (several sides or cells per wavefront, one wavefront holding
 # the 64 slots of each); fewer slots take the narrow ones. The wide cases bind more scenarios than
 # slots (every lane busy, slots reused) or few scenarios with boundaries, hanging edges and sita < 1.
+# One rain pixel keeps every slot in the uniform rain state (one rain time, one capacity per class);
+# the two-pixel cases keep the per-cell rain time and capacity arrays.
 @pytest.mark.parametrize('case', ['two_level', 'multires_tide_river', 'multires_theta', 'two_level_wide',
-                                  'multires_tide_river_wide', 'multires_theta_wide'])
+                                  'multires_tide_river_wide', 'multires_theta_wide', 'two_pixel', 'two_pixel_wide'])
 def test_batch_matches_each_scenario_alone(gpu_device, case):
     if case == 'two_level':
         base = synth.two_level_fractal(5, blocks=48, duration=600.)
@@ -71,9 +87,15 @@ def test_batch_matches_each_scenario_alone(gpu_device, case):
     elif case == 'multires_tide_river_wide':
         base = synth.small_multires(14, duration=480., discharge=1.5)
         variants, slots = synth.scenario_variants(base, 5, seed=15), 64
-    else:
+    elif case == 'multires_theta_wide':
         base = synth.small_multires(16, duration=420., theta=0.7, pond=0.2)
         variants, slots = synth.scenario_variants(base, 5, seed=17), 64
+    elif case == 'two_pixel':
+        base = synth.small_multires(18, duration=480., discharge=1.0)
+        variants, slots = _two_pixels(synth.scenario_variants(base, 3, seed=19)), 4
+    else:
+        base = synth.two_level_fractal(20, blocks=20, duration=300.)
+        variants, slots = _two_pixels(synth.scenario_variants(base, 6, seed=21)), 64
     _say(f'[batch] {case}: {base.mesh.e_num - 1} cells, {len(variants)} scenarios, {slots} slots; alone runs')
     alone = [_run_alone(v, gpu_device, 120.) for v in variants]
     _say(f'[batch] {case}: batched run')
@@ -138,3 +160,71 @@ def test_batch_graph_replays_match_eager(gpu_device):
     for i in fa:
         for name in fa[i]:
             np.testing.assert_array_equal(fa[i][name], fb[i][name], err_msg=f'{i} {name}')
+
+
+def test_batch_rain_state_follows_the_written_arrays(gpu_device):
+    """A slot's rain state follows what the host writes: uniform at the cold start (one rain pixel),
+    per cell after a rain time that differs between cells is written, uniform again after one rain
+    time and one capacity per class are written. In every state the slot steps as the plain engine
+    given the same writes, bit for bit, and reads back the arrays as written."""
+    from nh_flood_2d_amd.clock import seconds_to_ticks
+    from nh_flood_2d_amd.engine import Engine, make_params
+    from nh_flood_2d_amd.forcing import cell_forcing_arrays
+    from nh_flood_2d_amd.layout import build_layout, initial_state
+    from parity import coefficients_at
+
+    sc = synth.small_multires(22, duration=900., discharge=1.0, pond=0.25)
+    forcing = synth.scenario_forcing(sc)
+    layout = build_layout(sc.mesh, sc.land_parameters, cell_forcing_arrays(sc.mesh, forcing), sc.sita)
+    params = make_params(sc.afa, sc.sita, sc.min_h, 32)
+    state = initial_state(sc.mesh, layout)
+    state['depth'] = sc.initial_depth.astype(np.float32).copy()
+    plain = Engine(layout, params, state, device=gpu_device)
+    batch = BatchEngine(2, device=gpu_device)
+    view = batch.view(1)(layout, params, state)
+    engines = (plain, view)
+    for e in engines:
+        e.reset(0)
+        e.compute_cfl()
+    now = 0
+    cls = layout.arrays['cell_meta'].astype(np.int64) & 255
+    f0 = layout.cell_f0
+
+    def advance(batches):
+        nonlocal now
+        for _ in range(batches):
+            t = now / 2 ** 40
+            rates = forcing.rain.rates(forcing.rain.index_at(t)).astype(np.float32)
+            coeff = coefficients_at(forcing, t).ravel()[:2 * len(forcing.boundaries)]
+            for e in engines:
+                e.set_forcing(coeff, rates, True)
+                e.begin(seconds_to_ticks(t + 30.), now)
+            rows = [e.run_batch(32) for e in engines]
+            np.testing.assert_array_equal(rows[0], rows[1])
+            done = rows[0][rows[0][:, 3] == 1]
+            now = (int(done[-1, 1]) << 32) | int(done[-1, 2])
+        for name in ('depth', 'q', 'rain_time', 'capacity', 'max_depth', 'max_speed', 'peak_tick'):
+            np.testing.assert_array_equal(plain.read_state(name), view.read_state(name), err_msg=name)
+
+    def write_both(name, values):
+        for e in engines:
+            e.write_state(name, values)
+        np.testing.assert_array_equal(view.read_state(name), values, err_msg=f'{name} read back')
+
+    _say('[batch] rain state: uniform from the cold start')
+    advance(4)
+    rt = plain.read_state('rain_time')
+    assert rt[1:].min() == rt[1:].max() > 0
+    _say('[batch] rain state: per cell after a rain time that differs between cells')
+    uneven = rt.copy()
+    uneven[1::3] += np.float32(7.5)
+    write_both('rain_time', uneven)
+    advance(4)
+    _say('[batch] rain state: uniform again after one rain time and per-class capacities')
+    write_both('rain_time', np.full_like(rt, np.float32(40.)))
+    cap = f0.astype(np.float32) * np.float32(0.5)
+    cap[0] = 0.
+    write_both('capacity', cap)
+    advance(4)
+    plain.close()
+    batch.close()

@@ -455,10 +455,11 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
         if (first + k < m.s_num) nq[k] = bflux_value(d, s, sh, first + k, dt, f[k]);
-    // in place, as in k_flux; kb_donor scales them where their donor is limited
+    // in place, as in k_flux; kb_donor scales them where their donor is limited; a discharge
+    // that keeps its bits (a dry face stays at zero) is not stored
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        if (first + k < m.s_num) d.q[IX(first + k)] = nq[k];
+        if (first + k < m.s_num && __float_as_uint(nq[k]) != __float_as_uint(f[k].q)) d.q[IX(first + k)] = nq[k];
 }
 
 // --------------------------------------------------------------------------- donor scale

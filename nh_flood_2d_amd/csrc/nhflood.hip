@@ -442,9 +442,11 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
         if (first + u * NHF_BLOCK < end) nq[u] = flux_value(d, first + u * NHF_BLOCK, dt, f[u]);
     // In place, as the reference's sq_t (solver_compact.py:486): no other thread of this pass
     // reads these sides' discharges. k_donor scales them afterwards where their donor is limited.
+    // A discharge that keeps its bits (a dry face stays at zero) is not stored.
 #pragma unroll
     for (int u = 0; u < SIDES; ++u)
-        if (first + u * NHF_BLOCK < end) m.q[first + u * NHF_BLOCK] = nq[u];
+        if (first + u * NHF_BLOCK < end && __float_as_uint(nq[u]) != __float_as_uint(f[u].q))
+            m.q[first + u * NHF_BLOCK] = nq[u];
 }
 
 // --------------------------------------------------------------------------- donor scale

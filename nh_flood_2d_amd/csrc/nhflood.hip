@@ -63,9 +63,20 @@ struct Dev {
     int32_t cells_per_thread;  // cells per thread of the two cell passes (1 or 2)
     int32_t acc_owners;  // budget accumulators allocated (wavefronts of the largest grid)
     int32_t gates_on;    // gate control bound (nhf_set_gates)
+    // Per cell: nonzero when the donor pass must look at the cell (see donor_check): bit 0 set by
+    // the flux pass for a cell one of whose outflows exceeds its certified share, bit 1 always set
+    // for cells the certificate does not cover (boundary cells, long side lists, pipe exchange).
+    uint8_t *donor_flag;
+    int32_t donor_keep;  // 1: k_donor leaves the flags as they are (the per-kernel timing pass)
     nhf_gates g;
     PartDev parts;
 };
+
+// The cells the donor certificate (donor_check) does not cover: the donor pass looks at them
+// every step.
+static __device__ __forceinline__ uint8_t donor_always(const nhf_mesh &m, uint32_t meta, int4 rec) {
+    return (meta_boundary(meta) >= 0 || m.pipe_q || (rec.x < 0 && rec.w > NHF_LIST_MAX)) ? 2 : 0;
+}
 
 // The part of a workgroup whose first cell (side) is `first`; 0 for a single mesh.
 template <bool PARTS>
@@ -143,8 +154,20 @@ __global__ void k_reset(Dev d, int part, uint64_t now) {
     const int4 span = part_span(d, part);
     for (int i = span.z * d.terms + tid; i < span.w * d.terms; i += stride) d.acc[i] = make_double2(0.0, 0.0);
     for (int i = span.x + tid; i < span.y; i += stride) d.donor[i] = 0.0f;
+    // the donor flags of the part's real cells: their standing values (padding cells stay 0)
+    const int cell_end = d.parts.count ? d.parts.range[part].y : d.m.e_num;
+    for (int i = (span.x > 0 ? span.x : 1) + tid; i < span.y; i += stride)
+        d.donor_flag[i] = i < cell_end ? donor_always(d.m, d.m.cell_meta[i], cell_record(d.m, i)) : 0;
     if (d.face_qn)
         for (int i = 4 * span.x + tid; i < 4 * span.y; i += stride) d.face_qn[i] = 0.0f;
+}
+
+// The standing donor flags of every real cell, once per context (k_reset sets a part's again).
+__global__ __launch_bounds__(NHF_BLOCK) void k_init_flags(Dev d) {
+    for (int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1; cell < d.m.e_num; cell += gridDim.x * NHF_BLOCK) {
+        const bool real = !d.parts.count || cell < d.parts.range[d.parts.cell_chunk[(cell - 1) >> 10]].y;
+        d.donor_flag[cell] = real ? donor_always(d.m, d.m.cell_meta[cell], cell_record(d.m, cell)) : 0;
+    }
 }
 
 __global__ void k_begin(Dev d, int part, uint64_t target, uint64_t origin) {
@@ -362,8 +385,9 @@ struct FluxSide {
     float q, zl, zh, dl, dh;
 };
 
-// The new discharge of side si from its loaded values (solver_compact.py:439-486).
-static __device__ __forceinline__ float flux_value(const Dev &d, int si, float dt, const FluxSide &f) {
+// The new discharge of side si from its loaded values (solver_compact.py:439-486). `len` gets
+// the side's length on a wet face (loaded with its coefficients), 0 on a dry one.
+static __device__ __forceinline__ float flux_value(const Dev &d, int si, float dt, const FluxSide &f, float &len) {
     const nhf_mesh &m = d.m;
     const float g = d.p.g;
     const float bf = (f.meta & NHF_SIDE_EXTERIOR) ? 0.0f : 1.0f;
@@ -372,7 +396,9 @@ static __device__ __forceinline__ float flux_value(const Dev &d, int si, float d
     const float wh = fmx(hl, hr) - fmx(fmx(f.zl, f.zh), -999.0f);
     // a face below min_h carries nothing (solver_compact.py:462,484): its arithmetic is skipped
     float nq = 0.0f;
+    len = 0.0f;
     if (!(wh < d.p.min_h)) {
+        len = m.length[si];
         float difference = hr - hl;
         if (m.edge_index) {
             int idx = m.edge_index[si];
@@ -398,6 +424,25 @@ static __device__ __forceinline__ float flux_value(const Dev &d, int si, float d
         nq = (d.p.theta * q + d.p.theta_half * neighbours + dq) / fr;
     }
     return nq * bf;
+}
+
+// The donor certificate. A cell's donor scale is below 1 only when the volume it sends out
+// exceeds what it holds (solver_compact.py:491-514). With no boundary or pipe term, and at most
+// NHF_LIST_MAX sides, each no longer than the cell is wide (a 2:1 quadtree of squares), that
+// cannot happen while every side the cell donates to carries |q| dt <= 0.12 h len, h the cell's
+// depth: the outgoing sum is then at most 8 x 0.12 h area, plus F32 rounding, below the
+// available volume of at least h area. The flux pass flags the donor of any side that breaks
+// that bound (or whose bound is too small to hold through rounding, below 2^-100), and the
+// donor pass looks only at flagged cells; every other cell keeps scale 1 exactly, as k_donor
+// would find it.
+#define NHF_DONOR_SHARE 0.12f
+
+static __device__ __forceinline__ void donor_check(const Dev &d, const FluxSide &f, float nq, float len, float dt) {
+    if (nq == 0.0f) return;  // a dry face, or no flow: the donor sends nothing
+    const bool lower = nq >= 0.0f;  // the donor test of k_donor
+    const float hd = lower ? f.dl : f.dh;
+    const float bound = NHF_DONOR_SHARE * hd * len;
+    if (!(fabsf(nq) * dt <= bound && bound >= 0x1p-100f)) d.donor_flag[lower ? f.lh.x : f.lh.y] = 1;
 }
 
 // A workgroup takes NHF_BLOCK * SIDES consecutive sides; thread t the sides t, t + NHF_BLOCK, ...
@@ -436,10 +481,10 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
             f[u].dh = m.depth[f[u].lh.y];
         }
     }
-    float nq[SIDES];
+    float nq[SIDES], len[SIDES];
 #pragma unroll
     for (int u = 0; u < SIDES; ++u)
-        if (first + u * NHF_BLOCK < end) nq[u] = flux_value(d, first + u * NHF_BLOCK, dt, f[u]);
+        if (first + u * NHF_BLOCK < end) nq[u] = flux_value(d, first + u * NHF_BLOCK, dt, f[u], len[u]);
     // In place, as the reference's sq_t (solver_compact.py:486): no other thread of this pass
     // reads these sides' discharges. k_donor scales them afterwards where their donor is limited.
     // A discharge that keeps its bits (a dry face stays at zero) is not stored.
@@ -447,6 +492,9 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
     for (int u = 0; u < SIDES; ++u)
         if (first + u * NHF_BLOCK < end && __float_as_uint(nq[u]) != __float_as_uint(f[u].q))
             m.q[first + u * NHF_BLOCK] = nq[u];
+#pragma unroll
+    for (int u = 0; u < SIDES; ++u)
+        if (first + u * NHF_BLOCK < end) donor_check(d, f[u], nq[u], len[u], dt);
 }
 
 // --------------------------------------------------------------------------- donor scale
@@ -591,9 +639,18 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
     const bool raining = c->raining;
     const int first = block_first + threadIdx.x;
     uint32_t limited = 0, faces = 0, scale_bits = __float_as_uint(1.0f);
+    // Only cells the flux pass flagged, or that the certificate does not cover, are looked at
+    // (donor_check): every other cell's scale is 1 and it changes nothing. A flagged cell's flag
+    // goes back to its standing value once it has been looked at.
+    auto unflag = [&](int cell, uint8_t flag, const DonorCell &v) {
+        const uint8_t standing = donor_always(m, v.meta, v.rec);
+        if (!d.donor_keep && flag != standing) d.donor_flag[cell] = standing;
+    };
     // first round: the cells' own values and their side records
     if constexpr (CPT == 1) {   // the plain form keeps every value in registers
-        const bool live = first < end;
+        const uint8_t flag = first < end ? d.donor_flag[first] : 0;
+        const bool live = flag != 0;
+        if (!__any(live)) return;
         DonorCell v = {};
         if (live) load_donor_cell(m, first, raining, v);
         if (__all(!live || v.rec.x >= 0)) {
@@ -601,10 +658,13 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
         } else if (live) {
             donor_cell<false>(d, first, dt, raining, v, limited, faces, scale_bits);
         }
+        if (live) unflag(first, flag, v);
     } else {   // two cells: both first rounds, then both bodies (named values, no arrays)
         static_assert(CPT == 2, "one or two cells per thread");
         const int second = first + BLOCK;
-        const bool live0 = first < end, live1 = second < end;
+        const uint8_t flag0 = first < end ? d.donor_flag[first] : 0, flag1 = second < end ? d.donor_flag[second] : 0;
+        const bool live0 = flag0 != 0, live1 = flag1 != 0;
+        if (!__any(live0 || live1)) return;
         DonorCell v0 = {}, v1 = {};
         if (live0) load_donor_cell(m, first, raining, v0);
         if (live1) load_donor_cell(m, second, raining, v1);
@@ -618,6 +678,8 @@ __global__ __launch_bounds__(BLOCK) void k_donor(Dev d) {
         } else if (live1) {
             donor_cell<false>(d, second, dt, raining, v1, limited, faces, scale_bits);
         }
+        if (live0) unflag(first, flag0, v0);
+        if (live1) unflag(second, flag1, v1);
     }
     // limiter counters: one integer atomic per wavefront that limited a cell or face (rare)
     if (__any(limited | faces)) {
@@ -1234,7 +1296,7 @@ static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Carve {
     size_t ctl, cell_chunk, side_chunk, range, cfl_part, coeff, values, level, rain, donor, face_qn, grad, key, idx,
-        acc, total;
+        acc, donor_flag, total;
 };
 
 static int blocks_for(int n) { return (n + NHF_BLOCK - 1) / NHF_BLOCK; }
@@ -1358,6 +1420,7 @@ static Carve carve(const nhf_mesh *m, const nhf_params *p, const PartPlan *plan 
     c.key = off; off = align_up(off + sizeof(uint64_t) * cfl_blocks);
     c.idx = off; off = align_up(off + sizeof(int32_t) * cfl_blocks);
     c.acc = off; off = align_up(off + sizeof(double2) * (size_t)acc_owner_count(m->e_num) * terms);
+    c.donor_flag = off; off = align_up(off + (size_t)m->e_num);
     c.total = off;
     return c;
 }
@@ -1420,6 +1483,8 @@ static void bind_dev(Dev &d, const nhf_mesh *mesh, const nhf_params *params, con
     d.cfl_key = (uint64_t *)(ws + cv.key);
     d.cfl_idx = (int32_t *)(ws + cv.idx);
     d.acc = (double2 *)(ws + cv.acc);
+    d.donor_flag = (uint8_t *)(ws + cv.donor_flag);
+    d.donor_keep = 0;
     d.terms = 4 + 6 * (plan ? plan->max_b : mesh->b_num);
     d.cfl_blocks = plan ? plan->cfl_total : cfl_grid(mesh->s_num);
     d.cfl_max = plan ? plan->cfl_total : cfl_max_blocks(mesh->s_num);
@@ -1462,6 +1527,16 @@ static int32_t create_ctx(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params 
     }
     nhf_ctx *ctx = new nhf_ctx();
     bind_dev(ctx->d, mesh, params, plan, ws, cv);
+    {
+        const int blocks = blocks_for(mesh->e_num);
+        hipLaunchKernelGGL(k_init_flags, dim3(blocks < 2048 ? blocks : 2048), dim3(NHF_BLOCK), 0, (hipStream_t)stream,
+                           ctx->d);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) {
+            delete ctx;
+            return set_error("nhf_create: donor flags", err);
+        }
+    }
     ctx->parts = 0;
     ctx->host_ctl = nullptr;
     if (plan) {
@@ -1740,6 +1815,9 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
     for (int i = 0; i < K_COUNT; ++i) us[i] = 0.0;
     hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, s, d, 0);
     const int sides = d.m.s_num - 1;
+    // the donor pass's repeats look at the cells the flux pass flagged, as the step's one launch
+    Dev keep = d;
+    keep.donor_keep = 1;
     for (int k = K_GRAD; k < K_COUNT; ++k) {
         if ((k == K_GRAD && d.m.n_grad == 0) || (k == K_FACE && d.theta_one) || (k == K_FLUX && sides <= 0) ||
             k == K_GATE)
@@ -1750,7 +1828,7 @@ int32_t nhf_time_kernels(nhf_ctx *ctx, int32_t repeats, double *us) {
             case K_GRAD: hipLaunchKernelGGL(k_gradients<false>, dim3(blocks_for(d.m.n_grad)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_FACE: hipLaunchKernelGGL(k_face_avg<false>, dim3(blocks_for(d.m.e_num)), dim3(NHF_BLOCK), 0, s, d); break;
             case K_FLUX: launch_k_flux(d, sides, s); break;
-            case K_DONOR: launch_k_donor(d, s); break;
+            case K_DONOR: launch_k_donor(keep, s); break;
             case K_DEPTH: launch_k_depth(d, s); break;
             case K_CFL: hipLaunchKernelGGL(k_cfl<false>, dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1, 0); break;
             case K_FINISH: hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(1024), 0, s, d, 0, 0, 0); break;

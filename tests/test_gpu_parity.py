@@ -229,3 +229,27 @@ def test_stored_and_computed_cell_sizes_give_the_same_bits(gpu_device):
     np.testing.assert_array_equal(qa.view(np.uint32), qb.view(np.uint32))
     np.testing.assert_array_equal(sa.view(np.uint32), sb.view(np.uint32))
     np.testing.assert_allclose(ba.sum(axis=1), bb.sum(axis=1), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize('theta', [1.0, 0.7])
+def test_donor_limiting_matches_oracle(gpu_device, theta):
+    """A thin sheet of water on the multi-resolution mesh's slopes drains faster than some cells
+    hold, so the donor limiter scales their outflows. The device looks only at cells the face pass
+    flags (one of whose outflows exceeds its certified share of the cell's volume) or that the
+    certificate does not cover; the trace's limiter columns, depth and discharge must still be the
+    oracle's bits at every step, and the limiter must have acted."""
+    sc = synth.small_multires(11, pond=0.04, discharge=1.0, theta=theta)
+    sc.initial_depth[1:] = np.where(sc.mesh.z[1:] < 2.0, np.float32(0.03), np.float32(0.)).astype(np.float32)
+    pair = Pair(sc, gpu_device)
+    limited = faces = 0
+    for step in range(50):
+        g, c = pair.fixed_step(1.0)
+        assert g[0, 3] == c[0, 3] == abi.COMPLETED, (step, g, c)
+        np.testing.assert_array_equal(g[0, :9], c[0, :9])
+        for state in ('depth', 'q'):
+            np.testing.assert_array_equal(pair.gpu.read_state(state).view(np.uint32),
+                                          pair.cpu.read_state(state).view(np.uint32), err_msg=f'{state} step {step}')
+        limited += int(g[0, 6])
+        faces += int(g[0, 7])
+    _say(f'[limiter] theta {theta}: {limited} limited cells, {faces} limited faces over 50 steps')
+    assert limited > 0 and faces > 0

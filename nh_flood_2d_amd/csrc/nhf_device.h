@@ -381,6 +381,23 @@ static __device__ __forceinline__ float neighbour_sum(float q, float left, float
     return result;
 }
 
+// The donor certificate (k_donor's test, solver_compact.py:491-514). A cell's donor scale is
+// below 1 only when the volume it sends out exceeds what it holds. With no boundary or pipe term
+// and at most NHF_LIST_MAX sides, each no longer than the cell is wide (a 2:1 quadtree of
+// squares), that cannot happen while every side the cell donates to carries
+// |q| dt <= NHF_DONOR_SHARE h len, h the cell's depth: the outgoing sum is then at most
+// 8 x 0.12 h area, with F32 rounding, below the available volume of at least h area. The face
+// pass flags the donor of any side that breaks that bound, or whose bound is too small to hold
+// through rounding (below 2^-100); the donor pass looks at flagged cells only, and every other
+// cell keeps scale 1 exactly. The donor of a side is its lower cell when q >= 0.
+#define NHF_DONOR_SHARE 0.12f
+
+static __device__ __forceinline__ bool donor_share_exceeded(float nq, float dt, float h_donor, float len) {
+    if (nq == 0.0f) return false;  // a dry face, or no flow: the donor sends nothing
+    const float bound = NHF_DONOR_SHARE * h_donor * len;
+    return !(fabsf(nq) * dt <= bound && bound >= 0x1p-100f);
+}
+
 // Neumaier step on a (sum, compensation) pair.
 static __device__ __forceinline__ void acc_add(double2 &a, double v) {
     double s = a.x + v;

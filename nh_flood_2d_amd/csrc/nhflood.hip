@@ -426,23 +426,11 @@ static __device__ __forceinline__ float flux_value(const Dev &d, int si, float d
     return nq * bf;
 }
 
-// The donor certificate. A cell's donor scale is below 1 only when the volume it sends out
-// exceeds what it holds (solver_compact.py:491-514). With no boundary or pipe term, and at most
-// NHF_LIST_MAX sides, each no longer than the cell is wide (a 2:1 quadtree of squares), that
-// cannot happen while every side the cell donates to carries |q| dt <= 0.12 h len, h the cell's
-// depth: the outgoing sum is then at most 8 x 0.12 h area, plus F32 rounding, below the
-// available volume of at least h area. The flux pass flags the donor of any side that breaks
-// that bound (or whose bound is too small to hold through rounding, below 2^-100), and the
-// donor pass looks only at flagged cells; every other cell keeps scale 1 exactly, as k_donor
-// would find it.
-#define NHF_DONOR_SHARE 0.12f
-
+// The donor certificate (donor_share_exceeded): the flux pass flags the donor of a side that
+// carries more than its share of the donor's volume.
 static __device__ __forceinline__ void donor_check(const Dev &d, const FluxSide &f, float nq, float len, float dt) {
-    if (nq == 0.0f) return;  // a dry face, or no flow: the donor sends nothing
     const bool lower = nq >= 0.0f;  // the donor test of k_donor
-    const float hd = lower ? f.dl : f.dh;
-    const float bound = NHF_DONOR_SHARE * hd * len;
-    if (!(fabsf(nq) * dt <= bound && bound >= 0x1p-100f)) d.donor_flag[lower ? f.lh.x : f.lh.y] = 1;
+    if (donor_share_exceeded(nq, dt, lower ? f.dl : f.dh, len)) d.donor_flag[lower ? f.lh.x : f.lh.y] = 1;
 }
 
 // A workgroup takes NHF_BLOCK * SIDES consecutive sides; thread t the sides t, t + NHF_BLOCK, ...

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define NHF_ABI_VERSION 11
+#define NHF_ABI_VERSION 12
 #define NHF_TRACE_COLS 9
 #define NHF_MAX_BATCH 64
 
@@ -230,9 +230,22 @@ int32_t nhf_flush_gate_budget(nhf_ctx *ctx, double *out);
  * reset_extrema/write_extrema see them, :674-679, output.py:130-150). The step leaves a cell
  * whose depth rose in its last step marked by a negative max_depth (its maximum is its depth,
  * reached at the end of that step) and writes the pair out when the rise stops; this call writes
- * out every marked cell. Call it before reading either array, and before writing depth alone.
+ * out every marked cell, and in the uniform rain state (nhf_set_rain_state) every cell's
+ * rain_time and capacity. Call it before reading max_depth, peak_tick, rain_time or capacity,
+ * and before writing any of them or depth alone (every part of a parts context).
  * No reference counterpart (the reference writes both arrays on every rise). */
 int32_t nhf_settle_extrema(nhf_ctx *ctx);
+
+/* The rain state (no reference counterpart). With one rain pixel every cell rains alike (the
+ * depth update of solver_compact.py:543-548), so once every cell has the same rain time and each
+ * land class one capacity, that stays so: in the uniform state (uniform != 0; rain_time of every
+ * cell, class_capacity [256] the capacity of each class code, a host array) the context keeps one
+ * rain time and a capacity per class, advanced once per step, and the depth pass neither reads
+ * nor writes the per-cell arrays; nhf_settle_extrema writes them out. uniform == 0 returns to the
+ * per-cell arrays, which must be current (settled) then. The caller sets the state after writing
+ * rain_time or capacity; a context starts per-cell. Results are the same bits in both states.
+ * Refused with more than one rain pixel. */
+int32_t nhf_set_rain_state(nhf_ctx *ctx, int32_t uniform, float rain_time, const float *class_capacity);
 
 /* Launch mode: 1 replays each batch as one captured hipGraph, 0 launches kernels one by one. */
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable);
@@ -441,6 +454,9 @@ int32_t nhu_flush_budget(nhf_ctx *ctx, int32_t part, double *out);
 /* nhf_limit_capture / nhf_cfl_state of one part (side and cell ids are global). */
 int32_t nhu_limit_capture(nhf_ctx *ctx, int32_t part, float *values, int32_t *ids, int32_t reset);
 int32_t nhu_cfl_state(nhf_ctx *ctx, int32_t part, float *ndt, int32_t *side);
+/* nhf_set_rain_state of one part: the part's own rain pixel, class_capacity indexed by the
+ * context's class codes (those in the part's cell_meta). */
+int32_t nhu_set_rain_state(nhf_ctx *ctx, int32_t part, int32_t uniform, float rain_time, const float *class_capacity);
 
 #ifdef __cplusplus
 }

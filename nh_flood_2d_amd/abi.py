@@ -5,7 +5,7 @@ import ctypes
 from ctypes import POINTER, c_double, c_float, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
-ABI_VERSION = 11
+ABI_VERSION = 12
 TRACE_COLS = 9
 MAX_BATCH = 64
 MAX_SLOTS = 64
@@ -100,6 +100,7 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
         'nhf_set_tuning': (c_int32, [vp, c_int32, c_int32, c_int32]),
         'nhf_time_kernels': (c_int32, [vp, c_int32, vp]),
         'nhf_settle_extrema': (c_int32, [vp]),
+        'nhf_set_rain_state': (c_int32, [vp, c_int32, c_float, vp]),
         'nhf_set_gates': (c_int32, [vp, POINTER(GatesStruct)]),
         'nhf_gate_check': (c_int32, [vp]),
         'nhf_flush_gate_budget': (c_int32, [vp, vp]),
@@ -152,6 +153,7 @@ def bind(lib: ctypes.CDLL) -> ctypes.CDLL:
             'nhu_flush_budget': (c_int32, [vp, c_int32, vp]),
             'nhu_limit_capture': (c_int32, [vp, c_int32, vp, vp, c_int32]),
             'nhu_cfl_state': (c_int32, [vp, c_int32, vp, vp]),
+            'nhu_set_rain_state': (c_int32, [vp, c_int32, c_int32, c_float, vp]),
         })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -168,6 +170,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhf_compute_cfl', 'nhf_flush_budget', 'nhf_limit_capture', 'nhf_cfl_state',
             'nhf_sync', 'nhf_set_graph', 'nhf_set_timing', 'nhf_kernel_times', 'nhf_clock_probe',
             'nhf_set_tuning', 'nhf_tuning_range', 'nhf_time_kernels', 'nhf_settle_extrema',
+            'nhf_set_rain_state',
             'nhf_set_gates', 'nhf_gate_check', 'nhf_flush_gate_budget', 'nhf_run_static',
             'nhb_workspace_bytes', 'nhb_create', 'nhb_destroy', 'nhb_reset', 'nhb_set_forcing', 'nhb_begin',
             'nhb_run_batch', 'nhb_launch_batch', 'nhb_wait_batch', 'nhb_compute_cfl', 'nhb_flush_budget', 'nhb_limit_capture', 'nhb_cfl_state',
@@ -175,7 +178,7 @@ EXPORTED = ['nhf_abi_version', 'nhf_last_error', 'nhf_workspace_bytes', 'nhf_cre
             'nhb_tuning_range', 'nhb_settle_state', 'nhb_set_rain_state',
             'nhu_workspace_bytes', 'nhu_create', 'nhu_reset', 'nhu_set_forcing', 'nhu_begin', 'nhu_run_batch',
             'nhu_launch_batch', 'nhu_wait_batch', 'nhu_compute_cfl', 'nhu_flush_budget', 'nhu_limit_capture',
-            'nhu_cfl_state', 'nhf_rasterize']
+            'nhu_cfl_state', 'nhu_set_rain_state', 'nhf_rasterize']
 
 _hip_lib = None
 

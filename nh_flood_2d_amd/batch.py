@@ -21,13 +21,12 @@ from time import perf_counter
 import numpy as np
 
 from . import abi
-from .engine import STATE_CELLS, STATE_SIDES, TorchDevice
+from .engine import SETTLED, STATE_CELLS, STATE_SIDES, TorchDevice, uniform_rain
 from .layout import Layout, meta_boundary
 
 # per-slot cell state of a batch; the bed is shared
 SLOT_CELLS = tuple(n for n in STATE_CELLS if n != 'bed')
 # arrays the device may hold deferred (extrema of rising cells, the uniform rain state)
-SETTLED = ('max_depth', 'peak_tick', 'rain_time', 'capacity')
 
 
 def lanes_for(count: int) -> int:
@@ -301,13 +300,8 @@ class ScenarioView:
         b = self.batch
         rt = self._column(b.state['rain_time']).contiguous().cpu().numpy()[1:]
         cap = self._column(b.state['capacity']).contiguous().cpu().numpy()[1:]
-        cls = b.cell_class[1:]
-        table = np.zeros(256, dtype=np.float32)
-        table[cls] = cap
-        uniform = (b.layout.n_pixels <= 1 and rt.size > 0
-                   and bool(np.all(rt.view(np.uint32) == rt.view(np.uint32)[0]))
-                   and np.array_equal(table[cls].view(np.uint32), cap.view(np.uint32)))
-        rc = self.lib.nhb_set_rain_state(b.ctx, self.slot, int(uniform), ctypes.c_float(rt[0] if uniform else 0.),
+        uniform, rt0, table = uniform_rain(rt, cap, b.cell_class[1:], b.layout.n_pixels)
+        rc = self.lib.nhb_set_rain_state(b.ctx, self.slot, int(uniform), ctypes.c_float(rt0),
                                          table.ctypes.data if uniform else None)
         abi.check(self.lib, rc, 'nhb_set_rain_state')
 

@@ -44,8 +44,8 @@ struct BDev {
     const int32_t *bslot, *bcell_start;
     // Uniform rain state of a slot with one rain pixel (rain_uniform[slot] = 1): every cell's rain
     // time is rt_u[slot] and its capacity cap_u[class][slot], so the depth pass neither reads nor
-    // writes the per-cell arrays (kb_settle_state writes them out); the prepare step advances
-    // rt_u and the capacity of each used class (class_used) once per step.
+    // writes the per-cell arrays (kb_settle_state writes them out); the flux pass advances
+    // rt_u and the capacity of each used class (class_used) once per step (advance_rain_b).
     int32_t *rain_uniform;        // [slot]
     float *rt_u;                  // [slot]
     float *cap_u;                 // [256 classes][slot]
@@ -173,23 +173,22 @@ __global__ void kb_apply(BDev d, Stage st) {
             d.rain_rates[(size_t)s * d.n_pixels + i] = st.rain[(size_t)s * d.n_pixels + i];
 }
 
-// The rain state a prepared step of a uniform slot advances to, by the whole workgroup (thread
-// t takes classes t, t + blockDim, ...), after prepare_slot_b: where the slot's one pixel rains,
-// every cell's rain time becomes rt_u + dt and its capacity its class's Horton value there, the
-// arithmetic of the depth pass (solver_compact.py:543-548) once per class instead of per cell.
-__device__ void prepare_rain_b(const BDev &d, int s) {
+// The rain state a step of a uniform slot advances to, by the slot's thread of the first side
+// entity in the flux pass: where the slot's one pixel rains, every cell's rain time becomes
+// rt_u + dt and its capacity its class's Horton value there, the arithmetic of the depth pass
+// (solver_compact.py:543-548) once per class instead of per cell. The depth pass of the same
+// step reads it.
+__device__ void advance_rain_b(const BDev &d, int s, float dt) {
     const int sh = d.shift;
-    const Ctl *c = &d.ctl[s];
-    if (!d.rain_uniform[s] || !c->active || !d.raining[s] || !(d.rain_rates[s] > 0.0f)) return;
-    const float rt = d.rt_u[s] + c->dt;
-    for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+    if (!d.rain_uniform[s] || !d.raining[s] || !(d.rain_rates[s] > 0.0f)) return;
+    const float rt = d.rt_u[s] + dt;
+    for (int k = 0; k < 256; ++k) {
         if (!d.class_used[k]) continue;
         const float *hp = d.m.class_horton + 3 * k;
         const float f0 = hp[0], fc = hp[1], hk = hp[2];
         d.cap_u[((size_t)k << sh) | (size_t)s] = fc + (f0 - fc) * det_expf(-hk * rt / 3600.0f);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) d.rt_u[s] = rt;
+    d.rt_u[s] = rt;
 }
 
 // First step of a batch: clear the trace rows of every slot (one workgroup per slot), then
@@ -199,8 +198,6 @@ __global__ void kb_prepare(BDev d, int steps) {
     for (int i = threadIdx.x; i < steps * NHF_TRACE_COLS; i += blockDim.x) d.ctl[s].trace[i] = 0u;
     __syncthreads();
     if (threadIdx.x == 0) prepare_slot_b(d, s, 0);
-    __syncthreads();
-    prepare_rain_b(d, s);
 }
 
 // The classes real cells have (for the uniform rain state), once per context.
@@ -434,6 +431,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
     if (first >= d.m.s_num) return;
     const float dt = d.dt[s];
     if (dt == 0.0f) return;
+    // the first entity's threads, one per slot, advance the slots' uniform rain state
+    if (first == 1) advance_rain_b(d, s, dt);
     const nhf_mesh &m = d.m;
     BFlux f[K];
 #pragma unroll
@@ -724,7 +723,7 @@ static __device__ __forceinline__ void bdepth_finish(const BDev &d, int s, int s
     }
     float ssq = pipe + discharge;
     const float rainq = v.rainq;
-    // the uniform state's capacity is already this step's (prepare_rain_b)
+    // the uniform state's capacity is already this step's (advance_rain_b)
     float cap = v.cap, rt = 0.0f;
     if (rainq > 0.0f && !uniform) {
         rt = v.rain_time + dt;
@@ -1065,10 +1064,6 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_finish(BDev d, int mode, int ste
             d.dt[s] = 0.0f;
             if (step + 1 < steps) prepare_slot_b(d, s, step + 1);
         }
-    }
-    if (mode == 1 && step + 1 < steps) {
-        __syncthreads();
-        prepare_rain_b(d, s);
     }
 }
 
@@ -1647,8 +1642,10 @@ int32_t nhb_settle_state(nhb_ctx *ctx, int32_t slot) {
 int32_t nhb_set_rain_state(nhb_ctx *ctx, int32_t slot, int32_t uniform, float rain_time, const float *class_capacity) {
     if (check_slot(ctx, slot)) return -2;
     if (ctx->pending_steps) return set_error_text("nhb_set_rain_state: a batch is in flight");
-    if (uniform && (ctx->d.m.n_pixels > 1 || !class_capacity))
-        return set_error_text("nhb_set_rain_state: the uniform state needs one rain pixel and the class capacities");
+    // the flux pass advances the uniform state, so a mesh without sides keeps the per-cell one
+    if (uniform && (ctx->d.m.n_pixels > 1 || !class_capacity || ctx->d.m.s_num <= 1))
+        return set_error_text("nhb_set_rain_state: the uniform state needs one rain pixel, sides and the class "
+                              "capacities");
     if (uniform)
         HIP_TRY(hipMemcpyAsync(ctx->d.cap_stage, class_capacity, sizeof(float) * 256, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(kb_set_rain_state, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, (int)slot, (int)(uniform != 0),

@@ -68,6 +68,14 @@ struct Dev {
     // for cells the certificate does not cover (boundary cells, long side lists, pipe exchange).
     uint8_t *donor_flag;
     int32_t donor_keep;  // 1: k_donor leaves the flags as they are (the per-kernel timing pass)
+    // Uniform rain state of a part with one rain pixel (rain_uniform[part] = 1): every cell's rain
+    // time is rt_u[part] and its capacity cap_u[class], so the depth pass neither reads nor
+    // writes the per-cell arrays (k_settle_extrema writes them out); the flux pass advances
+    // rt_u and the capacity of the part's classes (class_part) once per step (advance_rain).
+    int32_t *rain_uniform, *rain_pixel_u;  // [parts]
+    float *rt_u;                           // [parts]
+    float *cap_u, *cap_stage;              // [256 classes]
+    int32_t *class_part;                   // [256]: the part whose cells have the class, -1 none
     nhf_gates g;
     PartDev parts;
 };
@@ -127,6 +135,24 @@ __device__ void prepare_slot(const Dev &d, int part, int slot) {
     }
 }
 
+// The rain state a step of a uniform part advances to, by one wavefront (lane l takes classes
+// l, l + 64, ...): where the part's one pixel rains, every cell's rain time becomes rt_u + dt and
+// its capacity its class's Horton value there, the depth pass's arithmetic
+// (solver_compact.py:543-548) once per class instead of per cell. The first wavefront of the
+// part's flux pass runs it, off the path between steps; the depth pass of the same step reads it.
+// Lane 0 stores the new rain time with a store that depends on the wavefront's one load of it.
+__device__ void advance_rain(const Dev &d, int part, const Ctl *c, int lane) {
+    if (!d.rain_uniform[part] || !c->raining || !(d.rain_rates[d.rain_pixel_u[part]] > 0.0f)) return;
+    const float rt = d.rt_u[part] + c->dt;
+    for (int k = lane; k < 256; k += 64) {
+        if (d.class_part[k] != part) continue;
+        const float *hp = d.m.class_horton + 3 * k;
+        const float f0 = hp[0], fc = hp[1], hk = hp[2];
+        d.cap_u[k] = fc + (f0 - fc) * det_expf(-hk * rt / 3600.0f);
+    }
+    if (lane == 0) d.rt_u[part] = rt;
+}
+
 // --------------------------------------------------------------------------- small kernels
 
 // A part's cells [begin, padded end) and budget accumulators [first, last): the whole mesh for a
@@ -162,11 +188,38 @@ __global__ void k_reset(Dev d, int part, uint64_t now) {
         for (int i = 4 * span.x + tid; i < 4 * span.y; i += stride) d.face_qn[i] = 0.0f;
 }
 
-// The standing donor flags of every real cell, once per context (k_reset sets a part's again).
+// Per context, before k_init_flags: no class has a part yet, and every part starts in the
+// per-cell rain state.
+__global__ void k_init_classes(Dev d, int parts) {
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) d.class_part[k] = -1;
+    for (int p = threadIdx.x; p < parts; p += blockDim.x) {
+        d.rain_uniform[p] = 0;
+        d.rt_u[p] = 0.0f;
+        d.rain_pixel_u[p] = 0;
+    }
+}
+
+// The standing donor flags of every real cell, and the part of each land class, once per context
+// (k_reset sets a part's flags again).
 __global__ __launch_bounds__(NHF_BLOCK) void k_init_flags(Dev d) {
     for (int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1; cell < d.m.e_num; cell += gridDim.x * NHF_BLOCK) {
-        const bool real = !d.parts.count || cell < d.parts.range[d.parts.cell_chunk[(cell - 1) >> 10]].y;
+        const int part = d.parts.count ? d.parts.cell_chunk[(cell - 1) >> 10] : 0;
+        const bool real = !d.parts.count || cell < d.parts.range[part].y;
         d.donor_flag[cell] = real ? donor_always(d.m, d.m.cell_meta[cell], cell_record(d.m, cell)) : 0;
+        if (real) d.class_part[meta_class(d.m.cell_meta[cell])] = part;  // a class lies in one part
+    }
+}
+
+// Enter (uniform != 0) or leave the uniform rain state of one part: `rt` the rain time of every
+// cell, cap_stage the capacity of each class code (one workgroup).
+__global__ void k_set_rain_state(Dev d, int part, int uniform, float rt, int pixel) {
+    if (uniform)
+        for (int k = threadIdx.x; k < 256; k += blockDim.x)
+            if (d.class_part[k] == part) d.cap_u[k] = d.cap_stage[k];
+    if (threadIdx.x == 0) {
+        d.rain_uniform[part] = uniform ? 1 : 0;
+        d.rt_u[part] = rt;
+        d.rain_pixel_u[part] = pixel;
     }
 }
 
@@ -234,13 +287,19 @@ __global__ void k_clear_acc(Dev d, int part) {
 
 // Writes out the deferred extrema: a cell marked as rising (max_depth < 0) gets its depth as
 // maximum and its clock's current tick (the end of its last step) as peak tick.
+// In the uniform rain state it also writes out every cell's rain time and capacity.
 __global__ __launch_bounds__(NHF_BLOCK) void k_settle_extrema(Dev d) {
     const nhf_mesh &m = d.m;
     for (int cell = blockIdx.x * NHF_BLOCK + threadIdx.x + 1; cell < m.e_num; cell += gridDim.x * NHF_BLOCK) {
+        const int part = d.parts.count ? d.parts.cell_chunk[(cell - 1) >> 10] : 0;
+        if (d.parts.count && cell >= d.parts.range[part].y) continue;  // padding
         if (m.max_depth[cell] < 0.0f) {
-            const int part = d.parts.count ? d.parts.cell_chunk[(cell - 1) >> 10] : 0;
             m.max_depth[cell] = m.depth[cell];
             m.peak_tick[cell] = d.ctl[part].now;
+        }
+        if (d.rain_uniform[part]) {
+            m.rain_time[cell] = d.rt_u[part];
+            m.capacity[cell] = d.cap_u[meta_class(m.cell_meta[cell])];
         }
     }
 }
@@ -439,14 +498,16 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
     const nhf_mesh &m = d.m;
     const int block_first = xcd_block() * NHF_BLOCK * SIDES + 1;
     const Ctl *c = d.ctl;
-    int end = m.s_num;
+    int end = m.s_num, part = 0, begin = 1;
     if (PARTS) {
         if (block_first >= m.s_num) return;
-        const int part = side_part<true>(d, block_first);
+        part = side_part<true>(d, block_first);
         c = d.ctl + part;
+        begin = d.parts.range[part].z;
         end = d.parts.range[part].w;
     }
     if (!c->active) return;
+    if (block_first == begin && threadIdx.x < 64) advance_rain(d, part, c, threadIdx.x);
     const int first = block_first + threadIdx.x;
     if (first >= end) return;
     const float dt = c->dt;
@@ -704,8 +765,9 @@ struct GateSources {
 // RECORD: compute the cell's update without storing anything and return its gate source terms
 // (the gate accounting runs this between the donor and depth passes, on the same inputs).
 template <bool REGULAR, bool RECORD = false>
-static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float dt, bool raining, uint64_t step_start,
-                                                  const DepthCell &v, DepthTerms &t, GateSources *gs = nullptr) {
+static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float dt, bool raining, bool uniform,
+                                                  uint64_t step_start, const DepthCell &v, DepthTerms &t,
+                                                  GateSources *gs = nullptr) {
     const nhf_mesh &m = d.m;
     const float size = m.size_base > 0.0f ? cell_size(m, v.meta, cell) : v.size;
     CellList l;
@@ -755,9 +817,11 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
     // compensated window volume (solver_compact.py:535-539)
     const float pv_increment = pipe * dt - v.pv_correction;
     const float pv_total = v.pv_before + pv_increment;
-    // a cell with rain recomputes its capacity from its rain time; only a cell without reads it
-    const float cap = rainq > 0.0f ? fc + (f0 - fc) * det_expf(-hk * (v.rain_time + dt) / 3600.0f)
-                                   : m.capacity[cell];
+    // a cell with rain recomputes its capacity from its rain time; only a cell without reads it;
+    // in the uniform rain state the class's capacity is already this step's (advance_rain)
+    const float cap = uniform ? d.cap_u[meta_class(v.meta)]
+                              : (rainq > 0.0f ? fc + (f0 - fc) * det_expf(-hk * (v.rain_time + dt) / 3600.0f)
+                                              : m.capacity[cell]);
     float ql = 0.0f, qr = 0.0f, qb = 0.0f, qt = 0.0f;
     if (l.short_list) {
 #pragma unroll
@@ -844,7 +908,7 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
         m.pipe_correction[cell] = (pv_total - v.pv_before) - pv_increment;
         m.pipe_volume[cell] = pv_total;
     }
-    if (rainq > 0.0f) {
+    if (rainq > 0.0f && !uniform) {
         m.rain_time[cell] = v.rain_time + dt;
         m.capacity[cell] = cap;
     }
@@ -865,13 +929,14 @@ static __device__ __forceinline__ void depth_cell(const Dev &d, int cell, float 
 }
 
 // First round of the depth pass: every load that needs only the cell index.
-static __device__ __forceinline__ void load_depth_cell(const nhf_mesh &m, int cell, bool raining, DepthCell &v) {
+static __device__ __forceinline__ void load_depth_cell(const nhf_mesh &m, int cell, bool raining, bool uniform,
+                                                       DepthCell &v) {
     v.meta = m.cell_meta[cell];
     v.h = m.depth[cell];
     v.size = m.size_base > 0.0f ? 0.0f : m.size[cell];  // else computed from meta
     v.max_d = m.max_depth[cell];
     v.max_s = m.max_speed[cell];
-    v.rain_time = raining ? m.rain_time[cell] : 0.0f;
+    v.rain_time = (raining && !uniform) ? m.rain_time[cell] : 0.0f;
     v.pix = (raining && m.n_pixels > 1) ? (uint32_t)m.rain_index[cell] : 0u;
     v.pq = m.pipe_q ? m.pipe_q[cell] : 0.0f;
     if (m.pipe_volume) {
@@ -888,9 +953,10 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     Ctl *c = d.ctl;
     int end = m.e_num;
     int2 bounds = make_int2(0, m.b_num);  // the part's boundaries: first, count
+    int part = 0;
     if (PARTS) {
         if (block_first >= m.e_num) return;
-        const int part = cell_part<true>(d, block_first);
+        part = cell_part<true>(d, block_first);
         c = d.ctl + part;
         end = d.parts.range[part].y;
         const int4 e = d.parts.extra[part];
@@ -899,6 +965,7 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     if (!c->active) return;
     const float dt = c->dt;
     const bool raining = c->raining;
+    const bool uniform = d.rain_uniform[part] != 0;
     const uint64_t step_start = c->now;  // the clock commits this step's end in k_finish
     const int b_num = bounds.y;
     const int first = block_first + threadIdx.x;
@@ -910,35 +977,35 @@ __global__ __launch_bounds__(BLOCK) void k_depth(Dev d) {
     if constexpr (CPT == 1) {   // the plain form keeps every value in registers
         const bool live = first < end;
         DepthCell v = {};
-        if (live) load_depth_cell(m, first, raining, v);
+        if (live) load_depth_cell(m, first, raining, uniform, v);
         t[0] = DepthTerms{};
         t[0].bi = -1;
         t[0].kind = -1;
         if (__all(!live || v.rec.x >= 0)) {
-            if (live) depth_cell<true>(d, first, dt, raining, step_start, v, t[0]);
+            if (live) depth_cell<true>(d, first, dt, raining, uniform, step_start, v, t[0]);
         } else if (live) {
-            depth_cell<false>(d, first, dt, raining, step_start, v, t[0]);
+            depth_cell<false>(d, first, dt, raining, uniform, step_start, v, t[0]);
         }
     } else {   // two cells: both first rounds, then both bodies (named values, no arrays)
         static_assert(CPT == 2, "one or two cells per thread");
         const int second = first + BLOCK;
         const bool live0 = first < end, live1 = second < end;
         DepthCell v0 = {}, v1 = {};
-        if (live0) load_depth_cell(m, first, raining, v0);
-        if (live1) load_depth_cell(m, second, raining, v1);
+        if (live0) load_depth_cell(m, first, raining, uniform, v0);
+        if (live1) load_depth_cell(m, second, raining, uniform, v1);
         t[0] = DepthTerms{};
         t[0].bi = t[0].kind = -1;
         t[1] = DepthTerms{};
         t[1].bi = t[1].kind = -1;
         if (__all(!live0 || v0.rec.x >= 0)) {
-            if (live0) depth_cell<true>(d, first, dt, raining, step_start, v0, t[0]);
+            if (live0) depth_cell<true>(d, first, dt, raining, uniform, step_start, v0, t[0]);
         } else if (live0) {
-            depth_cell<false>(d, first, dt, raining, step_start, v0, t[0]);
+            depth_cell<false>(d, first, dt, raining, uniform, step_start, v0, t[0]);
         }
         if (__all(!live1 || v1.rec.x >= 0)) {
-            if (live1) depth_cell<true>(d, second, dt, raining, step_start, v1, t[1]);
+            if (live1) depth_cell<true>(d, second, dt, raining, uniform, step_start, v1, t[1]);
         } else if (live1) {
-            depth_cell<false>(d, second, dt, raining, step_start, v1, t[1]);
+            depth_cell<false>(d, second, dt, raining, uniform, step_start, v1, t[1]);
         }
     }
     // FP64 budget: each wavefront owns one compensated accumulator per term and adds the sum of
@@ -1081,13 +1148,14 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_gate_account(Dev d) {
             const int cell = (int)code;
             const bool raining = c->raining;
             DepthCell cv = {};
-            load_depth_cell(m, cell, raining, cv);
+            const bool uniform = d.rain_uniform[0] != 0;
+            load_depth_cell(m, cell, raining, uniform, cv);
             DepthTerms t = {};
             GateSources gs;
             if (cv.rec.x >= 0)
-                depth_cell<true, true>(d, cell, dt, raining, c->now, cv, t, &gs);
+                depth_cell<true, true>(d, cell, dt, raining, uniform, c->now, cv, t, &gs);
             else
-                depth_cell<false, true>(d, cell, dt, raining, c->now, cv, t, &gs);
+                depth_cell<false, true>(d, cell, dt, raining, uniform, c->now, cv, t, &gs);
 #pragma unroll
             for (int k = 0; k < 6; ++k) v[k] = gs.v[k];
         }
@@ -1284,7 +1352,7 @@ static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Carve {
     size_t ctl, cell_chunk, side_chunk, range, cfl_part, coeff, values, level, rain, donor, face_qn, grad, key, idx,
-        acc, donor_flag, total;
+        acc, donor_flag, rain_uniform, rain_pixel_u, rt_u, cap_u, cap_stage, class_part, total;
 };
 
 static int blocks_for(int n) { return (n + NHF_BLOCK - 1) / NHF_BLOCK; }
@@ -1409,6 +1477,12 @@ static Carve carve(const nhf_mesh *m, const nhf_params *p, const PartPlan *plan 
     c.idx = off; off = align_up(off + sizeof(int32_t) * cfl_blocks);
     c.acc = off; off = align_up(off + sizeof(double2) * (size_t)acc_owner_count(m->e_num) * terms);
     c.donor_flag = off; off = align_up(off + (size_t)m->e_num);
+    c.rain_uniform = off; off = align_up(off + sizeof(int32_t) * parts);
+    c.rain_pixel_u = off; off = align_up(off + sizeof(int32_t) * parts);
+    c.rt_u = off; off = align_up(off + sizeof(float) * parts);
+    c.cap_u = off; off = align_up(off + sizeof(float) * 256);
+    c.cap_stage = off; off = align_up(off + sizeof(float) * 256);
+    c.class_part = off; off = align_up(off + sizeof(int32_t) * 256);
     c.total = off;
     return c;
 }
@@ -1473,6 +1547,12 @@ static void bind_dev(Dev &d, const nhf_mesh *mesh, const nhf_params *params, con
     d.acc = (double2 *)(ws + cv.acc);
     d.donor_flag = (uint8_t *)(ws + cv.donor_flag);
     d.donor_keep = 0;
+    d.rain_uniform = (int32_t *)(ws + cv.rain_uniform);
+    d.rain_pixel_u = (int32_t *)(ws + cv.rain_pixel_u);
+    d.rt_u = (float *)(ws + cv.rt_u);
+    d.cap_u = (float *)(ws + cv.cap_u);
+    d.cap_stage = (float *)(ws + cv.cap_stage);
+    d.class_part = (int32_t *)(ws + cv.class_part);
     d.terms = 4 + 6 * (plan ? plan->max_b : mesh->b_num);
     d.cfl_blocks = plan ? plan->cfl_total : cfl_grid(mesh->s_num);
     d.cfl_max = plan ? plan->cfl_total : cfl_max_blocks(mesh->s_num);
@@ -1517,6 +1597,8 @@ static int32_t create_ctx(nhf_ctx **out, const nhf_mesh *mesh, const nhf_params 
     bind_dev(ctx->d, mesh, params, plan, ws, cv);
     {
         const int blocks = blocks_for(mesh->e_num);
+        hipLaunchKernelGGL(k_init_classes, dim3(1), dim3(NHF_BLOCK), 0, (hipStream_t)stream, ctx->d,
+                           plan ? (int)plan->range.size() : 1);
         hipLaunchKernelGGL(k_init_flags, dim3(blocks < 2048 ? blocks : 2048), dim3(NHF_BLOCK), 0, (hipStream_t)stream,
                            ctx->d);
         hipError_t err = hipGetLastError();
@@ -2041,6 +2123,38 @@ int32_t nhf_settle_extrema(nhf_ctx *ctx) {
     return 0;
 }
 
+// The rain state of one part (part 0 of a single mesh): the part's one rain pixel when it has
+// at most one, the capacities copied through cap_stage.
+static int32_t rain_state_part(nhf_ctx *ctx, int part, int32_t uniform, float rain_time, const float *class_capacity,
+                               const char *name) {
+    if (ctx->pending_steps) {
+        snprintf(nhf_error_text, sizeof nhf_error_text, "%s: a batch is in flight", name);
+        return -2;
+    }
+    const int pixels = ctx->parts ? ctx->part_pixels[part].y : ctx->d.m.n_pixels;
+    const int pixel = ctx->parts ? ctx->part_pixels[part].x : 0;
+    // the flux pass advances the uniform state, so a mesh without sides keeps the per-cell one
+    if (uniform && (pixels > 1 || !class_capacity || ctx->d.m.s_num <= 1)) {
+        snprintf(nhf_error_text, sizeof nhf_error_text,
+                 "%s: the uniform state needs one rain pixel, sides and the class capacities", name);
+        return -2;
+    }
+    if (uniform)
+        HIP_TRY(hipMemcpyAsync(ctx->d.cap_stage, class_capacity, sizeof(float) * 256, hipMemcpyHostToDevice,
+                               ctx->stream));
+    hipLaunchKernelGGL(k_set_rain_state, dim3(1), dim3(NHF_BLOCK), 0, ctx->stream, ctx->d, part, (int)(uniform != 0),
+                       rain_time, pixel);
+    HIP_TRY(hipGetLastError());
+    // the host's capacities are read by the copy before this returns
+    if (uniform) HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int32_t nhf_set_rain_state(nhf_ctx *ctx, int32_t uniform, float rain_time, const float *class_capacity) {
+    if (int32_t rc = single_only(ctx, "nhf_set_rain_state")) return rc;
+    return rain_state_part(ctx, 0, uniform, rain_time, class_capacity, "nhf_set_rain_state");
+}
+
 int32_t nhf_set_graph(nhf_ctx *ctx, int32_t enable) {
     if (!ctx) return set_error_text("nhf_set_graph: null context");
     ctx->graph_enabled = enable != 0;
@@ -2200,6 +2314,8 @@ int32_t nhf_run_static(const nhf_mesh *mesh, const nhf_params *params, uint64_t 
     double *totals = (double *)(ws + align_up(cv.total));
     const int terms = ctx.d.terms;
     int32_t rc = 0;
+    // the per-cell rain state: the scratch holds whatever the last call left
+    hipLaunchKernelGGL(k_set_rain_state, dim3(1), dim3(NHF_BLOCK), 0, ctx.stream, ctx.d, 0, 0, 0.0f, 0);
     if ((rc = reset_part(&ctx, 0, now_ticks)) || (rc = forcing_part(&ctx, 0, coefficients, rain_rates, raining)) ||
         (rc = compute_cfl_any(&ctx)) || (rc = begin_part(&ctx, 0, target_ticks, origin_ticks)) ||
         (rc = launch_batch(&ctx, steps, false)))
@@ -2290,6 +2406,11 @@ int32_t nhu_limit_capture(nhf_ctx *ctx, int32_t part, float *values, int32_t *id
 int32_t nhu_cfl_state(nhf_ctx *ctx, int32_t part, float *ndt, int32_t *side) {
     if (int32_t rc = part_only(ctx, part, "nhu_cfl_state")) return rc;
     return cfl_state_part(ctx, part, ndt, side);
+}
+
+int32_t nhu_set_rain_state(nhf_ctx *ctx, int32_t part, int32_t uniform, float rain_time, const float *class_capacity) {
+    if (int32_t rc = part_only(ctx, part, "nhu_set_rain_state")) return rc;
+    return rain_state_part(ctx, part, uniform, rain_time, class_capacity, "nhu_set_rain_state");
 }
 
 }  // extern "C"

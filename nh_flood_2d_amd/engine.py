@@ -16,6 +16,23 @@ from .layout import Layout
 
 STATE_CELLS = ('depth', 'bed', 'rain_time', 'capacity', 'max_depth', 'max_speed', 'peak_tick')
 STATE_SIDES = ('q',)
+# arrays the step may hold back on the device until nhf_settle_extrema writes them out
+SETTLED = ('max_depth', 'peak_tick', 'rain_time', 'capacity')
+
+
+def uniform_rain(rain_time: np.ndarray, capacity: np.ndarray, classes: np.ndarray, n_pixels: int):
+    """Whether cells with these rain times, capacities and land class codes (the real cells of one
+    mesh, any order) can keep the uniform rain state: at most one rain pixel, one rain time, one
+    capacity per class, compared bit for bit. Returns (uniform, rain time, capacity table [256])."""
+    rt = np.asarray(rain_time, np.float32)
+    cap = np.asarray(capacity, np.float32)
+    cls = np.asarray(classes, np.int64)
+    table = np.zeros(256, dtype=np.float32)
+    table[cls] = cap
+    uniform = (n_pixels <= 1 and rt.size > 0
+               and bool(np.all(rt.view(np.uint32) == rt.view(np.uint32)[0]))
+               and np.array_equal(table[cls].view(np.uint32), cap.view(np.uint32)))
+    return uniform, (float(rt[0]) if uniform else 0.0), table
 
 
 def check_extrema(name: str, values) -> None:
@@ -160,6 +177,7 @@ class Engine:
         self.ctx = handle
         self.terms = 4 + 6 * layout.b_num
         self._trace = np.zeros((abi.MAX_BATCH, abi.TRACE_COLS), dtype=np.uint32)
+        self._rain_state()
 
     def close(self):
         if getattr(self, 'ctx', None):
@@ -178,7 +196,7 @@ class Engine:
         abi.check(self.lib, self.lib.nhf_settle_extrema(self.ctx), 'nhf_settle_extrema')
 
     def read_state(self, name: str) -> np.ndarray:
-        if name in ('max_depth', 'peak_tick'):
+        if name in SETTLED:
             self.settle_extrema()
         dtype = np.uint64 if name == 'peak_tick' else np.float32
         values = self.device.read(self.buffers[name], dtype)
@@ -187,7 +205,7 @@ class Engine:
         return self.layout.cells_from_device(values)
 
     def write_state(self, name: str, values: np.ndarray) -> None:
-        if name in ('depth', 'max_depth', 'peak_tick'):
+        if name in SETTLED or name == 'depth':
             self.settle_extrema()     # a marked cell's maximum is its depth: keep the pair final
         dtype = np.uint64 if name == 'peak_tick' else np.float32
         values = np.asarray(values, dtype=dtype)
@@ -197,6 +215,19 @@ class Engine:
         else:
             values = self.layout.cells_to_device(values)
         self.device.write(self.buffers[name], values)
+        if name in ('rain_time', 'capacity'):
+            self._rain_state()
+
+    def _rain_state(self) -> None:
+        """The rain state after the rain time or capacity was written (nhf_set_rain_state):
+        uniform when uniform_rain allows it, else per cell."""
+        rt = self.device.read(self.buffers['rain_time'], np.float32)[1:]
+        cap = self.device.read(self.buffers['capacity'], np.float32)[1:]
+        cls = self.layout.arrays['cell_meta'][1:].astype(np.int64) & 255
+        uniform, rt0, table = uniform_rain(rt, cap, cls, self.layout.n_pixels)
+        rc = self.lib.nhf_set_rain_state(self.ctx, int(uniform), ctypes.c_float(rt0),
+                                         table.ctypes.data if uniform else None)
+        abi.check(self.lib, rc, 'nhf_set_rain_state')
 
     # ----------------------------------------------------------------- gate control
     def set_gates(self, spec: dict) -> None:

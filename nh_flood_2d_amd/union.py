@@ -22,7 +22,7 @@ from time import perf_counter
 import numpy as np
 
 from . import abi
-from .engine import STATE_CELLS, STATE_SIDES, TorchDevice, check_extrema
+from .engine import SETTLED, STATE_CELLS, STATE_SIDES, TorchDevice, check_extrema, uniform_rain
 from .layout import META_LEVEL_SHIFT, Layout
 
 ALIGN = abi.PART_ALIGN
@@ -332,7 +332,7 @@ class PartView:
         return buf[self.s0:self.s0 + self.ns] if name in STATE_SIDES else buf[self.c0:self.c0 + self.nc]
 
     def read_state(self, name: str) -> np.ndarray:
-        if name in ('max_depth', 'peak_tick'):
+        if name in SETTLED:
             self.union.settle_extrema()
         dev = self.union.device
         dev.stream.synchronize()
@@ -346,7 +346,7 @@ class PartView:
         return self.layout.cells_from_device(local)
 
     def write_state(self, name: str, values: np.ndarray) -> None:
-        if name in ('depth', 'max_depth', 'peak_tick'):
+        if name in SETTLED or name == 'depth':
             self.union.settle_extrema()
         dtype = np.uint64 if name == 'peak_tick' else np.float32
         values = np.asarray(values, dtype=dtype)
@@ -360,6 +360,21 @@ class PartView:
         with dev.torch.cuda.stream(dev.stream):
             self._slice(name).copy_(dev.torch.from_numpy(np.ascontiguousarray(host)).to(dev.device))
         dev.stream.synchronize()
+        if name in ('rain_time', 'capacity'):
+            self._rain_state()
+
+    def _rain_state(self) -> None:
+        """The part's rain state after its rain time or capacity was written (nhu_set_rain_state),
+        its class codes those of the union's cell_meta."""
+        dev = self.union.device
+        with dev.torch.cuda.stream(dev.stream):
+            rt = self._slice('rain_time').cpu().numpy()
+            cap = self._slice('capacity').cpu().numpy()
+        meta = self.union.plan.arrays['cell_meta'][self.c0:self.c0 + self.nc]
+        uniform, rt0, table = uniform_rain(rt, cap, meta.astype(np.int64) & 255, self.layout.n_pixels)
+        rc = self.lib.nhu_set_rain_state(self.union.ctx, self.part, int(uniform), ctypes.c_float(rt0),
+                                         table.ctypes.data if uniform else None)
+        abi.check(self.lib, rc, 'nhu_set_rain_state')
 
     # ------------------------------------------------------------------ per-part protocol
     def reset(self, now_ticks: int) -> None:

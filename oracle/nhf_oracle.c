@@ -1084,6 +1084,14 @@ int32_t nhf_settle_extrema(nhf_ctx *c) {
     (void)c;
     return 0;
 }
+/* The oracle always keeps the per-cell rain time and capacity arrays. */
+int32_t nhf_set_rain_state(nhf_ctx *c, int32_t uniform, float rain_time, const float *class_capacity) {
+    (void)c;
+    (void)uniform;
+    (void)rain_time;
+    (void)class_capacity;
+    return 0;
+}
 int32_t nhf_set_graph(nhf_ctx *c, int32_t enable) {
     (void)c;
     (void)enable;

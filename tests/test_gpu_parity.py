@@ -253,3 +253,45 @@ def test_donor_limiting_matches_oracle(gpu_device, theta):
         faces += int(g[0, 7])
     _say(f'[limiter] theta {theta}: {limited} limited cells, {faces} limited faces over 50 steps')
     assert limited > 0 and faces > 0
+
+
+def test_rain_state_follows_the_written_arrays(gpu_device):
+    """With one rain pixel the device keeps one rain time and one capacity per land class while
+    every cell agrees (the uniform rain state), and the per-cell arrays otherwise; the oracle
+    always keeps the per-cell arrays. Uniform from the cold start, per cell after a rain time that
+    differs between cells is written, uniform again after one rain time and one capacity per class
+    are written: in every state each batch's trace and every state array, rain time and capacity
+    included, are the oracle's bits."""
+    sc = synth.small_multires(22, duration=900., discharge=1.0, pond=0.25)
+    pair = Pair(sc, gpu_device)
+    assert pair.layout.n_pixels == 1
+
+    def advance(batches):
+        for _ in range(batches):
+            g, c = pair.adaptive_batch(pair.now / 2 ** 40 + 30.)
+            np.testing.assert_array_equal(g, c)
+        for name in ('depth', 'q', 'rain_time', 'capacity', 'max_depth', 'max_speed', 'peak_tick'):
+            a, b = pair.gpu.read_state(name), pair.cpu.read_state(name)
+            np.testing.assert_array_equal(a.view(np.uint8), b.view(np.uint8), err_msg=name)
+
+    def write_both(name, values):
+        for e in pair.engines:
+            e.write_state(name, values)
+        np.testing.assert_array_equal(pair.gpu.read_state(name), values, err_msg=f'{name} read back')
+
+    _say('[rain state] uniform from the cold start')
+    advance(4)
+    rt = pair.gpu.read_state('rain_time')
+    assert rt[1:].min() == rt[1:].max() > 0
+    _say('[rain state] per cell after a rain time that differs between cells')
+    uneven = rt.copy()
+    uneven[1::3] += np.float32(7.5)
+    write_both('rain_time', uneven)
+    advance(4)
+    assert pair.gpu.read_state('rain_time')[1:].max() > pair.gpu.read_state('rain_time')[1:].min()
+    _say('[rain state] uniform again after one rain time and per-class capacities')
+    write_both('rain_time', np.full_like(rt, np.float32(40.)))
+    cap = pair.layout.cell_f0.astype(np.float32) * np.float32(0.5)
+    cap[0] = 0.
+    write_both('capacity', cap)
+    advance(4)

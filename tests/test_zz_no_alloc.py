@@ -41,6 +41,10 @@ class Twin:
         self.device = device
         self.ctx = Engine(self.layout, self.params, state, device=device)
         self.bare = Engine(self.layout, self.params, state, device=device)
+        # nhf_run_static keeps the rain time and capacity per cell in the bare arrays: the bare
+        # context must not hold a uniform rain state over them (its settle would write it out)
+        abi.check(device.lib, device.lib.nhf_set_rain_state(self.bare.ctx, 0, ctypes.c_float(0.), None),
+                  'nhf_set_rain_state')
         self.now = 0
 
     def forcing_at(self, t):

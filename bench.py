@@ -107,10 +107,11 @@ def algorithmic_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
     return {'step': a + b, 'flux': a, 'cell_update': b}
 
 
-def launch_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
+def launch_bytes(layout, e_num: int, s_num: int, r_len: int, uniform_rain: bool = False) -> dict:
     """Bytes each launch itself must move in this layout (its inputs read once, its outputs
     written once, conditional extrema writes excluded): Nc cells, Ns sides, R side-list entries,
-    E hanging edges, G gradient cells."""
+    E hanging edges, G gradient cells. In the uniform rain state (one rain pixel, one rain time)
+    the depth pass holds the rain time and capacities per land class instead of per cell."""
     nc, ns, r = e_num - 1, s_num - 1, r_len
     e_hang, g = layout.n_hang, layout.n_grad
     pixel = 4 if layout.n_pixels > 1 else 0
@@ -121,9 +122,10 @@ def launch_bytes(layout, e_num: int, s_num: int, r_len: int) -> dict:
         # the donor certificate leaves the donor pass one flag byte per cell to read; the cells
         # it looks at (flagged, boundary) and their rewrites are excluded
         'donor': nc,
-        # per cell: meta 4, side record 16, depth r+w 8, capacity r+w 8, rain time r+w 8, max
-        # speed 4, max depth 4 (+ rain pixel); per side: q 4 (cell sizes come from the meta word)
-        'depth': (52 + pixel) * nc + 4 * ns,
+        # per cell: meta 4, side record 16, depth r+w 8, max speed 4, max depth 4 (+ rain pixel),
+        # capacity r+w 8 and rain time r+w 8 unless the uniform rain state holds them; per side:
+        # q 4 (cell sizes come from the meta word)
+        'depth': (36 + (0 if uniform_rain else 16) + pixel) * nc + 4 * ns,
         # per side: side_cells 8, length 4, q 4 (reference ids only on ties); per cell: bed +
         # depth 8
         'cfl': 16 * ns + 8 * nc,
@@ -1003,7 +1005,7 @@ def main():
     us = eng.time_kernels(args.kernel_repeats)
     ms = us * 1e-3 * args.kernel_repeats
     count = np.where(us > 0, args.kernel_repeats, 0)
-    nbytes = launch_bytes(layout, mesh.e_num, mesh.s_num, r_len)
+    nbytes = launch_bytes(layout, mesh.e_num, mesh.s_num, r_len, uniform_rain=eng.rain_uniform)
     alg = algorithmic_bytes(layout, mesh.e_num, mesh.s_num, r_len)
     traffic = read_traffic(cells)
     kernels = {}

@@ -301,6 +301,7 @@ class ScenarioView:
         rt = self._column(b.state['rain_time']).contiguous().cpu().numpy()[1:]
         cap = self._column(b.state['capacity']).contiguous().cpu().numpy()[1:]
         uniform, rt0, table = uniform_rain(rt, cap, b.cell_class[1:], b.layout.n_pixels)
+        uniform = uniform and b.layout.s_num > 1   # the flux pass advances the uniform state
         rc = self.lib.nhb_set_rain_state(b.ctx, self.slot, int(uniform), ctypes.c_float(rt0),
                                          table.ctypes.data if uniform else None)
         abi.check(self.lib, rc, 'nhb_set_rain_state')

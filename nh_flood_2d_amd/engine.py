@@ -225,9 +225,11 @@ class Engine:
         cap = self.device.read(self.buffers['capacity'], np.float32)[1:]
         cls = self.layout.arrays['cell_meta'][1:].astype(np.int64) & 255
         uniform, rt0, table = uniform_rain(rt, cap, cls, self.layout.n_pixels)
+        uniform = uniform and self.layout.s_num > 1   # the flux pass advances the uniform state
         rc = self.lib.nhf_set_rain_state(self.ctx, int(uniform), ctypes.c_float(rt0),
                                          table.ctypes.data if uniform else None)
         abi.check(self.lib, rc, 'nhf_set_rain_state')
+        self.rain_uniform = uniform
 
     # ----------------------------------------------------------------- gate control
     def set_gates(self, spec: dict) -> None:

@@ -372,6 +372,7 @@ class PartView:
             cap = self._slice('capacity').cpu().numpy()
         meta = self.union.plan.arrays['cell_meta'][self.c0:self.c0 + self.nc]
         uniform, rt0, table = uniform_rain(rt, cap, meta.astype(np.int64) & 255, self.layout.n_pixels)
+        uniform = uniform and self.layout.s_num > 1   # the flux pass advances the uniform state
         rc = self.lib.nhu_set_rain_state(self.union.ctx, self.part, int(uniform), ctypes.c_float(rt0),
                                          table.ctypes.data if uniform else None)
         abi.check(self.lib, rc, 'nhu_set_rain_state')

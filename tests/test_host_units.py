@@ -551,3 +551,27 @@ def test_negative_max_depth_is_rejected():
     with pytest.raises(ValueError, match='max_depth'):
         eng.write_state('max_depth', np.full(sc.mesh.e_num, -2., np.float32))
     eng.close()
+
+
+def test_uniform_rain_needs_one_pixel_one_rain_time_and_one_capacity_per_class():
+    """The host enters the uniform rain state only when the arrays allow it, compared bit for bit:
+    one rain pixel, every cell the same rain time, every cell of a class the same capacity."""
+    from nh_flood_2d_amd.engine import uniform_rain
+    cls = np.array([0, 3, 3, 7, 0], np.int64)
+    rt = np.full(5, 12.5, np.float32)
+    cap = np.array([1e-5, 2e-5, 2e-5, 3e-5, 1e-5], np.float32)
+    ok, rt0, table = uniform_rain(rt, cap, cls, 1)
+    assert ok and rt0 == 12.5
+    np.testing.assert_array_equal(table[[0, 3, 7]], np.array([1e-5, 2e-5, 3e-5], np.float32))
+    assert uniform_rain(rt, cap, cls, 0)[0]                        # no raster: one pixel
+    assert not uniform_rain(rt, cap, cls, 2)[0]                    # two pixels rain differently
+    uneven = rt.copy()
+    uneven[2] = np.nextafter(uneven[2], np.float32(np.inf))
+    assert not uniform_rain(uneven, cap, cls, 1)[0]                # one ulp apart
+    signed = np.zeros(5, np.float32)
+    signed[4] = -0.0
+    assert not uniform_rain(signed, cap, cls, 1)[0]                # +0 and -0 differ in bits
+    split = cap.copy()
+    split[2] = np.float32(2.5e-5)
+    assert not uniform_rain(rt, split, cls, 1)[0]                  # class 3 holds two capacities
+    assert not uniform_rain(np.zeros(0, np.float32), np.zeros(0, np.float32), cls[:0], 1)[0]

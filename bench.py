@@ -139,7 +139,8 @@ TRAFFIC_SUMMARY = ROOT / 'profiles' / 'hbm_traffic_multires_channel.txt'
 PMC_KERNELS = {'k_gradients': 'gradients', 'k_face_avg': 'face_avg', 'k_flux': 'flux', 'k_donor': 'donor',
                'k_depth': 'depth', 'k_cfl': 'cfl', 'k_finish': 'finish',
                'kb_gradients': 'batch_gradients', 'kb_face_avg': 'batch_face_avg', 'kb_flux': 'batch_flux',
-               'kb_donor': 'batch_donor', 'kb_depth': 'batch_depth', 'kb_cfl': 'batch_cfl', 'kb_finish': 'batch_finish'}
+               'kb_donor': 'batch_donor', 'kb_donor_wide': 'batch_donor', 'kb_depth': 'batch_depth',
+               'kb_cfl': 'batch_cfl', 'kb_finish': 'batch_finish'}
 
 
 def kernel_base_name(name: str) -> str:
@@ -244,7 +245,7 @@ def trace_summary(trace_csv: str, bench_json: str) -> str:
               'Average timing pass (us) | HIP-event average (us) |', '| --- | ---: | ---: | ---: | ---: | ---: |']
     for name in batch:
         d = np.array(every[name]) / 1e3
-        short = name[3:]
+        short = name[3:].replace('_wide', '')   # kb_donor_wide: the 64-slot donor pass
         n = bk.get(short, {}).get('launches', 0)
         tail = d[-n:] if n else d[:0]
         ev = bk.get(short, {}).get('avg_us')

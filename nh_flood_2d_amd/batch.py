@@ -21,7 +21,7 @@ from time import perf_counter
 import numpy as np
 
 from . import abi
-from .engine import SETTLED, STATE_CELLS, STATE_SIDES, TorchDevice, uniform_rain
+from .engine import SETTLED, STATE_CELLS, STATE_SIDES, TorchDevice, check_extrema, uniform_rain
 from .layout import Layout, meta_boundary
 
 # per-slot cell state of a batch; the bed is shared
@@ -288,6 +288,7 @@ class ScenarioView:
             return
         if name in SETTLED or name == 'depth':
             self.batch.settle(self.slot)   # a marked cell's maximum is its depth: keep the pair final
+        check_extrema(name, values)
         values = self.layout.sides_to_device(values) if name in STATE_SIDES else self.layout.cells_to_device(values)
         self._write_column(self.batch.state[name], values)
         if name in ('rain_time', 'capacity'):

@@ -642,6 +642,60 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     }
 }
 
+// The 64-slot donor pass over a block of 64 consecutive cells per wavefront: lane i loads the
+// flag of cell `base + i` (one 64-byte load for the block), a ballot gives the flagged cells as a
+// wave-uniform mask, and the wavefront then takes them K at a time, each cell with its 64 slots
+// on the 64 lanes, both load rounds of the K cells before their arithmetic (as kb_donor). Few
+// cells are flagged, so most wavefronts end after the one load: the pass launches a 64th of the
+// wavefronts of one cell per wavefront. Every lane stays live to the end (the flags are read
+// back across lanes); lanes of a slot that does not step load and store nothing.
+template <int K, int B>
+__global__ __launch_bounds__(NHF_BLOCK) void kb_donor_wide(BDev d) {
+    const int sh = 6;
+    const int s = (int)(threadIdx.x & 63u);
+    const int base = __builtin_amdgcn_readfirstlane(xcd_block() * NHF_WAVES + (int)(threadIdx.x >> 6)) * B + 1;
+    const int e_num = d.m.e_num;
+    if (base >= e_num) return;
+    const int own = base + s;
+    const int fl = (s < B && own < e_num) ? (int)d.donor_flag[own] : 0;
+    uint64_t mask = __ballot(fl != 0);
+    const float dt = d.dt[s];
+    const bool on = dt != 0.0f;
+    Ctl *c = &d.ctl[s];
+    while (mask) {
+        int cell[K], flag[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            cell[k] = 0;
+            flag[k] = 0;
+            if (mask) {
+                const int b = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                cell[k] = base + b;
+                flag[k] = __builtin_amdgcn_readlane(fl, b);
+            }
+        }
+        BDonor v[K];
+        if (on) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (cell[k]) bdonor_first(d, s, sh, cell[k], v[k]);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (cell[k]) bdonor_second(d, s, sh, cell[k], v[k]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (cell[k]) {
+                    bdonor_finish(d, s, sh, cell[k], dt, c, v[k]);
+                    const uint8_t standing =
+                        (meta_boundary(v[k].meta) >= 0 || (v[k].rec.x < 0 && v[k].rec.w > NHF_LIST_MAX)) ? 2 : 0;
+                    if (flag[k] != standing) d.donor_flag[cell[k]] = standing;
+                }
+            }
+        }
+    }
+}
+
 // --------------------------------------------------------------------------- (b) depth update
 
 // Grid-stride over (cell, slot): the stride is a multiple of the slot count, so each thread
@@ -1137,14 +1191,17 @@ static size_t align_b(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static int b_blocks(size_t n) { return (int)((n + NHF_BLOCK - 1) / NHF_BLOCK); }
 
-// Sides per wavefront of the 64-slot flux pass and cells per wavefront of its donor pass
-// (measured on the 1 M-cell batch mesh: flux 821 / 609 / 533 / 579 us for 1 / 2 / 4 / 8 sides,
-// donor 445 / 333 / 284 us for 1 / 2 / 4 cells).
+// Sides per wavefront of the 64-slot flux pass (measured on the 1 M-cell batch mesh: flux 821 /
+// 609 / 533 / 579 us for 1 / 2 / 4 / 8 sides) and flagged cells per load round of its donor pass.
 #ifndef NHB_FLUX_SIDES
 #define NHB_FLUX_SIDES 4
 #endif
 #ifndef NHB_DONOR_CELLS
-#define NHB_DONOR_CELLS 4
+#define NHB_DONOR_CELLS 2
+#endif
+// Cells whose flags one wavefront of the 64-slot donor pass scans (at most 64).
+#ifndef NHB_DONOR_BLOCK
+#define NHB_DONOR_BLOCK 16
 #endif
 // Grid strides per load round of the 64-slot depth pass (1 / 2 / 3: 559 / 745 / 966 us; two or
 // more lower the occupancy) and of its CFL pass.
@@ -1272,9 +1329,11 @@ static int32_t b_launch_batch(nhb_ctx *ctx, int steps, bool timed) {
             BLAUNCH(KB_FLUX, (w ? kb_flux<true, NHB_FLUX_SIDES> : kb_flux<false, 1>),
                     dim3(b_blocks_xcd((w ? (sides + NHB_FLUX_SIDES - 1) / NHB_FLUX_SIDES : sides) * S)), dim3(NHF_BLOCK), 0, s,
                     d);
-        BLAUNCH(KB_DONOR, (w ? kb_donor<true, NHB_DONOR_CELLS> : kb_donor<false, 1>),
-                dim3(b_blocks_xcd((w ? (cells + NHB_DONOR_CELLS - 1) / NHB_DONOR_CELLS : cells) * S)), dim3(NHF_BLOCK), 0, s,
-                d);
+        if (w)
+            BLAUNCH(KB_DONOR, (kb_donor_wide<NHB_DONOR_CELLS, NHB_DONOR_BLOCK>),
+                    dim3(b_blocks_xcd((cells + NHB_DONOR_BLOCK - 1) / NHB_DONOR_BLOCK * 64)), dim3(NHF_BLOCK), 0, s, d);
+        else
+            BLAUNCH(KB_DONOR, (kb_donor<false, 1>), dim3(b_blocks_xcd(cells * S)), dim3(NHF_BLOCK), 0, s, d);
         BLAUNCH(KB_DEPTH, (w ? kb_depth<true, NHB_DEPTH_ROUND> : kb_depth<false, 1>), dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
         BLAUNCH(KB_CFL, (w ? kb_cfl<true, NHB_CFL_ROUND> : kb_cfl<false, 1>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
         BLAUNCH(KB_FINISH, kb_finish, dim3((unsigned)S), dim3(NHF_BLOCK), 0, s, d, 1, step, steps);

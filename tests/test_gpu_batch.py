@@ -133,6 +133,49 @@ def test_batch_matches_each_scenario_alone(gpu_device, case):
     batch.close()
 
 
+def test_batch_donor_limiting_matches_each_scenario_alone(gpu_device):
+    """Thin sheets of water on the multi-resolution mesh's slopes drain faster than some cells
+    hold, in the 64-slot batch: its donor pass scans the flags of 16 cells per wavefront and takes
+    the flagged ones two at a time, in several rounds where flagged cells cluster. Each scenario
+    (sheet depth, sheet extent and Manning factor drawn per scenario) must take the dt sequence,
+    final state and mass balance of its run alone, limiter counts included, and the limiter must
+    have acted in every scenario."""
+    base = synth.small_multires(11, duration=240., discharge=1.0)
+    rng = np.random.default_rng(23)
+    variants = []
+    for i in range(6):
+        sheet = np.float32(rng.uniform(0.021, 0.032))
+        below = np.float32(rng.uniform(2.5, 4.0))
+        depth = np.where(base.mesh.z < below, sheet, np.float32(0.)).astype(np.float32)
+        depth[0] = 0.
+        factor = float(np.exp(rng.normal(0., 0.3)))
+        land = {k: dict(v, manning_n=float(np.float32(v['manning_n'] * factor)))
+                for k, v in base.land_parameters.items()}
+        variants.append(dataclasses.replace(base, name=f'sheet_v{i}', land_parameters=land, initial_depth=depth))
+    _say(f'[batch] thin sheets: {base.mesh.e_num - 1} cells, {len(variants)} scenarios, 64 slots; alone runs')
+    alone = [_run_alone(v, gpu_device, 60.) for v in variants]
+    _say('[batch] thin sheets: batched run')
+    batch = BatchEngine(64, device=gpu_device)
+    sims = {}
+    runner = BatchRunner(batch, batch_size=32)
+    for i, v in enumerate(variants):
+        def make(factory, i=i, v=v):
+            sims[i] = synth.simulation(v, engine_factory=factory, yield_step=60.)
+            return sims[i]
+        runner.submit(i, make)
+    while runner.step():
+        pass
+    _compare(variants, alone, runner.results, sims)
+    for i, (_, _, state, _) in enumerate(alone):
+        got = sims[i].state()
+        for name, values in state.items():
+            np.testing.assert_array_equal(got[name], values, err_msg=f'thin sheet scenario {i} {name}')
+    cells = [sims[i].record['limiter_activity']['limited_cell_steps'] for i in range(len(variants))]
+    _say(f'[batch] thin sheets: limited cell-steps per scenario {cells}')
+    assert min(cells) > 0
+    batch.close()
+
+
 def test_batch_graph_replays_match_eager(gpu_device):
     base = synth.small_multires(6, duration=300., discharge=1.0)
     variants = synth.scenario_variants(base, 2, seed=10)

@@ -280,10 +280,11 @@ __device__ __forceinline__ void face_difference_b(const BDev &d, int sh, int s, 
     usable = ok;
 }
 
+// The hanging-edge gradients of flat thread index t (one gradient cell and slot), for the step
+// about to run: from the depth and bed the previous step ended with.
 template <bool W>
-__global__ __launch_bounds__(NHF_BLOCK) void kb_gradients(BDev d) {
+__device__ __forceinline__ void gradient_b(const BDev &d, size_t t) {
     const int sh = W ? 6 : d.shift;
-    size_t t = (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x;
     int g = entity_of<W>(t, sh);
     int s = (int)(t & ((1u << sh) - 1));
     if (g >= d.m.n_grad || d.dt[s] == 0.0f) return;
@@ -307,6 +308,11 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_gradients(BDev d) {
         }
         d.grad[IX(2 * g + axis)] = value;
     }
+}
+
+template <bool W>
+__global__ __launch_bounds__(NHF_BLOCK) void kb_gradients(BDev d) {
+    gradient_b<W>(d, (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x);
 }
 
 // --------------------------------------------------------------------------- theta < 1
@@ -951,8 +957,19 @@ __global__ __launch_bounds__(NHF_BLOCK) __attribute__((amdgpu_waves_per_eu(W ? 8
 // U strides per round: the side records of all U sides first, then their bed, depth, length,
 // discharge and id loads, then the bounds (the minimum of the total order does not depend on the
 // order the sides are taken in).
+// The first `grad_blocks` workgroups (a multiple of 8, so each CFL workgroup keeps its XCD)
+// compute the next step's hanging-edge gradients instead: they need only the depth and bed this
+// step ends with, final once the depth pass is done. They are computed for the slots that stepped
+// in this step; a slot that stops stepping does not step again before the next batch, whose first
+// step has a gradient launch of its own.
 template <bool W, int U>
-__global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
+__global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode, int grad_blocks) {
+    if ((int)blockIdx.x < grad_blocks) {
+        gradient_b<W>(d, (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x);
+        return;
+    }
+    const int block = (int)blockIdx.x - grad_blocks, blocks = (int)gridDim.x - grad_blocks;
+    const int xb = (block & 7) * (blocks >> 3) + (block >> 3);  // the XCD-contiguous order
     const int sh = W ? 6 : d.shift;
     const int S = 1 << sh;
     const int s = threadIdx.x & (S - 1);
@@ -963,7 +980,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
     if (on) {
         const float afa = d.p.afa, g = d.p.g;
         const size_t total = (size_t)(m.s_num - 1) << sh;
-        const size_t stride = (size_t)gridDim.x * NHF_BLOCK;
+        const size_t stride = (size_t)blocks * NHF_BLOCK;
         // the cell pairs of a round are loaded during the round before (NHB_CFL_PREFETCH), so
         // each round waits for one load latency instead of two
         int si[U];
@@ -976,7 +993,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
                 cells[u] = live ? reinterpret_cast<const int2 *>(m.side_cells)[ids[u]] : make_int2(0, 0);
             }
         };
-        size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
+        size_t t = (size_t)xb * NHF_BLOCK + threadIdx.x;
         if (NHB_CFL_PREFETCH) records(t, si, lh);
         for (; t < total; t += U * stride) {
             if (!NHB_CFL_PREFETCH) records(t, si, lh);
@@ -1040,8 +1057,8 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_cfl(BDev d, int mode) {
                 key = s_key[w][threadIdx.x];
                 idx = s_idx[w][threadIdx.x];
             }
-        d.cfl_key[(size_t)blockIdx.x * S + s] = key;
-        d.cfl_idx[(size_t)blockIdx.x * S + s] = idx;
+        d.cfl_key[(size_t)block * S + s] = key;
+        d.cfl_idx[(size_t)block * S + s] = idx;
     }
 }
 
@@ -1318,8 +1335,11 @@ static int32_t b_launch_batch(nhb_ctx *ctx, int steps, bool timed) {
     const size_t cells = (size_t)(d.m.e_num - 1), sides = (size_t)(d.m.s_num - 1);
     BLAUNCH(KB_PREPARE, kb_prepare, dim3((unsigned)S), dim3(64), 0, s, d, steps);
     const bool w = S == 64;
+    const int grad_blocks = d.m.n_grad > 0 ? b_blocks_xcd((size_t)d.m.n_grad * S) : 0;
     for (int step = 0; step < steps; ++step) {
-        if (d.m.n_grad > 0)
+        // gradients: a launch of their own at the batch's first step (the host may have written
+        // the state); the previous step's CFL launch computes them for every later step
+        if (d.m.n_grad > 0 && step == 0)
             BLAUNCH(KB_GRAD, w ? kb_gradients<true> : kb_gradients<false>, dim3(b_blocks((size_t)d.m.n_grad * S)),
                     dim3(NHF_BLOCK), 0, s, d);
         if (!d.theta_one)
@@ -1335,7 +1355,9 @@ static int32_t b_launch_batch(nhb_ctx *ctx, int steps, bool timed) {
         else
             BLAUNCH(KB_DONOR, (kb_donor<false, 1>), dim3(b_blocks_xcd(cells * S)), dim3(NHF_BLOCK), 0, s, d);
         BLAUNCH(KB_DEPTH, (w ? kb_depth<true, NHB_DEPTH_ROUND> : kb_depth<false, 1>), dim3(d.depth_blocks), dim3(NHF_BLOCK), 0, s, d);
-        BLAUNCH(KB_CFL, (w ? kb_cfl<true, NHB_CFL_ROUND> : kb_cfl<false, 1>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, s, d, 1);
+        const int gb = step + 1 < steps ? grad_blocks : 0;
+        BLAUNCH(KB_CFL, (w ? kb_cfl<true, NHB_CFL_ROUND> : kb_cfl<false, 1>), dim3(gb + d.cfl_blocks), dim3(NHF_BLOCK), 0,
+                s, d, 1, gb);
         BLAUNCH(KB_FINISH, kb_finish, dim3((unsigned)S), dim3(NHF_BLOCK), 0, s, d, 1, step, steps);
     }
 #undef BLAUNCH
@@ -1640,9 +1662,9 @@ int32_t nhb_compute_cfl(nhb_ctx *ctx) {
     int32_t rc = apply_stage(ctx);
     if (rc) return rc;
     if (ctx->slots == 64)
-        hipLaunchKernelGGL((kb_cfl<true, NHB_CFL_ROUND>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+        hipLaunchKernelGGL((kb_cfl<true, NHB_CFL_ROUND>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0);
     else
-        hipLaunchKernelGGL((kb_cfl<false, 1>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0);
+        hipLaunchKernelGGL((kb_cfl<false, 1>), dim3(d.cfl_blocks), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0);
     hipLaunchKernelGGL(kb_finish, dim3(ctx->slots), dim3(NHF_BLOCK), 0, ctx->stream, d, 0, 0, 0);
     HIP_TRY(hipGetLastError());
     return 0;

@@ -1072,13 +1072,23 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_finish(BDev d, int mode, int ste
     if (mode == 1 && d.dt[s] == 0.0f) return;
     __shared__ uint64_t s_key[NHF_WAVES];
     __shared__ int32_t s_idx[NHF_WAVES];
+    // Keys only, four partials per round (every key is unique: it holds the side's reference id),
+    // remembering which partial holds the minimum; its side index is read once at the end.
     uint64_t key = ~0ull;
-    int32_t idx = 0;
-    for (int b = threadIdx.x; b < d.cfl_blocks; b += blockDim.x) {
-        uint64_t k2 = d.cfl_key[(size_t)b * S + s];
-        if (k2 < key) {
-            key = k2;
-            idx = d.cfl_idx[(size_t)b * S + s];
+    int32_t idx = -1;  // the partial holding `key`
+    for (int b0 = threadIdx.x; b0 < d.cfl_blocks; b0 += 4 * NHF_BLOCK) {
+        uint64_t k[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = b0 + u * NHF_BLOCK;
+            k[u] = b < d.cfl_blocks ? d.cfl_key[(size_t)b * S + s] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k[u] < key) {
+                key = k[u];
+                idx = b0 + u * NHF_BLOCK;
+            }
         }
     }
     wave_min_key(key, idx);
@@ -1128,7 +1138,7 @@ __global__ __launch_bounds__(NHF_BLOCK) void kb_finish(BDev d, int mode, int ste
             c->cfl_arg = 0;
         } else {
             c->ndt = __uint_as_float((uint32_t)(key >> 32));
-            c->cfl_arg = idx;
+            c->cfl_arg = d.cfl_idx[(size_t)idx * S + s];
         }
         if (mode == 1) {
             commit_slot(c, step);

@@ -428,10 +428,12 @@ static __device__ __forceinline__ float bflux_value(const BDev &d, int s, int sh
 // slots, they cost little), so a wet face waits for no third round (501 against 527 us). The
 // hanging-edge index stays in the wet branch: loaded with the records it measured slower (482
 // against 455 us on the 64-slot benchmark batch).
+// Workgroups take the sides in launch order, as k_flux does (402 against 446 us per launch on
+// the 64-slot benchmark batch with the XCD-contiguous order).
 template <bool W, int K>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_flux(BDev d) {
     const int sh = W ? 6 : d.shift;
-    size_t t = (size_t)xcd_block() * NHF_BLOCK + threadIdx.x;
+    size_t t = (size_t)blockIdx.x * NHF_BLOCK + threadIdx.x;
     const int first = entity_of<W>(t, sh) * K + 1;
     const int s = (int)(t & ((1u << sh) - 1));
     if (first >= d.m.s_num) return;

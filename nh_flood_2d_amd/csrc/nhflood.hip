@@ -494,10 +494,13 @@ static __device__ __forceinline__ void donor_check(const Dev &d, const FluxSide 
 }
 
 // A workgroup takes NHF_BLOCK * SIDES consecutive sides; thread t the sides t, t + NHF_BLOCK, ...
+// Workgroups take the side ranges in launch order, not the XCD-contiguous order of the cell
+// passes: the flux pass streams its sides, and one front over the array across all XCDs ran
+// faster (17.9 against 18.7 us on the benchmark mesh; the 64-slot batch's flux 402 against 446 us).
 template <int SIDES, bool PARTS>
 __global__ __launch_bounds__(NHF_BLOCK) void k_flux(Dev d) {
     const nhf_mesh &m = d.m;
-    const int block_first = xcd_block() * NHF_BLOCK * SIDES + 1;
+    const int block_first = (int)blockIdx.x * NHF_BLOCK * SIDES + 1;
     const Ctl *c = d.ctl;
     int end = m.s_num, part = 0, begin = 1;
     if (PARTS) {

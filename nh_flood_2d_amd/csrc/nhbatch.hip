@@ -1220,8 +1220,9 @@ static size_t align_b(size_t v) { return (v + 255) & ~(size_t)255; }
 
 static int b_blocks(size_t n) { return (int)((n + NHF_BLOCK - 1) / NHF_BLOCK); }
 
-// Sides per wavefront of the 64-slot flux pass (measured on the 1 M-cell batch mesh: flux 821 /
-// 609 / 533 / 579 us for 1 / 2 / 4 / 8 sides) and flagged cells per load round of its donor pass.
+// Sides per wavefront of the 64-slot flux pass (measured on the 1 M-cell batch mesh in launch
+// order: 434 / 401 / 472 us for 2 / 4 / 8 sides) and flagged cells per load round of its donor
+// pass.
 #ifndef NHB_FLUX_SIDES
 #define NHB_FLUX_SIDES 4
 #endif

@@ -436,8 +436,11 @@ __global__ __launch_bounds__(NHF_BLOCK) void k_face_avg(Dev d) {
 // Sides per thread of the flux pass: each thread issues the loads of both its sides before it
 // uses either (the pass is bound by memory latency). Measured on a 3.2 M-side mesh: 1 side
 // 11.8 us, 2 sides 256 apart 11.4 us, 4 sides 12.3 us, 2 adjacent sides (one 16-byte load of
-// their cells) 12.0 us.
+// their cells) 12.0 us; on the 3.17 M-side benchmark state in launch order 19.1 / 18.5 / 19.8 us
+// for 1 / 2 / 4 sides.
+#ifndef NHF_FLUX_SIDES
 #define NHF_FLUX_SIDES 2
+#endif
 
 struct FluxSide {
     int2 lh;

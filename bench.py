@@ -194,6 +194,34 @@ def pmc_summary(fetch_csv: str, write_csv: str, launches: dict | None = None) ->
     return '\n'.join(lines) + '\n'
 
 
+def l2_summary(counter_csv: str, launches: dict | None = None) -> str:
+    """Per-kernel L2 hits and misses per dispatch from a rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum
+    run of this script, over the same dispatches as pmc_summary (single-mesh kernels before the
+    first batch kernel; each kernel's last `launches` dispatches: the per-kernel timing pass)."""
+    import csv
+    from collections import defaultdict
+    rows = sorted(csv.DictReader(open(counter_csv)), key=lambda r: int(r['Dispatch_Id']))
+    first_batch = min((int(r['Dispatch_Id']) for r in rows if kernel_base_name(r['Kernel_Name']).startswith('kb_')),
+                      default=None)
+    per = defaultdict(lambda: defaultdict(dict))
+    for row in rows:
+        name, i = kernel_base_name(row['Kernel_Name']), int(row['Dispatch_Id'])
+        if name.startswith('k_') and first_batch is not None and i > first_batch:
+            continue
+        if name in PMC_KERNELS:
+            per[PMC_KERNELS[name]][i][row['Counter_Name']] = float(row['Counter_Value'])
+    lines = []
+    for k in sorted(per, key=list(PMC_KERNELS.values()).index):
+        ids = sorted(per[k])
+        n = (launches or {}).get(k)
+        ids = ids[-n:] if n else ids
+        hit = sum(per[k][i].get('TCC_HIT_sum', 0.) for i in ids) / len(ids)
+        miss = sum(per[k][i].get('TCC_MISS_sum', 0.) for i in ids) / len(ids)
+        rate = hit / (hit + miss) if hit + miss else 0.
+        lines.append(f'kernel {k} dispatches {len(ids)} l2_hits {hit:.0f} l2_misses {miss:.0f} hit_rate {rate:.3f}')
+    return '\n'.join(lines) + '\n'
+
+
 def trace_summary(trace_csv: str, bench_json: str) -> str:
     """Per-kernel durations from a rocprofv3 kernel trace of this script: every dispatch, and the
     dispatches of the per-kernel HIP-event pass of the single-mesh section (the figures the JSON
@@ -897,11 +925,21 @@ def main():
     ap.add_argument('--serve-max-h', type=float, default=6.0)
     ap.add_argument('--trace-summary', nargs=2, metavar=('KERNEL_TRACE_CSV', 'BENCH_JSON'),
                     help='summarise a rocprofv3 --kernel-trace run of this script (no GPU needed) and exit')
+    ap.add_argument('--l2-summary', nargs=2, metavar=('COUNTER_CSV', 'BENCH_JSON'),
+                    help='summarise a rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum run of this script (no GPU needed) and exit')
     ap.add_argument('--pmc-summary', nargs=3, metavar=('FETCH_CSV', 'WRITE_CSV', 'BENCH_JSON'),
                     help='summarise two rocprofv3 --pmc runs of this script (no GPU needed) and exit')
     args = ap.parse_args()
     if args.trace_summary:
         print(trace_summary(*args.trace_summary), end='')
+        return
+    if args.l2_summary:
+        counter_csv, bench_json = args.l2_summary
+        with open(bench_json) as f:
+            res = json.loads([ln for ln in f if ln.startswith('{')][-1])
+        launches = {k: v['launches'] for k, v in res.get('kernels', {}).items()}
+        launches.update({f'batch_{k}': v['launches'] for k, v in res.get('batched', {}).get('kernels', {}).items()})
+        print(l2_summary(counter_csv, launches), end='')
         return
     if args.pmc_summary:
         fetch_csv, write_csv, bench_json = args.pmc_summary

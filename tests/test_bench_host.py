@@ -41,6 +41,23 @@ def test_pmc_summary_scales_counters_by_the_copy_calibration(tmp_path):
     assert f'hbm_bytes {2 * 500 * 1024 + 50 * 1024}' in depth
 
 
+def test_l2_summary_averages_the_timing_pass_dispatches(tmp_path):
+    """L2 hits and misses per dispatch: each kernel over its last `launches` dispatches (the JSON
+    line's kernel counts), single-mesh kernels only before the first batch kernel."""
+    rows = [(1, 'void k_depth<128, 1, false>(Dev)', 10, 30), (2, 'void k_depth<128, 1, false>(Dev)', 20, 20),
+            (3, 'void kb_depth<true, 1>(BDev)', 5, 15), (4, 'void k_depth<128, 1, false>(Dev)', 99, 1)]
+    counters = tmp_path / 'l2.csv'
+    counters.write_text('Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n' + ''.join(
+        f'{i},"{name}",TCC_HIT_sum,{hit}\n{i},"{name}",TCC_MISS_sum,{miss}\n' for i, name, hit, miss in rows))
+    result = tmp_path / 'bench.json'
+    result.write_text(json.dumps({'kernels': {'depth': {'launches': 1}},
+                                  'batched': {'kernels': {'depth': {'launches': 1}}}}) + '\n')
+    run = subprocess.run([sys.executable, str(ROOT / 'bench.py'), '--l2-summary', str(counters), str(result)],
+                         capture_output=True, text=True, timeout=120, check=True)
+    assert run.stdout.splitlines() == ['kernel depth dispatches 1 l2_hits 20 l2_misses 20 hit_rate 0.500',
+                                       'kernel batch_depth dispatches 1 l2_hits 5 l2_misses 15 hit_rate 0.250']
+
+
 def test_fp64_error_reports_both_runs_and_the_one_ulp_sensitivity():
     """The precision object of the bench line, with the F32 oracle standing in for the HIP engine
     (the same passes): over 10 s of a developed 2:1 mesh with tide and river, F32 against FP64

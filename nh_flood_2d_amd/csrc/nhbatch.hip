@@ -614,9 +614,11 @@ static __device__ __forceinline__ void bdonor_finish(const BDev &d, int s, int s
     if (faces) atomicAdd(&c->limited_faces, faces);
 }
 
-// K consecutive cells per thread (per wavefront with 64 slots): both load rounds of every cell
-// are issued before any arithmetic or store. A later cell may read a side before an earlier cell
-// of the same thread scales it, which is one of the orders k_donor's argument already covers.
+// The donor pass of a batch with fewer than 64 slots (kb_donor<false, 1>: one (cell, slot) per
+// thread; 64-slot batches take kb_donor_wide below). With K consecutive cells per thread both
+// load rounds of every cell are issued before any arithmetic or store: a later cell may read a
+// side before an earlier cell of the same thread scales it, which is one of the orders k_donor's
+// argument already covers.
 template <bool W, int K>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_donor(BDev d) {
     const int sh = W ? 6 : d.shift;

@@ -663,7 +663,9 @@ template <int K, int B>
 __global__ __launch_bounds__(NHF_BLOCK) void kb_donor_wide(BDev d) {
     const int sh = 6;
     const int s = (int)(threadIdx.x & 63u);
-    const int base = __builtin_amdgcn_readfirstlane(xcd_block() * NHF_WAVES + (int)(threadIdx.x >> 6)) * B + 1;
+    // one eighth of the grid per XCD (the cell passes' chunked order measured 22.9 against 18.7 us)
+    const int xb = (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+    const int base = __builtin_amdgcn_readfirstlane(xb * NHF_WAVES + (int)(threadIdx.x >> 6)) * B + 1;
     const int e_num = d.m.e_num;
     if (base >= e_num) return;
     const int own = base + s;

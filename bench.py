@@ -1058,7 +1058,8 @@ def main():
     # phases of the step: (a) face flux = k_flux; (b) cell update = k_donor + k_depth (the donor
     # limiter needs every face of a cell before any face is scaled, hence two launches). Each
     # phase's rate is taken over the bytes its kernels must move in this layout (model_bytes);
-    # the survey's algorithmic figure (every per-step input read once) is kept beside it.
+    # the algorithmic figure (every per-step input read once, as the reference's passes do) is kept
+    # beside it.
     phases = {'flux': ['flux'], 'cell_update': ['donor', 'depth']}
     # Phase (a) needs the friction and slope coefficients (centre distance, Manning n: 8 B) and
     # the hanging-edge data only on a wet face: a face below min_h carries nothing and reads
@@ -1073,7 +1074,7 @@ def main():
         model = flux_wet_bytes if p == 'flux' else sum(nbytes[k] for k in ks)
         phase[p] = dict(kernels=ks, avg_us=us, algorithmic_bytes=alg[p], model_bytes=model,
                         achieved_gbs=model / (us * 1e-6) / 1e9, hbm_bytes=hbm)
-    # the roofline's unit is the longer of the two phases the survey gives algorithmic bytes for
+    # the roofline's unit is the longer of the two phases with algorithmic bytes of their own
     # ((a) the face flux, (b) the cell update: k_donor + k_depth, two launches because the donor
     # limiter needs every face of a cell before any face is scaled); `achieved` is the phase's
     # algorithmic bytes over the sum of its launches' average durations, beside the rates over
@@ -1110,7 +1111,7 @@ def main():
                       '17 B per side (cell ids 8, flags 1, q read and write 8), 8 B more per wet face (centre '
                       'distance, Manning n), 8 B per cell (bed, depth); for (b) the depth pass\'s launch bytes and '
                       'one donor flag byte per cell (the donor certificate leaves the donor pass only flagged and '
-                      'boundary cells to read); algorithmic_bytes is the survey figure, every input read once as '
+                      'boundary cells to read); algorithmic_bytes is the algorithmic figure, every input read once as '
                       'the reference does; hbm_bytes is what rocprofv3 counters measured',
         'kernels': kernels,
         'step': {'algorithmic_bytes': alg['step'],

@@ -10,7 +10,7 @@ stored volume's, both relative to the budget's volume scale (initial storage plu
 
 The FP64 run takes several minutes of host time; it is a measurement, not a test (pytest does not
 collect this file). Run on a machine with an MI355X and the built library:
-    python tests/precision_report.py --out gpurun_out/precision_c2_hour.md
+    python tests/precision_report.py --out precision_c2_hour.md
 """
 from __future__ import annotations
 

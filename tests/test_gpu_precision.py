@@ -6,7 +6,7 @@ run, relative to the budget's volume scale (initial storage plus every input); t
 Sum h*A is stated separately, because it is the F32 depths themselves summed (their rounding, not
 an accounting error: every budget sum is FP64 on both sides).
 
-Bars: each budget term <= 1e-7 relative (the north star's); the stored volume and the depths to
+Bars: each budget term <= 1e-7 relative (the volume-budget tolerance); the stored volume and the depths to
 the envelopes each test states. F32 and FP64 part where a cell's depth meets the wet/dry gate
 (min_h) in a different step; away from such crossings they differ by rounding only.
 

@@ -953,7 +953,7 @@ def main():
         print(pmc_summary(fetch_csv, write_csv, launches), end='')
         return
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+        print('bench.py runs on one GPU only: there is no multi-GPU path '
               '(independent scenarios would shard whole scenarios per GPU)', file=sys.stderr)
         sys.exit(2)
 

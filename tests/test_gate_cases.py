@@ -5,7 +5,8 @@ Restates the mechanism tests of the reference's ``tests/test_gate_solver.py`` (s
 trigger is strict and needs a wet control cell, commands latch and never reclose, control waits
 for the formal start, the gate bed moves without moving water, exchanges close the pond and
 channel budgets, and a run split at a checkpoint continues bit for bit. The gate-bed timeline
-replay (``read_bed_at``) is not part of this build, so its assertions are left out.
+replay (``read_bed_at``; here ``output.BedTimeline``) is tested with the flood maps
+(``tests/test_maps_host.py``).
 """
 import csv
 import json

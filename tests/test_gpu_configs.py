@@ -30,7 +30,7 @@ def _say(text):
 def test_uneven_32_mesh_union_at_size(gpu_device):
     items = synth.uneven_batch(21, 32, 50_000, 4_000_000, workers=16)
     cells = [sc.mesh.e_num - 1 for sc, _ in items]
-    _say(f'[C4] {len(items)} meshes, {sum(cells)} cells ({min(cells)} to {max(cells)}); alone runs')
+    _say(f'[union of 32 meshes] {len(items)} meshes, {sum(cells)} cells ({min(cells)} to {max(cells)}); alone runs')
 
     def plain(layout, params, state, pipe_q=None):
         e = Engine(layout, params, state, device=gpu_device, pipe_q=pipe_q)
@@ -42,7 +42,7 @@ def test_uneven_32_mesh_union_at_size(gpu_device):
         sim.run()
         alone.append((sim.state(), sim.record['dt_sequence_sha256'], sim.steps))
         sim.engine.close()
-    _say(f'[C4] alone: steps {[a[2] for a in alone]}; the union')
+    _say(f'[union of 32 meshes] alone: steps {[a[2] for a in alone]}; the union')
     sc0 = items[0][0]
     union = UnionEngine([layout for _, layout in items], make_params(sc0.afa, sc0.sita, sc0.min_h, 32),
                         device=gpu_device)
@@ -58,12 +58,12 @@ def test_uneven_32_mesh_union_at_size(gpu_device):
         for name, values in state.items():
             assert got[name].tobytes() == values.tobytes(), f'part {p} {name}'
     union.close()
-    _say(f'[C4] {len(items)} parts match their runs alone bit for bit')
+    _say(f'[union of 32 meshes] {len(items)} parts match their runs alone bit for bit')
 
 
 def test_serving_load_at_size(gpu_device):
     load = synth.serving_load(0, 5)
-    _say(f'[C5] {len(load)} solves, cells {[sc.mesh.e_num - 1 for _, sc, _ in load]}, '
+    _say(f'[serving load] {len(load)} solves, cells {[sc.mesh.e_num - 1 for _, sc, _ in load]}, '
          f'hours {[round(sc.duration / 3600, 2) for _, sc, _ in load]}; alone runs')
     assert max(sc.mesh.e_num - 1 for _, sc, _ in load) > 1_000_000
     assert min(sc.duration for _, sc, _ in load) >= 3600.
@@ -89,7 +89,7 @@ def test_serving_load_at_size(gpu_device):
         sim.run()
         alone[i] = (rows, sim.state(), sim.record['dt_sequence_sha256'], sim.steps)
         sim.engine.close()
-    _say(f'[C5] alone: steps {[alone[i][3] for i in alone]}; served run')
+    _say(f'[serving load] alone: steps {[alone[i][3] for i in alone]}; served run')
     stream = FrameStream(None, every_output=True)
     served_rows, finals = {}, {}
     requests = []
@@ -117,7 +117,7 @@ def test_serving_load_at_size(gpu_device):
     report = FloodServer(batch_size=64).serve(requests, realtime=True)
     stream.close()
     summary = report.summary()
-    _say(f"[C5] served: p50 {summary['p50_latency_s']:.2f} s, p99 {summary['p99_latency_s']:.2f} s, "
+    _say(f"[serving load] served: p50 {summary['p50_latency_s']:.2f} s, p99 {summary['p99_latency_s']:.2f} s, "
          f"{summary['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s, peak live {summary['peak_live']}")
     assert report.peak_live >= 2      # solves overlapped on the device
     for i, (rows, state, sha, steps) in alone.items():

@@ -27,7 +27,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _write_inputs(root: Path, name: str):
-    """C1's mesh, rain raster and JSON configuration under root/name; returns the JSON paths."""
+    """The 256 x 256-cell, 10 m CPU-runnable configuration's mesh, rain raster and JSON
+    configuration under root/name; returns the JSON paths."""
     sc = synth.plain_uniform(0, n=256, dx=10.)
     path = root / name
     cases.write_mesh(path, sc.mesh)
